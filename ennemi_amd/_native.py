@@ -1,0 +1,143 @@
+"""ctypes binding of libennemi_hip.so (declared in include/ennemi_hip.h).
+
+PyTorch-ROCm is used only to own device buffers and name the stream; every
+estimate is computed by the HIP kernels behind this binding.  There is no
+CPU fallback: if the library or a gfx950 device is missing, the first call
+raises ``NativeUnavailable``.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import os
+import threading
+from typing import List, Optional, Sequence, Tuple
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libennemi_hip.so")
+
+KIND_KSG, KIND_CMI, KIND_ROSS, KIND_CROSS, KIND_KL = 0, 1, 2, 3, 4
+MAX_COLS = 16
+
+F_FP64 = 0x1
+F_BRUTE = 0x2
+F_GRAPH = 0x4
+F_PER_INSTANCE = 0x8
+
+EXPORTED_SYMBOLS = ("enm_init", "enm_shutdown", "enm_last_error", "enm_run_batch",
+                    "enm_get_stats", "enm_autotune", "enm_set_tuning", "enm_get_tuning")
+
+
+class NativeUnavailable(RuntimeError):
+    """The HIP library or a gfx950 GPU is not available."""
+
+
+class EnmInstance(ctypes.Structure):
+    _fields_ = [("kind", ctypes.c_int32), ("k", ctypes.c_int32), ("n", ctypes.c_int64),
+                ("dx", ctypes.c_int32), ("dy", ctypes.c_int32), ("dz", ctypes.c_int32),
+                ("n_segs", ctypes.c_int32), ("seg_index", ctypes.c_int64),
+                ("col", ctypes.c_int64 * MAX_COLS)]
+
+
+class EnmStats(ctypes.Structure):
+    _fields_ = [("knn_pairs", ctypes.c_uint64), ("knn_rechecks", ctypes.c_uint64),
+                ("count_pairs", ctypes.c_uint64), ("count_rechecks", ctypes.c_uint64),
+                ("count_queries_1d", ctypes.c_uint64), ("sort_keys", ctypes.c_uint64),
+                ("n_launches", ctypes.c_uint32), ("graph_replayed", ctypes.c_uint32)]
+
+    def as_dict(self) -> dict:
+        return {name: int(getattr(self, name)) for name, _ in self._fields_}
+
+
+def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
+    """Load the shared library and declare its C signatures (no GPU needed)."""
+    if not os.path.exists(path):
+        raise NativeUnavailable(
+            f"{path} is missing; build it with `python -m ennemi_amd.build` "
+            "(ennemi_amd has no CPU implementation)")
+    lib = ctypes.CDLL(path)
+    vp = ctypes.c_void_p
+    lib.enm_init.argtypes = [ctypes.c_int, ctypes.POINTER(vp)]
+    lib.enm_init.restype = ctypes.c_int
+    lib.enm_shutdown.argtypes = [vp]
+    lib.enm_shutdown.restype = None
+    lib.enm_last_error.argtypes = [vp]
+    lib.enm_last_error.restype = ctypes.c_char_p
+    lib.enm_run_batch.argtypes = [vp, ctypes.POINTER(EnmInstance), ctypes.c_int32,
+                                  ctypes.POINTER(ctypes.c_int64), vp, vp, vp, vp,
+                                  ctypes.c_uint32, vp]
+    lib.enm_run_batch.restype = ctypes.c_int
+    lib.enm_get_stats.argtypes = [vp, ctypes.POINTER(EnmStats)]
+    lib.enm_get_stats.restype = ctypes.c_int
+    lib.enm_autotune.argtypes = [vp, ctypes.POINTER(EnmInstance), ctypes.c_int32,
+                                 ctypes.POINTER(ctypes.c_int64), vp, vp, ctypes.c_uint32, vp]
+    lib.enm_autotune.restype = ctypes.c_int
+    lib.enm_set_tuning.argtypes = [vp, ctypes.c_int32, ctypes.c_int32]
+    lib.enm_set_tuning.restype = ctypes.c_int
+    lib.enm_get_tuning.argtypes = [vp, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32)]
+    lib.enm_get_tuning.restype = ctypes.c_int
+    return lib
+
+
+class Context:
+    """One library context on cuda:0 (a process-wide singleton by default)."""
+
+    def __init__(self, device: int = 0):
+        import torch
+        if not torch.cuda.is_available():
+            raise NativeUnavailable("ennemi_amd needs a ROCm GPU (torch.cuda.is_available() is False)")
+        self.lib = load_library()
+        self.device = device
+        torch.cuda.init()
+        with torch.cuda.device(device):
+            ptr = ctypes.c_void_p()
+            rc = self.lib.enm_init(device, ctypes.byref(ptr))
+            if rc != 0:
+                raise NativeUnavailable("enm_init failed: " + self.lib.enm_last_error(None).decode())
+        self.ctx = ptr
+        self.lock = threading.Lock()
+
+    def error(self) -> str:
+        return self.lib.enm_last_error(self.ctx).decode()
+
+    def check(self, rc: int, what: str) -> None:
+        if rc != 0:
+            raise RuntimeError(f"{what} failed ({rc}): {self.error()}")
+
+    def run(self, inst_arr, n_inst: int, seg_arr, arena_ptr: int, out_ptr: int,
+            eps_ptr: int = 0, cnt_ptr: int = 0, flags: int = 0, stream_ptr: int = 0) -> None:
+        rc = self.lib.enm_run_batch(self.ctx, inst_arr, n_inst, seg_arr, arena_ptr, out_ptr,
+                                    eps_ptr or None, cnt_ptr or None, flags, stream_ptr or None)
+        self.check(rc, "enm_run_batch")
+
+    def stats(self) -> dict:
+        st = EnmStats()
+        self.check(self.lib.enm_get_stats(self.ctx, ctypes.byref(st)), "enm_get_stats")
+        return st.as_dict()
+
+    def set_tuning(self, queries_per_lane: int, waves_per_cu: int) -> None:
+        self.check(self.lib.enm_set_tuning(self.ctx, queries_per_lane, waves_per_cu), "enm_set_tuning")
+
+    def get_tuning(self) -> Tuple[int, int]:
+        q = ctypes.c_int32()
+        w = ctypes.c_int32()
+        self.check(self.lib.enm_get_tuning(self.ctx, ctypes.byref(q), ctypes.byref(w)), "enm_get_tuning")
+        return q.value, w.value
+
+    def autotune(self, inst_arr, n_inst, seg_arr, arena_ptr, out_ptr, flags=0, stream_ptr=0) -> None:
+        self.check(self.lib.enm_autotune(self.ctx, inst_arr, n_inst, seg_arr, arena_ptr, out_ptr,
+                                         flags, stream_ptr or None), "enm_autotune")
+
+
+_context: Optional[Context] = None
+_context_lock = threading.Lock()
+
+
+def context() -> Context:
+    global _context
+    with _context_lock:
+        if _context is None:
+            _context = Context(0)
+        return _context

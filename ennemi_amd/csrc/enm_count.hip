@@ -1,0 +1,285 @@
+// Inclusive max-norm ball counts, replacing
+// cKDTree.query_ball_point(pts, eps - 1e-12, p=np.inf, return_length=True)
+// of the reference (ennemi/_entropy_estimators.py:109-110,152-154,196,243-245):
+//   n_i = #{ j in scope : max_d fl64(|a_d[i] - a_d[j]|) <= fl64(eps_i - 1e-12) },
+// which is 0 when the shifted radius is negative.  Queries and candidates are
+// the same key-sorted order, so a query's own position is known.
+//
+// 1-D spaces (the marginals of KSG, Ross and a 1-D condition): the ball is a
+// contiguous run of the sorted key column around the query, because
+// fl64(v_j - v_i) is monotone in v_j on each side.  Each lane gallops out from
+// its own position (steps 1, 2, 4, ...) and binary-searches the boundary with
+// the same FP64 predicate the reference evaluates, so counts are exact.
+// Algorithmic traffic: 8 B radius + 4 B count per query plus the probes.
+//
+// Multi-D spaces: a wave takes 64 consecutive queries, bounds the union of
+// their balls on the key axis, binary-searches that window in the sorted key
+// column (wave-uniform), and scans it with wave-uniform candidate loads.  In
+// FP32-filter mode each lane counts candidates whose FP32 distance is <= the
+// radius rounded down minus the guard (certainly inside) and <= the radius
+// rounded up plus the guard (possibly inside); if any lane saw a candidate in
+// between, the wave rescans its window and settles those in FP64.
+//
+// Each item also writes the sum over its 64 queries of psi(n_i) and a flag
+// for n_i == 0 (psi(0) = +inf in the reference, :338-339).
+#include "enm_device.h"
+#include "enm_launch.h"
+
+namespace {
+
+constexpr int CH = 8;
+
+struct CountArgs {
+    const Item* items;
+    int n_items;
+    unsigned* counter;
+    const CountDev* descs;
+    const OrderDev* orders;
+    const unsigned long long* maxabs;
+    const int* perm;
+    int brute;
+};
+
+__device__ __forceinline__ double query_radius(const CountDev& cd, const CountArgs& A,
+                                               const double* __restrict__ res64, int i) {
+    const double eps = cd.same_order ? res64[cd.eps + i] : res64[cd.eps + A.perm[cd.eps_perm + i]];
+    return eps - 1e-12;
+}
+
+__device__ __forceinline__ void finish_item(const CountDev& cd, const Item& it, bool active,
+                                            int cnt, int* __restrict__ res32,
+                                            double* __restrict__ res64, int i) {
+    if (active) res32[cd.counts + i] = cnt;
+    const double p = (active && cnt > 0) ? enm_psi((double)cnt) : 0.0;
+    const double s = wave_sum(p);
+    const int z = wave_or(active && cnt == 0);
+    if (threadIdx.x == 0) {
+        res64[cd.part + it.tile] = s;
+        res32[cd.zero + it.tile] = z;
+    }
+}
+
+__global__ __launch_bounds__(ENM_WAVE)
+void count1d_kernel(CountArgs A, const double* __restrict__ col64, double* __restrict__ res64,
+                    int* __restrict__ res32, unsigned long long* __restrict__ stats) {
+    for (;;) {
+        const unsigned idx = pop_ticket(A.counter);
+        if (idx >= (unsigned)A.n_items) break;
+        const Item it = A.items[idx];
+        const CountDev cd = A.descs[uni(it.desc)];
+        const OrderDev od = A.orders[cd.order];
+        const double* v = col64 + od.c64;  // sorted key column
+        const int i = it.t0 + (int)threadIdx.x;
+        const bool active = i < it.t1;
+        int cnt = 0;
+        if (active) {
+            const double r = query_radius(cd, A, res64, i);
+            if (!(r < 0.0)) {
+                const double x = v[i];
+                // Right boundary: last j in [i, hi) with fl(v[j] - x) <= r.
+                int good = i, step = 1, bad = it.hi;
+                while (good + step < it.hi) {
+                    if (v[good + step] - x <= r) { good += step; step <<= 1; }
+                    else { bad = good + step; break; }
+                }
+                while (bad - good > 1) {
+                    const int mid = good + ((bad - good) >> 1);
+                    if (v[mid] - x <= r) good = mid; else bad = mid;
+                }
+                const int right = good;
+                // Left boundary: first j in [lo, i] with fl(x - v[j]) <= r.
+                good = i; step = 1; bad = it.lo - 1;
+                while (good - step >= it.lo) {
+                    if (x - v[good - step] <= r) { good -= step; step <<= 1; }
+                    else { bad = good - step; break; }
+                }
+                while (good - bad > 1) {
+                    const int mid = good - ((good - bad) >> 1);
+                    if (x - v[mid] <= r) good = mid; else bad = mid;
+                }
+                cnt = right - good + 1;
+            }
+        }
+        finish_item(cd, it, active, cnt, res32, res64, i);
+        if (threadIdx.x == 0) atomicAdd(stats + ST_CNT_1D, (unsigned long long)(it.t1 - it.t0));
+    }
+}
+
+template <int D, bool F32>
+__device__ void countnd_item(const CountArgs& A, const Item& it, const CountDev& cd,
+                             const double* __restrict__ col64, const float* __restrict__ col32,
+                             double* __restrict__ res64, int* __restrict__ res32,
+                             unsigned long long* __restrict__ stats) {
+    const OrderDev od = A.orders[cd.order];
+    const int64_t n = od.n;
+    const int i = it.t0 + (int)threadIdx.x;
+    const bool active = i < it.t1;
+    bool f32ok = true;
+    const double g = enm_guard(A.maxabs[od.inst], &f32ok);
+
+    const double* c64[D];
+    const float* c32[D];
+    double q64[D];
+    float q32[D];
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+        const int c = d < cd.ncol ? cd.cols[d] : cd.cols[0];
+        c64[d] = col64 + od.c64 + c * n;
+        c32[d] = col32 + od.c32 + c * n;
+        q64[d] = active ? c64[d][i] : 0.0;
+        q32[d] = (float)q64[d];
+    }
+    double r = active ? query_radius(cd, A, res64, i) : -1.0;
+    const bool live = active && !(r < 0.0);
+    if (!live) r = -INFINITY;
+
+    // Window on the key axis (a superset of every live lane's ball).
+    int w0 = it.lo, w1 = it.hi;
+    if (!A.brute) {
+        const double key = q64[0];
+        const double slack = 0x1p-50 * (fabs(key) + fabs(r));
+        double lb = live ? (key - r * (1.0 + 0x1p-50)) - slack : INFINITY;
+        double ub = live ? (key + r * (1.0 + 0x1p-50)) + slack : -INFINITY;
+        lb = wave_min(lb);
+        ub = wave_max(ub);
+        if (lb > ub) { w0 = w1 = it.t0; }
+        else {
+            w0 = uni(lower_bound_u(c64[0], it.lo, it.hi, lb));
+            w1 = uni(upper_bound_u(c64[0], w0, it.hi, ub));
+        }
+    }
+
+    int cnt = 0;
+    if (F32) {
+        const float lo32 = live ? f32_down(r - g) : -INFINITY;
+        const float hi32 = live ? f32_up(r + g) : -INFINITY;
+        int chi = 0;
+        int j = w0;
+        for (; j + CH <= w1; j += CH) {
+#pragma unroll
+            for (int c = 0; c < CH; ++c) {
+                float m = fabsf(c32[0][j + c] - q32[0]);
+#pragma unroll
+                for (int d = 1; d < D; ++d) m = fmaxf(m, fabsf(c32[d][j + c] - q32[d]));
+                cnt += (m <= lo32);
+                chi += (m <= hi32);
+            }
+        }
+        for (; j < w1; ++j) {
+            float m = fabsf(c32[0][j] - q32[0]);
+#pragma unroll
+            for (int d = 1; d < D; ++d) m = fmaxf(m, fabsf(c32[d][j] - q32[d]));
+            cnt += (m <= lo32);
+            chi += (m <= hi32);
+        }
+        unsigned rc = 0;
+        if (wave_any(chi != cnt)) {
+            const bool band = chi != cnt;
+            for (j = w0; j < w1; ++j) {
+                float m = fabsf(c32[0][j] - q32[0]);
+#pragma unroll
+                for (int d = 1; d < D; ++d) m = fmaxf(m, fabsf(c32[d][j] - q32[d]));
+                if (band && m > lo32 && m <= hi32) {
+                    double e = fabs(c64[0][j] - q64[0]);
+#pragma unroll
+                    for (int d = 1; d < D; ++d) e = fmax(e, fabs(c64[d][j] - q64[d]));
+                    cnt += (e <= r);
+                    ++rc;
+                }
+            }
+        }
+        const unsigned long long rcs = wave_sum_u64(rc);
+        if (threadIdx.x == 0) atomicAdd(stats + ST_CNT_RECHECK, rcs);
+    } else {
+        int j = w0;
+        for (; j + CH <= w1; j += CH) {
+#pragma unroll
+            for (int c = 0; c < CH; ++c) {
+                double m = fabs(c64[0][j + c] - q64[0]);
+#pragma unroll
+                for (int d = 1; d < D; ++d) m = fmax(m, fabs(c64[d][j + c] - q64[d]));
+                cnt += (m <= r);
+            }
+        }
+        for (; j < w1; ++j) {
+            double m = fabs(c64[0][j] - q64[0]);
+#pragma unroll
+            for (int d = 1; d < D; ++d) m = fmax(m, fabs(c64[d][j] - q64[d]));
+            cnt += (m <= r);
+        }
+    }
+    finish_item(cd, it, active, cnt, res32, res64, i);
+    if (threadIdx.x == 0)
+        atomicAdd(stats + ST_CNT_PAIRS, (unsigned long long)(it.t1 - it.t0) * (unsigned long long)(w1 - w0));
+}
+
+template <int D, bool F32>
+__global__ __launch_bounds__(ENM_WAVE)
+void countnd_kernel(CountArgs A, const double* __restrict__ col64, const float* __restrict__ col32,
+                    double* __restrict__ res64, int* __restrict__ res32,
+                    unsigned long long* __restrict__ stats) {
+    for (;;) {
+        const unsigned idx = pop_ticket(A.counter);
+        if (idx >= (unsigned)A.n_items) break;
+        Item it = A.items[idx];
+        it.desc = uni(it.desc); it.tile = uni(it.tile);
+        it.lo = uni(it.lo); it.hi = uni(it.hi); it.t0 = uni(it.t0); it.t1 = uni(it.t1);
+        const CountDev cd = A.descs[it.desc];
+        if (F32) {
+            bool ok = true;
+            (void)enm_guard(A.maxabs[A.orders[cd.order].inst], &ok);
+            if (ok) countnd_item<D, true>(A, it, cd, col64, col32, res64, res32, stats);
+            else    countnd_item<D, false>(A, it, cd, col64, col32, res64, res32, stats);
+        } else {
+            countnd_item<D, false>(A, it, cd, col64, col32, res64, res32, stats);
+        }
+    }
+}
+
+template <int D>
+hipError_t launch_nd(const CountArgs& a, bool f32, int grid, const double* col64, const float* col32,
+                     double* res64, int* res32, unsigned long long* stats, hipStream_t st) {
+    if (f32)
+        hipLaunchKernelGGL((countnd_kernel<D, true>), dim3(grid), dim3(ENM_WAVE), 0, st,
+                           a, col64, col32, res64, res32, stats);
+    else
+        hipLaunchKernelGGL((countnd_kernel<D, false>), dim3(grid), dim3(ENM_WAVE), 0, st,
+                           a, col64, col32, res64, res32, stats);
+    return hipGetLastError();
+}
+
+}  // namespace
+
+int enm_count_dim_class(int d) {
+    if (d <= 6) return d;
+    if (d <= 8) return 8;
+    if (d <= 12) return 12;
+    if (d <= 16) return 16;
+    return -1;
+}
+
+hipError_t enm_launch_count(const Item* items, int n_items, unsigned* counter,
+                            const CountDev* descs, const OrderDev* orders,
+                            const unsigned long long* maxabs, const int* perm,
+                            const double* col64, const float* col32, double* res64, int* res32,
+                            unsigned long long* stats, int dclass, bool f32, bool brute,
+                            int grid, hipStream_t st) {
+    if (n_items <= 0) return hipSuccess;
+    CountArgs a{items, n_items, counter, descs, orders, maxabs, perm, brute ? 1 : 0};
+    if (dclass == 1 && !brute) {
+        hipLaunchKernelGGL(count1d_kernel, dim3(grid), dim3(ENM_WAVE), 0, st, a, col64, res64, res32, stats);
+        return hipGetLastError();
+    }
+    switch (dclass) {
+        case 1: return launch_nd<1>(a, f32, grid, col64, col32, res64, res32, stats, st);
+        case 2: return launch_nd<2>(a, f32, grid, col64, col32, res64, res32, stats, st);
+        case 3: return launch_nd<3>(a, f32, grid, col64, col32, res64, res32, stats, st);
+        case 4: return launch_nd<4>(a, f32, grid, col64, col32, res64, res32, stats, st);
+        case 5: return launch_nd<5>(a, f32, grid, col64, col32, res64, res32, stats, st);
+        case 6: return launch_nd<6>(a, f32, grid, col64, col32, res64, res32, stats, st);
+        case 8: return launch_nd<8>(a, f32, grid, col64, col32, res64, res32, stats, st);
+        case 12: return launch_nd<12>(a, f32, grid, col64, col32, res64, res32, stats, st);
+        case 16: return launch_nd<16>(a, f32, grid, col64, col32, res64, res32, stats, st);
+        default: return hipErrorInvalidValue;
+    }
+}

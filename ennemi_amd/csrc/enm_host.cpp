@@ -1,0 +1,837 @@
+// C ABI and host planner of libennemi_hip.so.
+//
+// enm_run_batch replaces the task fan-out of the reference
+// (_map_maybe_parallel over _lagged_mi, ennemi/_driver.py:492,715,736-785):
+// every instance of the call is planned into one set of work lists, and one
+// launch sequence on one stream runs them all:
+//
+//   fill     zero the counters, maxima and statistics (kernels, so the
+//            sequence can be captured in a hipGraph)
+//   sort     segmented sort of each distinct key column (block sort + merges)
+//   gather   each order's columns in sorted order, FP64 + FP32, magnitudes
+//   knn      persistent work-queue kernels, one per (dimension, list depth)
+//   count    persistent work-queue kernels, 1-D and per dimension
+//   combine  one wave per instance: psi sums -> estimate
+//   export   optional per-point radii and counts
+//
+// Plans are cached by the bytes of the call (instances, class sizes, device
+// pointers, flags, tuning).  With ENM_F_GRAPH the launch sequence of a plan is
+// captured once into a hipGraph and replayed on later identical calls.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <list>
+#include <map>
+#include <memory>
+#include <string>
+#include <tuple>
+#include <vector>
+
+#include "enm_internal.h"
+#include "enm_launch.h"
+#include "../../include/ennemi_hip.h"
+
+namespace {
+
+std::string g_init_error;
+
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    ~DevBuf() { if (p) (void)hipFree(p); }
+    hipError_t ensure(size_t b) {
+        if (b <= bytes && p) return hipSuccess;
+        if (p) { (void)hipFree(p); p = nullptr; bytes = 0; }
+        if (b == 0) b = 256;
+        hipError_t e = hipMalloc(&p, b);
+        if (e == hipSuccess) bytes = b;
+        return e;
+    }
+    template <class T> T* as() const { return reinterpret_cast<T*>(p); }
+};
+
+struct Group {
+    int is_count;
+    int dclass;
+    int pclass;
+    int first;      // first item in the kernel's item list
+    int n_items;
+    int counter;    // slot in the counter array
+    int inst;       // instance (per-instance mode) or -1
+};
+
+struct ExportSpace {
+    int64_t counts;   // res32 offset
+    int64_t perm;     // perm offset of the order
+};
+
+struct ExportInst {
+    int64_t point_off;
+    int64_t n;
+    int64_t eps;      // res64 offset (order A positions)
+    int64_t permA;    // perm offset of order A
+    int nspace;
+    ExportSpace sp[ENM_MAX_SPACES];
+};
+
+struct Plan {
+    std::string key;
+    // host tables
+    std::vector<SortSeg> segs;
+    std::vector<GatherItem> gitems;
+    std::vector<OrderDev> orders;
+    std::vector<KnnDev> kdescs;
+    std::vector<CountDev> cdescs;
+    std::vector<InstDev> insts;
+    std::vector<double> cls;
+    std::vector<Item> kitems, citems;
+    std::vector<Group> groups;
+    std::vector<ExportInst> exports;
+    int ntiles = 0;
+    int64_t maxlen = 0;
+    int64_t sort_elems = 0;
+    int64_t n_col64 = 0, n_col32 = 0, n_perm = 0, n_res64 = 0, n_res32 = 0;
+    int n_inst = 0;
+    bool f32 = true, brute = false, per_instance = false;
+    // device
+    DevBuf tables, sk0, si0, sk1, si1, col64, col32, perm, res64, res32, maxabs, counters, stats;
+    size_t off_segs = 0, off_gitems = 0, off_orders = 0, off_kdescs = 0, off_cdescs = 0, off_insts = 0,
+           off_cls = 0, off_kitems = 0, off_citems = 0, table_bytes = 0;
+    void* pinned = nullptr;
+    hipEvent_t upload_done = nullptr;
+    hipGraphExec_t graph = nullptr;
+    int n_launch = 0;
+    ~Plan() {
+        if (graph) (void)hipGraphExecDestroy(graph);
+        if (upload_done) (void)hipEventDestroy(upload_done);
+        if (pinned) (void)hipHostFree(pinned);
+    }
+};
+
+}  // namespace
+
+struct enm_ctx {
+    int device = 0;
+    int num_cus = 256;
+    std::string err;
+    int q_per_lane = 1;
+    int waves_per_cu = 16;
+    std::list<std::unique_ptr<Plan>> cache;   // most recent first
+    Plan* last = nullptr;
+    hipStream_t last_stream = nullptr;
+};
+
+namespace {
+
+const size_t kMaxCachedPlans = 4;
+
+int fail(enm_ctx* ctx, int code, const std::string& msg) {
+    ctx->err = msg;
+    return code;
+}
+
+int hip_fail(enm_ctx* ctx, hipError_t e, const char* what) {
+    return fail(ctx, ENM_E_HIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+// ---------------------------------------------------------------------------
+// Planning
+// ---------------------------------------------------------------------------
+
+struct Planner {
+    Plan& P;
+    const enm_instance* inst;
+    const int64_t* seg_len;
+    std::map<std::tuple<int64_t, int64_t, int>, int> sort_groups;  // (key src, n, segmented) -> group
+    std::vector<std::vector<int>> group_segs;                      // group -> sort segment ids
+    std::vector<std::vector<int64_t>> group_pos0;                  // group -> order position of each segment
+    std::map<std::string, int> order_ids;
+    std::vector<int> order_group;
+
+    explicit Planner(Plan& p, const enm_instance* in, const int64_t* sl) : P(p), inst(in), seg_len(sl) {}
+
+    int sort_group(int ii, int64_t key_src, bool segmented) {
+        const enm_instance& I = inst[ii];
+        const int nseg = segmented ? I.n_segs : 1;
+        // Class segments make the key for distinct class layouts differ.
+        auto key = std::make_tuple(key_src, (int64_t)I.n, segmented ? (int)(I.seg_index * 64 + nseg) : -1);
+        auto it = sort_groups.find(key);
+        if (it != sort_groups.end()) return it->second;
+        const int g = (int)group_segs.size();
+        sort_groups[key] = g;
+        group_segs.emplace_back();
+        group_pos0.emplace_back();
+        int64_t p0 = 0;
+        for (int s = 0; s < nseg; ++s) {
+            const int64_t len = segmented ? seg_len[I.seg_index + s] : I.n;
+            if (len > 0) {
+                SortSeg sg;
+                sg.src = key_src + p0;
+                sg.sbase = P.sort_elems;
+                sg.len = (int32_t)len;
+                sg.first_tile = P.ntiles;
+                sg.p0 = (int32_t)p0;
+                sg.pad = 0;
+                P.ntiles += (int)((len + ENM_SORT_TILE - 1) / ENM_SORT_TILE);
+                P.sort_elems += len;
+                P.maxlen = std::max(P.maxlen, len);
+                group_segs[g].push_back((int)P.segs.size());
+                group_pos0[g].push_back(p0);
+                P.segs.push_back(sg);
+            }
+            p0 += len;
+        }
+        return g;
+    }
+
+    // An order: the instance's points sorted by cols_src[0] within each scope
+    // range, with the columns cols_src gathered.
+    int order(int ii, const std::vector<int64_t>& cols_src, bool segmented) {
+        const enm_instance& I = inst[ii];
+        const int g = sort_group(ii, cols_src[0], segmented);
+        std::string key((const char*)cols_src.data(), cols_src.size() * sizeof(int64_t));
+        key += std::string((const char*)&g, sizeof(g));
+        auto it = order_ids.find(key);
+        if (it != order_ids.end()) return it->second;
+        OrderDev od;
+        std::memset(&od, 0, sizeof(od));
+        od.n = I.n;
+        od.ncol = (int32_t)cols_src.size();
+        od.inst = ii;
+        od.c64 = P.n_col64;
+        od.c32 = P.n_col32;
+        od.perm = P.n_perm;
+        for (size_t c = 0; c < cols_src.size(); ++c) od.src[c] = cols_src[c];
+        P.n_col64 += od.ncol * I.n;
+        P.n_col32 += od.ncol * I.n;
+        P.n_perm += I.n;
+        const int id = (int)P.orders.size();
+        P.orders.push_back(od);
+        order_ids[key] = id;
+        order_group.push_back(g);
+        for (size_t t = 0; t < group_segs[g].size(); ++t) {
+            const int s = group_segs[g][t];
+            for (int off = 0; off < P.segs[s].len; off += ENM_SORT_TILE) {
+                GatherItem gi;
+                gi.order = id;
+                gi.sseg = s;
+                gi.off = off;
+                gi.pos0 = (int32_t)group_pos0[g][t];
+                P.gitems.push_back(gi);
+            }
+        }
+        return id;
+    }
+
+    // Scope ranges [lo, hi) of an order (class segments or all points).
+    std::vector<std::pair<int, int>> ranges(int order_id) {
+        std::vector<std::pair<int, int>> r;
+        const int g = order_group[order_id];
+        for (size_t t = 0; t < group_segs[g].size(); ++t) {
+            const int lo = (int)group_pos0[g][t];
+            r.emplace_back(lo, lo + P.segs[group_segs[g][t]].len);
+        }
+        return r;
+    }
+};
+
+struct PendingItem {
+    Item it;
+    int64_t cost;
+    int inst;
+    int dclass, pclass;
+};
+
+int plan_batch(enm_ctx* ctx, Plan& P, const enm_instance* inst, int n_inst, const int64_t* seg_len) {
+    Planner pl(P, inst, seg_len);
+    std::vector<PendingItem> kpend, cpend;
+    P.n_inst = n_inst;
+    int64_t point_off = 0;
+    for (int ii = 0; ii < n_inst; ++ii) {
+        const enm_instance& I = inst[ii];
+        const int dx = I.dx, dy = I.dy, dz = I.dz;
+        const int ncols = dx + dy + dz;
+        char msg[256];
+        auto bad = [&](const char* what) {
+            std::snprintf(msg, sizeof msg, "instance %d: %s", ii, what);
+            return fail(ctx, ENM_E_ARG, msg);
+        };
+        if (I.kind < ENM_KIND_KSG || I.kind > ENM_KIND_KL) return bad("unknown kind");
+        if (I.k < 1) return bad("k must be >= 1");
+        if (I.n < 1 || I.n > 0x7fffffffLL) return bad("n out of range");
+        if (dx < 0 || dy < 0 || dz < 0 || ncols < 1 || ncols > ENM_MAX_COLS) return bad("bad column counts");
+        const bool classes = I.kind == ENM_KIND_ROSS || I.kind == ENM_KIND_CROSS;
+        if (I.n_segs < 1 || (!classes && I.n_segs != 1)) return bad("bad n_segs");
+        if (classes) {
+            int64_t tot = 0;
+            for (int s = 0; s < I.n_segs; ++s) tot += seg_len[I.seg_index + s];
+            if (tot != I.n) return bad("class sizes do not sum to n");
+        }
+        switch (I.kind) {
+            case ENM_KIND_KSG: if (dx < 1 || dy < 1 || dz != 0) return bad("KSG needs dx, dy >= 1, dz = 0"); break;
+            case ENM_KIND_CMI: if (dx < 1 || dy < 1 || dz < 1) return bad("CMI needs dx, dy, dz >= 1"); break;
+            case ENM_KIND_ROSS: if (dx < 1 || dy != 0 || dz != 0) return bad("ROSS needs dx >= 1 only"); break;
+            case ENM_KIND_CROSS: if (dx < 1 || dy != 0 || dz < 1) return bad("CROSS needs dx, dz >= 1, dy = 0"); break;
+            case ENM_KIND_KL: if (dx < 1 || dy != 0 || dz != 0) return bad("KL needs dx >= 1 only"); break;
+        }
+        std::vector<int64_t> X(I.col, I.col + dx), Y(I.col + dx, I.col + dx + dy), Z(I.col + dx + dy, I.col + ncols);
+
+        InstDev id;
+        std::memset(&id, 0, sizeof(id));
+        id.kind = I.kind;
+        id.k = I.k;
+        id.n = I.n;
+        id.dx = dx;
+        id.n_segs = I.n_segs;
+        id.seg_len = -1;
+        for (int s = 0; s < ENM_MAX_SPACES; ++s) { id.part[s] = 0; id.zero[s] = -1; id.ntile[s] = 0; }
+
+        ExportInst ex;
+        std::memset(&ex, 0, sizeof(ex));
+        ex.point_off = point_off;
+        ex.n = I.n;
+        point_off += I.n;
+
+        // Orders and spaces of this kind (column layouts documented in the header).
+        int A = -1;
+        std::vector<int64_t> acols;
+        bool segA = classes;
+        if (I.kind == ENM_KIND_CMI) {
+            acols.push_back(Z[0]);
+            acols.insert(acols.end(), X.begin(), X.end());
+            acols.insert(acols.end(), Y.begin(), Y.end());
+            acols.insert(acols.end(), Z.begin() + 1, Z.end());
+        } else if (I.kind == ENM_KIND_CROSS) {
+            acols.push_back(Z[0]);
+            acols.insert(acols.end(), X.begin(), X.end());
+            acols.insert(acols.end(), Z.begin() + 1, Z.end());
+        } else {
+            acols = X;
+            acols.insert(acols.end(), Y.begin(), Y.end());
+        }
+        A = pl.order(ii, acols, segA);
+
+        struct SpaceSpec { int order; std::vector<int> cols; bool segmented; };
+        std::vector<SpaceSpec> spaces;
+        auto iota = [](int a, int b) { std::vector<int> v; for (int t = a; t < b; ++t) v.push_back(t); return v; };
+        bool need_orig = false;
+        if (I.kind == ENM_KIND_KSG) {
+            spaces.push_back({A, iota(0, dx), false});
+            const int Yo = pl.order(ii, Y, false);
+            spaces.push_back({Yo, iota(0, dy), false});
+        } else if (I.kind == ENM_KIND_CMI) {
+            std::vector<int> xz{0}, yz{0}, z{0};
+            for (int t = 1; t <= dx; ++t) xz.push_back(t);
+            for (int t = dx + 1; t <= dx + dy; ++t) yz.push_back(t);
+            for (int t = dx + dy + 1; t < ncols; ++t) { xz.push_back(t); yz.push_back(t); z.push_back(t); }
+            spaces.push_back({A, xz, false});
+            spaces.push_back({A, yz, false});
+            spaces.push_back({A, z, false});
+        } else if (I.kind == ENM_KIND_ROSS) {
+            const int B = I.n_segs > 1 ? pl.order(ii, X, false) : A;
+            spaces.push_back({B, iota(0, dx), false});
+        } else if (I.kind == ENM_KIND_CROSS) {
+            const int B = I.n_segs > 1 ? pl.order(ii, acols, false) : A;
+            std::vector<int> z{0};
+            for (int t = dx + 1; t < ncols - dy; ++t) z.push_back(t);
+            spaces.push_back({B, iota(0, (int)acols.size()), false});
+            spaces.push_back({A, z, true});
+            spaces.push_back({B, z, false});
+        }
+        for (auto& s : spaces) if (s.order != A) need_orig = true;
+
+        // kNN descriptor and items.
+        KnnDev kd;
+        std::memset(&kd, 0, sizeof(kd));
+        kd.order = A;
+        kd.inst = ii;
+        kd.k = I.k;
+        kd.kl = I.kind == ENM_KIND_KL;
+        kd.eps = P.n_res64;
+        P.n_res64 += I.n;
+        kd.eps_orig = -1;
+        if (need_orig) { kd.eps_orig = P.n_res64; P.n_res64 += I.n; }
+        kd.part = -1;
+        const int kdi = (int)P.kdescs.size();
+        const int D = (int)acols.size();
+        const int dcl = enm_knn_dim_class(D);
+        const int pcl = enm_knn_list_class(I.k + 1);
+        if (dcl < 0) return bad("joint dimension above 16");
+        if (pcl < 0) return fail(ctx, ENM_E_UNSUPPORTED, "k above 31 is not supported by the register lists");
+        int ktiles = 0;
+        std::vector<PendingItem> mine;
+        for (auto rg : pl.ranges(A)) {
+            for (int t0 = rg.first; t0 < rg.second; t0 += ENM_TILE) {
+                PendingItem pi;
+                pi.it = Item{kdi, ktiles++, rg.first, rg.second, t0, std::min(t0 + ENM_TILE, rg.second)};
+                pi.cost = P.brute ? (int64_t)(rg.second - rg.first) : (int64_t)I.n;
+                pi.inst = ii; pi.dclass = dcl; pi.pclass = pcl;
+                mine.push_back(pi);
+            }
+        }
+        if (kd.kl) {
+            kd.part = P.n_res64;
+            P.n_res64 += ktiles;
+            id.nspace = 1;
+            id.part[0] = kd.part;
+            id.zero[0] = -1;
+            id.ntile[0] = ktiles;
+        }
+        P.kdescs.push_back(kd);
+        kpend.insert(kpend.end(), mine.begin(), mine.end());
+        ex.eps = kd.eps;
+        ex.permA = P.orders[A].perm;
+
+        // Count descriptors and items.
+        if (!kd.kl) id.nspace = (int)spaces.size();
+        for (size_t s = 0; s < spaces.size(); ++s) {
+            const SpaceSpec& sp = spaces[s];
+            CountDev cd;
+            std::memset(&cd, 0, sizeof(cd));
+            cd.order = sp.order;
+            cd.inst = ii;
+            cd.ncol = (int)sp.cols.size();
+            for (int c = 0; c < cd.ncol; ++c) cd.cols[c] = sp.cols[c];
+            cd.same_order = sp.order == A;
+            cd.eps = cd.same_order ? kd.eps : kd.eps_orig;
+            cd.eps_perm = P.orders[sp.order].perm;
+            cd.counts = P.n_res32;
+            P.n_res32 += I.n;
+            const int cdi = (int)P.cdescs.size();
+            const int ccl = enm_count_dim_class(cd.ncol);
+            int ctiles = 0;
+            std::vector<std::pair<int, int>> rgs;
+            if (sp.segmented) rgs = pl.ranges(sp.order);
+            else rgs.emplace_back(0, (int)I.n);
+            for (auto rg : rgs) {
+                for (int t0 = rg.first; t0 < rg.second; t0 += ENM_TILE) {
+                    PendingItem pi;
+                    pi.it = Item{cdi, ctiles++, rg.first, rg.second, t0, std::min(t0 + ENM_TILE, rg.second)};
+                    pi.cost = P.brute ? (int64_t)(rg.second - rg.first) : (int64_t)I.n;
+                    pi.inst = ii; pi.dclass = ccl; pi.pclass = 0;
+                    cpend.push_back(pi);
+                }
+            }
+            cd.part = P.n_res64;
+            P.n_res64 += ctiles;
+            cd.zero = P.n_res32;
+            P.n_res32 += ctiles;
+            id.part[s] = cd.part;
+            id.zero[s] = cd.zero;
+            id.ntile[s] = ctiles;
+            ex.sp[s].counts = cd.counts;
+            ex.sp[s].perm = cd.eps_perm;
+            P.cdescs.push_back(cd);
+        }
+        ex.nspace = (int)spaces.size();
+        if (I.kind == ENM_KIND_ROSS) {
+            id.seg_len = (int64_t)P.cls.size();
+            for (int s = 0; s < I.n_segs; ++s) P.cls.push_back((double)seg_len[I.seg_index + s]);
+        }
+        P.insts.push_back(id);
+        P.exports.push_back(ex);
+    }
+
+    // Work lists: grouped by kernel class, largest first (LPT), or per
+    // instance in per-instance mode.
+    auto build = [&](std::vector<PendingItem>& pend, std::vector<Item>& out, int is_count) {
+        std::stable_sort(pend.begin(), pend.end(), [&](const PendingItem& a, const PendingItem& b) {
+            if (P.per_instance && a.inst != b.inst) return a.inst < b.inst;
+            if (a.dclass != b.dclass) return a.dclass < b.dclass;
+            if (a.pclass != b.pclass) return a.pclass < b.pclass;
+            return a.cost > b.cost;
+        });
+        size_t i = 0;
+        while (i < pend.size()) {
+            size_t j = i;
+            while (j < pend.size() && pend[j].dclass == pend[i].dclass && pend[j].pclass == pend[i].pclass &&
+                   (!P.per_instance || pend[j].inst == pend[i].inst))
+                ++j;
+            Group g;
+            g.is_count = is_count;
+            g.dclass = pend[i].dclass;
+            g.pclass = pend[i].pclass;
+            g.first = (int)out.size();
+            g.n_items = (int)(j - i);
+            g.counter = (int)P.groups.size();
+            g.inst = P.per_instance ? pend[i].inst : -1;
+            for (size_t t = i; t < j; ++t) out.push_back(pend[t].it);
+            P.groups.push_back(g);
+            i = j;
+        }
+    };
+    build(kpend, P.kitems, 0);
+    build(cpend, P.citems, 1);
+    return ENM_OK;
+}
+
+template <class T>
+size_t append(std::vector<char>& blob, const std::vector<T>& v) {
+    size_t off = (blob.size() + 255) & ~(size_t)255;
+    blob.resize(off + v.size() * sizeof(T));
+    if (!v.empty()) std::memcpy(blob.data() + off, v.data(), v.size() * sizeof(T));
+    return off;
+}
+
+int allocate_and_upload(enm_ctx* ctx, Plan& P, hipStream_t st) {
+    std::vector<char> blob;
+    P.off_segs = append(blob, P.segs);
+    P.off_gitems = append(blob, P.gitems);
+    P.off_orders = append(blob, P.orders);
+    P.off_kdescs = append(blob, P.kdescs);
+    P.off_cdescs = append(blob, P.cdescs);
+    P.off_insts = append(blob, P.insts);
+    P.off_cls = append(blob, P.cls);
+    P.off_kitems = append(blob, P.kitems);
+    P.off_citems = append(blob, P.citems);
+    P.table_bytes = blob.size();
+    hipError_t e;
+    if (P.upload_done) {
+        e = hipEventSynchronize(P.upload_done);
+        if (e != hipSuccess) return hip_fail(ctx, e, "hipEventSynchronize");
+    } else {
+        e = hipEventCreateWithFlags(&P.upload_done, hipEventDisableTiming);
+        if (e != hipSuccess) return hip_fail(ctx, e, "hipEventCreate");
+    }
+    if (P.pinned) { (void)hipHostFree(P.pinned); P.pinned = nullptr; }
+    e = hipHostMalloc(&P.pinned, std::max<size_t>(P.table_bytes, 256), hipHostMallocDefault);
+    if (e != hipSuccess) return hip_fail(ctx, e, "hipHostMalloc");
+    std::memcpy(P.pinned, blob.data(), P.table_bytes);
+
+    auto need = [&](DevBuf& b, size_t bytes, const char* what) -> int {
+        hipError_t er = b.ensure(bytes);
+        if (er != hipSuccess) return fail(ctx, ENM_E_NOMEM, std::string("hipMalloc failed for ") + what);
+        return ENM_OK;
+    };
+    int rc;
+    if ((rc = need(P.tables, P.table_bytes, "tables"))) return rc;
+    if ((rc = need(P.sk0, P.sort_elems * 8, "sort keys"))) return rc;
+    if ((rc = need(P.sk1, P.sort_elems * 8, "sort keys"))) return rc;
+    if ((rc = need(P.si0, P.sort_elems * 4, "sort index"))) return rc;
+    if ((rc = need(P.si1, P.sort_elems * 4, "sort index"))) return rc;
+    if ((rc = need(P.col64, P.n_col64 * 8, "columns"))) return rc;
+    if ((rc = need(P.col32, P.n_col32 * 4, "columns"))) return rc;
+    if ((rc = need(P.perm, P.n_perm * 4, "permutations"))) return rc;
+    if ((rc = need(P.res64, P.n_res64 * 8, "radii"))) return rc;
+    if ((rc = need(P.res32, P.n_res32 * 4, "counts"))) return rc;
+    if ((rc = need(P.maxabs, (size_t)P.n_inst * 8, "maxima"))) return rc;
+    if ((rc = need(P.counters, P.groups.size() * 4 + 4, "counters"))) return rc;
+    if ((rc = need(P.stats, ST_N * 8, "stats"))) return rc;
+    e = hipMemcpyAsync(P.tables.p, P.pinned, P.table_bytes, hipMemcpyHostToDevice, st);
+    if (e != hipSuccess) return hip_fail(ctx, e, "hipMemcpyAsync(tables)");
+    e = hipEventRecord(P.upload_done, st);
+    if (e != hipSuccess) return hip_fail(ctx, e, "hipEventRecord");
+    return ENM_OK;
+}
+
+template <class T>
+const T* tab(const Plan& P, size_t off) {
+    return reinterpret_cast<const T*>(P.tables.as<char>() + off);
+}
+
+int grid_for(enm_ctx* ctx, int n_items) {
+    const int g = ctx->num_cus * ctx->waves_per_cu;
+    return std::max(1, std::min(n_items, g));
+}
+
+// Enqueue the whole launch sequence of a plan on `st`.
+int enqueue(enm_ctx* ctx, Plan& P, const double* arena, double* out, double* d_eps, int* d_counts,
+            hipStream_t st) {
+    hipError_t e;
+    int nl = 0;
+    auto chk = [&](hipError_t er, const char* what) -> int {
+        if (er != hipSuccess) return hip_fail(ctx, er, what);
+        return ENM_OK;
+    };
+    int rc;
+    unsigned long long* stats = P.stats.as<unsigned long long>();
+    if ((rc = chk(enm_launch_fill_u64(stats, ST_N, 0ull, st), "fill"))) return rc;
+    if ((rc = chk(enm_launch_fill_u64(P.maxabs.as<unsigned long long>(), P.n_inst, 0ull, st), "fill"))) return rc;
+    if ((rc = chk(enm_launch_fill_i32(P.counters.as<int>(), (int64_t)P.groups.size(), 0, st), "fill"))) return rc;
+    nl += 3;
+    int fb = 0;
+    if ((rc = chk(enm_launch_sort(tab<SortSeg>(P, P.off_segs), (int)P.segs.size(), P.ntiles, P.maxlen, arena,
+                                  P.sk0.as<double>(), P.si0.as<int>(), P.sk1.as<double>(), P.si1.as<int>(),
+                                  &fb, &nl, st), "sort")))
+        return rc;
+    const double* skey = fb ? P.sk1.as<double>() : P.sk0.as<double>();
+    const int* sidx = fb ? P.si1.as<int>() : P.si0.as<int>();
+    if ((rc = chk(enm_launch_gather(tab<GatherItem>(P, P.off_gitems), (int)P.gitems.size(),
+                                    tab<SortSeg>(P, P.off_segs), tab<OrderDev>(P, P.off_orders), arena, skey,
+                                    sidx, P.col64.as<double>(), P.col32.as<float>(), P.perm.as<int>(),
+                                    P.maxabs.as<unsigned long long>(), &nl, st), "gather")))
+        return rc;
+    const OrderDev* orders = tab<OrderDev>(P, P.off_orders);
+    const unsigned long long* maxabs = P.maxabs.as<unsigned long long>();
+    unsigned* counters = P.counters.as<unsigned>();
+    // Per-instance mode interleaves each instance's kNN, counts and combine.
+    auto run_group = [&](const Group& g) -> int {
+        if (!g.is_count) {
+            e = enm_launch_knn(tab<Item>(P, P.off_kitems) + g.first, g.n_items, counters + g.counter,
+                               tab<KnnDev>(P, P.off_kdescs), orders, maxabs, P.perm.as<int>(),
+                               P.col64.as<double>(), P.col32.as<float>(), P.res64.as<double>(), stats,
+                               g.dclass, g.pclass, P.f32, P.brute, grid_for(ctx, g.n_items), st);
+            ++nl;
+            return chk(e, "knn");
+        }
+        e = enm_launch_count(tab<Item>(P, P.off_citems) + g.first, g.n_items, counters + g.counter,
+                             tab<CountDev>(P, P.off_cdescs), orders, maxabs, P.perm.as<int>(),
+                             P.col64.as<double>(), P.col32.as<float>(), P.res64.as<double>(),
+                             P.res32.as<int>(), stats, g.dclass, P.f32, P.brute, grid_for(ctx, g.n_items), st);
+        ++nl;
+        return chk(e, "count");
+    };
+    if (P.per_instance) {
+        for (int ii = 0; ii < P.n_inst; ++ii) {
+            for (const Group& g : P.groups)
+                if (g.inst == ii && !g.is_count && (rc = run_group(g))) return rc;
+            for (const Group& g : P.groups)
+                if (g.inst == ii && g.is_count && (rc = run_group(g))) return rc;
+            if ((rc = chk(enm_launch_combine(tab<InstDev>(P, P.off_insts), tab<double>(P, P.off_cls), ii, 1,
+                                             P.res64.as<double>(), P.res32.as<int>(), out, st), "combine")))
+                return rc;
+            ++nl;
+        }
+    } else {
+        for (const Group& g : P.groups)
+            if (!g.is_count && (rc = run_group(g))) return rc;
+        for (const Group& g : P.groups)
+            if (g.is_count && (rc = run_group(g))) return rc;
+        if ((rc = chk(enm_launch_combine(tab<InstDev>(P, P.off_insts), tab<double>(P, P.off_cls), 0, P.n_inst,
+                                         P.res64.as<double>(), P.res32.as<int>(), out, st), "combine")))
+            return rc;
+        ++nl;
+    }
+    if (d_eps || d_counts) {
+        for (const ExportInst& ex : P.exports) {
+            if (d_eps) {
+                if ((rc = chk(enm_launch_export_eps(P.res64.as<double>() + ex.eps, P.perm.as<int>() + ex.permA,
+                                                    ex.n, d_eps + ex.point_off, st), "export")))
+                    return rc;
+                ++nl;
+            }
+            if (d_counts) {
+                int* base = d_counts + 3 * ex.point_off;
+                if ((rc = chk(enm_launch_fill_i32(base, 3 * ex.n, 0, st), "export"))) return rc;
+                ++nl;
+                for (int s = 0; s < ex.nspace; ++s) {
+                    if ((rc = chk(enm_launch_export_cnt(P.res32.as<int>() + ex.sp[s].counts,
+                                                        P.perm.as<int>() + ex.sp[s].perm, ex.n,
+                                                        base + s * ex.n, st), "export")))
+                        return rc;
+                    ++nl;
+                }
+            }
+        }
+    }
+    P.n_launch = nl;
+    return ENM_OK;
+}
+
+std::string plan_key(const enm_instance* inst, int n_inst, const int64_t* seg_len, const void* arena,
+                     const void* out, const void* eps, const void* cnt, uint32_t flags, int q, int w) {
+    std::string k;
+    k.append((const char*)inst, sizeof(enm_instance) * (size_t)n_inst);
+    for (int i = 0; i < n_inst; ++i)
+        if (inst[i].n_segs > 1)
+            k.append((const char*)(seg_len + inst[i].seg_index), sizeof(int64_t) * (size_t)inst[i].n_segs);
+    const void* ptrs[4] = {arena, out, eps, cnt};
+    k.append((const char*)ptrs, sizeof ptrs);
+    int misc[3] = {(int)flags, q, w};
+    k.append((const char*)misc, sizeof misc);
+    return k;
+}
+
+}  // namespace
+
+extern "C" {
+
+int enm_init(int device, enm_ctx** out_ctx) {
+    if (!out_ctx) { g_init_error = "out_ctx is NULL"; return ENM_E_ARG; }
+    *out_ctx = nullptr;
+    int ndev = 0;
+    hipError_t e = hipGetDeviceCount(&ndev);
+    if (e != hipSuccess || ndev <= 0) {
+        g_init_error = std::string("no HIP device: ") + hipGetErrorString(e);
+        return ENM_E_HIP;
+    }
+    if (device < 0 || device >= ndev) { g_init_error = "device index out of range"; return ENM_E_ARG; }
+    e = hipSetDevice(device);
+    if (e != hipSuccess) { g_init_error = std::string("hipSetDevice: ") + hipGetErrorString(e); return ENM_E_HIP; }
+    hipDeviceProp_t prop;
+    e = hipGetDeviceProperties(&prop, device);
+    if (e != hipSuccess) {
+        g_init_error = std::string("hipGetDeviceProperties: ") + hipGetErrorString(e);
+        return ENM_E_HIP;
+    }
+    if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0) {
+        g_init_error = std::string("libennemi_hip is built for gfx950, device is ") + prop.gcnArchName;
+        return ENM_E_UNSUPPORTED;
+    }
+    enm_ctx* c = new enm_ctx();
+    c->device = device;
+    c->num_cus = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
+    *out_ctx = c;
+    return ENM_OK;
+}
+
+void enm_shutdown(enm_ctx* ctx) {
+    if (!ctx) return;
+    (void)hipSetDevice(ctx->device);
+    if (ctx->last_stream) (void)hipStreamSynchronize(ctx->last_stream);
+    (void)hipDeviceSynchronize();
+    ctx->cache.clear();
+    delete ctx;
+}
+
+const char* enm_last_error(const enm_ctx* ctx) {
+    return ctx ? ctx->err.c_str() : g_init_error.c_str();
+}
+
+int enm_run_batch(enm_ctx* ctx, const enm_instance* inst, int32_t n_inst, const int64_t* seg_len,
+                  const double* d_arena, double* d_out, double* d_eps, int32_t* d_counts, uint32_t flags,
+                  void* stream) {
+    if (!ctx) { g_init_error = "ctx is NULL"; return ENM_E_ARG; }
+    ctx->err.clear();
+    if (!inst || n_inst <= 0) return fail(ctx, ENM_E_ARG, "no instances");
+    if (!d_arena || !d_out) return fail(ctx, ENM_E_ARG, "d_arena and d_out must be device pointers");
+    hipError_t e = hipSetDevice(ctx->device);
+    if (e != hipSuccess) return hip_fail(ctx, e, "hipSetDevice");
+    hipStream_t st = (hipStream_t)stream;
+    if (ctx->last_stream != st && ctx->last_stream != nullptr) {
+        // Plans reuse their workspace; finish work queued on another stream first.
+        e = hipStreamSynchronize(ctx->last_stream);
+        if (e != hipSuccess) return hip_fail(ctx, e, "hipStreamSynchronize");
+    }
+    ctx->last_stream = st;
+    const bool use_graph = (flags & ENM_F_GRAPH) != 0;
+    const std::string key = plan_key(inst, n_inst, seg_len, d_arena, d_out, d_eps, d_counts, flags,
+                                     ctx->q_per_lane, ctx->waves_per_cu);
+    // Reuse a cached plan for an identical call.
+    for (auto it = ctx->cache.begin(); it != ctx->cache.end(); ++it) {
+        if ((*it)->key == key) {
+            Plan* P = it->get();
+            if (it != ctx->cache.begin()) ctx->cache.splice(ctx->cache.begin(), ctx->cache, it);
+            ctx->last = P;
+            if (use_graph && P->graph) {
+                e = hipGraphLaunch(P->graph, st);
+                if (e != hipSuccess) return hip_fail(ctx, e, "hipGraphLaunch");
+                P->n_launch = 0;
+                return ENM_OK;
+            }
+            int rc = enqueue(ctx, *P, d_arena, d_out, d_eps, d_counts, st);
+            return rc;
+        }
+    }
+    std::unique_ptr<Plan> up(new Plan());
+    Plan& P = *up;
+    P.key = key;
+    P.f32 = (flags & ENM_F_FP64) == 0;
+    P.brute = (flags & ENM_F_BRUTE) != 0;
+    P.per_instance = (flags & ENM_F_PER_INSTANCE) != 0;
+    int rc = plan_batch(ctx, P, inst, n_inst, seg_len);
+    if (rc) return rc;
+    // Evict before allocating so a cache full of large plans frees memory first.
+    while (ctx->cache.size() >= kMaxCachedPlans) {
+        if (ctx->cache.back().get() == ctx->last) ctx->last = nullptr;
+        (void)hipStreamSynchronize(st);
+        ctx->cache.pop_back();
+    }
+    rc = allocate_and_upload(ctx, P, st);
+    if (rc) return rc;
+    if (use_graph) {
+        hipGraph_t graph = nullptr;
+        e = hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal);
+        if (e != hipSuccess) return hip_fail(ctx, e, "hipStreamBeginCapture");
+        rc = enqueue(ctx, P, d_arena, d_out, d_eps, d_counts, st);
+        hipError_t e2 = hipStreamEndCapture(st, &graph);
+        if (rc) { if (graph) (void)hipGraphDestroy(graph); return rc; }
+        if (e2 != hipSuccess) return hip_fail(ctx, e2, "hipStreamEndCapture");
+        e = hipGraphInstantiate(&P.graph, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        if (e != hipSuccess) return hip_fail(ctx, e, "hipGraphInstantiate");
+        e = hipGraphLaunch(P.graph, st);
+        if (e != hipSuccess) return hip_fail(ctx, e, "hipGraphLaunch");
+    } else {
+        rc = enqueue(ctx, P, d_arena, d_out, d_eps, d_counts, st);
+        if (rc) return rc;
+    }
+    ctx->last = up.get();
+    ctx->cache.push_front(std::move(up));
+    return ENM_OK;
+}
+
+int enm_get_stats(enm_ctx* ctx, enm_stats* out) {
+    if (!ctx || !out) return ENM_E_ARG;
+    std::memset(out, 0, sizeof(*out));
+    if (!ctx->last) return fail(ctx, ENM_E_ARG, "no batch has run");
+    hipError_t e = hipStreamSynchronize(ctx->last_stream);
+    if (e != hipSuccess) return hip_fail(ctx, e, "hipStreamSynchronize");
+    unsigned long long h[ST_N];
+    e = hipMemcpy(h, ctx->last->stats.p, sizeof h, hipMemcpyDeviceToHost);
+    if (e != hipSuccess) return hip_fail(ctx, e, "hipMemcpy(stats)");
+    out->knn_pairs = h[ST_KNN_PAIRS];
+    out->knn_rechecks = h[ST_KNN_RECHECK];
+    out->count_pairs = h[ST_CNT_PAIRS];
+    out->count_rechecks = h[ST_CNT_RECHECK];
+    out->count_queries_1d = h[ST_CNT_1D];
+    out->sort_keys = (uint64_t)ctx->last->sort_elems;
+    out->n_launches = (uint32_t)ctx->last->n_launch;
+    out->graph_replayed = ctx->last->graph != nullptr && ctx->last->n_launch == 0;
+    return ENM_OK;
+}
+
+int enm_set_tuning(enm_ctx* ctx, int32_t queries_per_lane, int32_t waves_per_cu) {
+    if (!ctx) return ENM_E_ARG;
+    if (queries_per_lane != 1 && queries_per_lane != 2)
+        return fail(ctx, ENM_E_ARG, "queries_per_lane must be 1 or 2");
+    if (waves_per_cu < 1 || waves_per_cu > 64) return fail(ctx, ENM_E_ARG, "waves_per_cu must be in 1..64");
+    ctx->q_per_lane = queries_per_lane;
+    ctx->waves_per_cu = waves_per_cu;
+    return ENM_OK;
+}
+
+int enm_get_tuning(const enm_ctx* ctx, int32_t* queries_per_lane, int32_t* waves_per_cu) {
+    if (!ctx) return ENM_E_ARG;
+    if (queries_per_lane) *queries_per_lane = ctx->q_per_lane;
+    if (waves_per_cu) *waves_per_cu = ctx->waves_per_cu;
+    return ENM_OK;
+}
+
+int enm_autotune(enm_ctx* ctx, const enm_instance* inst, int32_t n_inst, const int64_t* seg_len,
+                 const double* d_arena, double* d_out, uint32_t flags, void* stream) {
+    if (!ctx) return ENM_E_ARG;
+    hipStream_t st = (hipStream_t)stream;
+    hipEvent_t a, b;
+    if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess)
+        return fail(ctx, ENM_E_HIP, "hipEventCreate");
+    const int waves[] = {4, 8, 16, 32};
+    float best = 1e30f;
+    int best_w = ctx->waves_per_cu, best_q = ctx->q_per_lane;
+    const uint32_t fl = flags & ~(uint32_t)ENM_F_GRAPH;
+    for (int q = 1; q <= 1; ++q) {
+        for (int w : waves) {
+            ctx->q_per_lane = q;
+            ctx->waves_per_cu = w;
+            int rc = enm_run_batch(ctx, inst, n_inst, seg_len, d_arena, d_out, nullptr, nullptr, fl, stream);
+            if (rc) { (void)hipEventDestroy(a); (void)hipEventDestroy(b); return rc; }
+            (void)hipEventRecord(a, st);
+            rc = enm_run_batch(ctx, inst, n_inst, seg_len, d_arena, d_out, nullptr, nullptr, fl, stream);
+            (void)hipEventRecord(b, st);
+            if (rc) { (void)hipEventDestroy(a); (void)hipEventDestroy(b); return rc; }
+            (void)hipEventSynchronize(b);
+            float ms = 0.f;
+            (void)hipEventElapsedTime(&ms, a, b);
+            if (ms < best) { best = ms; best_w = w; best_q = q; }
+        }
+    }
+    ctx->q_per_lane = best_q;
+    ctx->waves_per_cu = best_w;
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    return ENM_OK;
+}
+
+}  // extern "C"

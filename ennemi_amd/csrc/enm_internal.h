@@ -1,0 +1,109 @@
+// Internal layout shared by the HIP kernels and the host planner of
+// libennemi_hip.so.  Nothing here is part of the public ABI.
+//
+// HBM layout of one batch.  All offsets are element offsets into five
+// workspace buffers, so each kernel reads candidates through one read-only
+// base pointer per element type (which lets the compiler keep wave-uniform
+// candidate reads on the scalar path) and writes through others:
+//
+//   col64 (double) : per order, its columns in sorted order [col][pos]
+//                    (column 0 is the sort key).
+//   col32 (float)  : per order, the FP32 rounding of col64's columns.
+//   perm  (int32)  : per order, the permutation pos -> instance point index.
+//   res64 (double) : radii eps_A (kNN order) and eps_orig (instance
+//                    numbering), per-tile partial sums.
+//   res32 (int32)  : per-space ball counts, per-tile zero flags.
+//
+// An "order" is one sorted view of an instance's points: the points of each
+// scope range (one class segment, or all points) sorted by the key column,
+// with the columns the stage needs gathered into that order.
+#pragma once
+#include <stdint.h>
+
+#define ENM_TILE 64            // queries per work item (one per lane of a wave64)
+#define ENM_SORT_TILE 2048     // elements per block-sort tile and merge chunk
+#define ENM_SORT_THREADS 256
+#define ENM_MAX_SPACES 3
+
+// One contiguous key range to sort: keys at arena[src + t], t in [0, len),
+// carried with their instance point index p0 + t.
+struct SortSeg {
+    int64_t src;    // arena offset of the first key
+    int64_t sbase;  // base position in the sort buffers
+    int32_t len;
+    int32_t first_tile;  // index of its first ENM_SORT_TILE chunk
+    int32_t p0;          // instance point index of the first key
+    int32_t pad;
+};
+
+// Gather work: one ENM_SORT_TILE chunk of one sort segment into one order.
+struct GatherItem {
+    int32_t order;
+    int32_t sseg;
+    int32_t off;   // chunk offset within the sort segment
+    int32_t pos0;  // order position of the sort segment's first element
+};
+
+struct OrderDev {
+    int64_t c64;        // col64 offset of column 0; column c at c64 + c*n
+    int64_t c32;        // col32 offset of column 0
+    int64_t perm;       // perm offset
+    int64_t n;          // points of the instance
+    int32_t ncol;
+    int32_t inst;
+    int64_t src[16];    // arena offsets of the instance columns gathered (src[0] = key)
+};
+
+// kNN over one order (its scope ranges are the class segments, or all points).
+struct KnnDev {
+    int32_t order;
+    int32_t inst;
+    int32_t k;
+    int32_t kl;         // 1: write sum(log eps) partials per tile (entropy)
+    int64_t eps;        // res64 offset: radius per order position
+    int64_t eps_orig;   // res64 offset: radius per instance point (or -1)
+    int64_t part;       // res64 offset: per-tile partial sum (kl only, else -1)
+    int64_t pad;
+};
+
+// Ball count in one space (a subset of an order's columns, key first).
+struct CountDev {
+    int32_t order;
+    int32_t inst;
+    int32_t ncol;
+    int32_t same_order;  // 1: radius read at the same order position (eps_A)
+    int32_t cols[16];    // indices into the order's columns; cols[0] == 0
+    int64_t eps;         // res64 offset of the radius source
+    int64_t eps_perm;    // perm offset of the order's permutation (radius via eps_orig)
+    int64_t counts;      // res32 offset: count per order position
+    int64_t part;        // res64 offset: per-tile sum of psi(count)
+    int64_t zero;        // res32 offset: per-tile "some count is 0" flag
+};
+
+// One work item: a tile of up to 64 consecutive order positions [t0, t1)
+// inside the scope range [lo, hi).
+struct Item {
+    int32_t desc;
+    int32_t tile;   // tile index within its descriptor (partials slot)
+    int32_t lo, hi;
+    int32_t t0, t1;
+};
+
+// Per-instance data for the final combination.
+struct InstDev {
+    int32_t kind;
+    int32_t k;
+    int64_t n;
+    int32_t dx;
+    int32_t nspace;
+    int32_t n_segs;
+    int32_t pad0;
+    int64_t seg_len;     // offset of the class sizes (as doubles) in the class table, or -1
+    int64_t part[ENM_MAX_SPACES];  // res64 offsets of per-tile partials
+    int64_t zero[ENM_MAX_SPACES];  // res32 offsets of per-tile zero flags
+    int32_t ntile[ENM_MAX_SPACES];
+    int32_t pad1;
+};
+
+// Device counters (enm_stats order).
+enum { ST_KNN_PAIRS = 0, ST_KNN_RECHECK, ST_CNT_PAIRS, ST_CNT_RECHECK, ST_CNT_1D, ST_SORT_KEYS, ST_N };

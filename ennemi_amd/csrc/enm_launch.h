@@ -1,0 +1,41 @@
+// Host-side launchers of the kernels in enm_*.hip (internal to the library).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "enm_internal.h"
+#include "../../include/ennemi_hip.h"
+
+hipError_t enm_launch_fill_u64(unsigned long long* p, int64_t n, unsigned long long v, hipStream_t st);
+hipError_t enm_launch_fill_i32(int* p, int64_t n, int v, hipStream_t st);
+
+hipError_t enm_launch_sort(const SortSeg* d_segs, int nsegs, int ntiles, int64_t maxlen,
+                           const double* arena, double* k0, int* i0, double* k1, int* i1,
+                           int* final_buf, int* n_launch, hipStream_t st);
+hipError_t enm_launch_gather(const GatherItem* d_items, int nitems, const SortSeg* d_segs,
+                             const OrderDev* d_orders, const double* arena,
+                             const double* skey, const int* sidx,
+                             double* col64, float* col32, int* perm,
+                             unsigned long long* maxabs, int* n_launch, hipStream_t st);
+
+int enm_knn_dim_class(int d);
+int enm_knn_list_class(int k1);
+hipError_t enm_launch_knn(const Item* items, int n_items, unsigned* counter,
+                          const KnnDev* descs, const OrderDev* orders,
+                          const unsigned long long* maxabs, const int* perm,
+                          const double* col64, const float* col32, double* res64,
+                          unsigned long long* stats, int dclass, int pclass, bool f32, bool brute,
+                          int grid, hipStream_t st);
+
+int enm_count_dim_class(int d);
+hipError_t enm_launch_count(const Item* items, int n_items, unsigned* counter,
+                            const CountDev* descs, const OrderDev* orders,
+                            const unsigned long long* maxabs, const int* perm,
+                            const double* col64, const float* col32, double* res64, int* res32,
+                            unsigned long long* stats, int dclass, bool f32, bool brute,
+                            int grid, hipStream_t st);
+
+hipError_t enm_launch_combine(const InstDev* insts, const double* cls, int first, int count,
+                              const double* res64, const int* res32, double* out, hipStream_t st);
+hipError_t enm_launch_export_eps(const double* src, const int* perm, int64_t n, double* dst, hipStream_t st);
+hipError_t enm_launch_export_cnt(const int* src, const int* perm, int64_t n, int* dst, hipStream_t st);

@@ -1,0 +1,160 @@
+/*
+ * ennemi_hip.h -- C ABI of the MI355X (gfx950) nearest-neighbour estimator
+ * library libennemi_hip.so.
+ *
+ * The reference (polsys/ennemi 1.5.0) has no FFI: its hot path sits behind
+ * four private Python estimators and a thread-pool fan-out.  This ABI
+ * replaces, in one batched call,
+ *
+ *   _map_maybe_parallel(_lagged_mi, params, ...)   ennemi/_driver.py:492,715,736-785
+ *
+ * whose tasks each end in one of
+ *
+ *   _estimate_single_mi(x, y, k)                    ennemi/_entropy_estimators.py:69-113
+ *   _estimate_conditional_mi(x, y, cond, k)         ennemi/_entropy_estimators.py:116-156
+ *   _estimate_semidiscrete_mi(x, y, k)              ennemi/_entropy_estimators.py:159-200
+ *   _estimate_conditional_semidiscrete_mi(...)      ennemi/_entropy_estimators.py:203-247
+ *   _estimate_single_entropy(x, k)                  ennemi/_entropy_estimators.py:21-42
+ *
+ * and the SciPy calls inside them, cKDTree.query(k=[k+1], p=inf) and
+ * cKDTree.query_ball_point(r, p=inf, return_length=True).  Each instance in a
+ * batch is one such estimator call on already validated, already rescaled
+ * float64 data (the host keeps ennemi's validation and preprocessing).
+ *
+ * Conventions: plain C types only.  Pointers named d_* are device pointers
+ * (HBM of the context's GPU); everything else is host memory, read during the
+ * call only.  Every function returns 0 on success and a negative ENM_E_* code
+ * on failure; nothing throws or aborts across the ABI, and the message of the
+ * last failure is kept in the context (enm_last_error).  A context is not
+ * thread-safe: use one per host thread or serialise calls.
+ */
+#ifndef ENNEMI_HIP_H
+#define ENNEMI_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ENM_ABI_VERSION 1
+
+/* Estimator kinds (enm_instance.kind). */
+#define ENM_KIND_KSG   0 /* _estimate_single_mi                    :69-113  */
+#define ENM_KIND_CMI   1 /* _estimate_conditional_mi               :116-156 */
+#define ENM_KIND_ROSS  2 /* _estimate_semidiscrete_mi              :159-200 */
+#define ENM_KIND_CROSS 3 /* _estimate_conditional_semidiscrete_mi  :203-247 */
+#define ENM_KIND_KL    4 /* _estimate_single_entropy               :21-42   */
+
+/* Maximum number of float64 columns (dimensions) of one instance. */
+#define ENM_MAX_COLS 16
+
+/* Flags for enm_run_batch. */
+#define ENM_F_FP64          0x1u  /* plain FP64 distances (no FP32 filter)            */
+#define ENM_F_BRUTE         0x2u  /* scan every candidate (no sorted-window pruning)  */
+#define ENM_F_GRAPH         0x4u  /* capture the batch in a hipGraph, replay on reuse */
+#define ENM_F_PER_INSTANCE  0x8u  /* one launch sequence per instance (no batching)   */
+
+/* Error codes. */
+#define ENM_OK            0
+#define ENM_E_ARG        -1  /* invalid argument (message says which)       */
+#define ENM_E_HIP        -2  /* a HIP runtime call failed                   */
+#define ENM_E_NOMEM      -3  /* device workspace allocation failed          */
+#define ENM_E_UNSUPPORTED -4 /* shape outside what the kernels support      */
+
+typedef struct enm_ctx enm_ctx;
+
+/*
+ * One estimator call.  The instance's points are numbered 0..n-1; column c
+ * holds coordinate c of every point at d_arena[col[c] .. col[c]+n).
+ * Columns are ordered x (dx of them), then y (dy), then z (dz):
+ *
+ *   KSG   : x, y        -> psi(n) + psi(k) - mean(psi(n_x) + psi(n_y))
+ *   CMI   : x, y, z     -> psi(k) - mean(psi(n_xz) + psi(n_yz) - psi(n_z))
+ *   ROSS  : x           -> psi(n) + psi(k) - mean(psi(n_x))
+ *                          - sum_c psi(n_c) n_c / n
+ *   CROSS : x, z        -> psi(k) - mean(psi(n_xz) + psi(n_yz) - psi(n_z))
+ *   KL    : x           -> psi(n) - psi(k) + dx (mean(log eps) + log 2)
+ *
+ * ROSS and CROSS need the points grouped by discrete class: n_segs classes
+ * of sizes seg_len[seg_index .. seg_index+n_segs) (sum = n) in class order,
+ * as np.unique orders them.  Other kinds use n_segs = 1.
+ * Counts use the radius eps - 1e-12 with an inclusive test, as the reference
+ * does (ennemi/_entropy_estimators.py:104-110); psi is the reference's
+ * truncated series with psi(0) = +inf poisoning the mean (:327-350).
+ */
+typedef struct enm_instance {
+    int32_t kind;
+    int32_t k;
+    int64_t n;
+    int32_t dx, dy, dz;
+    int32_t n_segs;
+    int64_t seg_index;
+    int64_t col[ENM_MAX_COLS];
+} enm_instance;
+
+/* Work counters of the last enm_run_batch (device-side, exact). */
+typedef struct enm_stats {
+    uint64_t knn_pairs;       /* (query, candidate) distance evaluations in kNN     */
+    uint64_t knn_rechecks;    /* FP64 re-evaluations triggered by the FP32 filter   */
+    uint64_t count_pairs;     /* pair evaluations in multi-dimensional ball counts  */
+    uint64_t count_rechecks;  /* FP64 re-evaluations in the count guard band        */
+    uint64_t count_queries_1d;/* binary-search (1-D) ball counts                    */
+    uint64_t sort_keys;       /* keys sorted                                        */
+    uint32_t n_launches;      /* kernel launches (0 when a graph was replayed)      */
+    uint32_t graph_replayed;  /* 1 if the batch ran as a hipGraph replay            */
+} enm_stats;
+
+/* Create a context on HIP device `device`. */
+int enm_init(int device, enm_ctx** out_ctx);
+
+/* Release the context, its workspace and cached graphs. */
+void enm_shutdown(enm_ctx* ctx);
+
+/* Message of the last failure on ctx (or of enm_init when ctx is NULL). */
+const char* enm_last_error(const enm_ctx* ctx);
+
+/*
+ * Run n_inst estimator instances and write one float64 value per instance to
+ * d_out[0..n_inst).  Optional exports (either pointer may be NULL), in each
+ * instance's own point numbering, instance after instance (point offset of
+ * instance i = sum of n over instances before i):
+ *   d_eps    : the k-th-neighbour radius of every point (float64);
+ *   d_counts : 3 int32 ball counts per point, laid out [space][point] per
+ *              instance (spaces x,y for KSG; xz,yz,z for CMI/CROSS; x for
+ *              ROSS; unused slots are 0).
+ * `stream` is a hipStream_t (NULL = the legacy default stream); all work is
+ * enqueued on it and the call returns without synchronising.
+ */
+int enm_run_batch(enm_ctx* ctx,
+                  const enm_instance* inst, int32_t n_inst,
+                  const int64_t* seg_len,
+                  const double* d_arena,
+                  double* d_out,
+                  double* d_eps,
+                  int32_t* d_counts,
+                  uint32_t flags,
+                  void* stream);
+
+/* Counters of the last enm_run_batch; synchronises the stream it used. */
+int enm_get_stats(enm_ctx* ctx, enm_stats* out);
+
+/*
+ * Autotuning.  Tile shapes of the kNN and count kernels are picked per
+ * (joint dimension, list depth) class.  enm_autotune times the candidate
+ * shapes on the given batch (a call of enm_run_batch per shape) and keeps
+ * the fastest in the context; enm_set_tuning sets one explicitly.
+ * queries_per_lane in {1,2}, waves_per_cu in {1..8}.
+ */
+int enm_autotune(enm_ctx* ctx,
+                 const enm_instance* inst, int32_t n_inst,
+                 const int64_t* seg_len, const double* d_arena,
+                 double* d_out, uint32_t flags, void* stream);
+int enm_set_tuning(enm_ctx* ctx, int32_t queries_per_lane, int32_t waves_per_cu);
+int enm_get_tuning(const enm_ctx* ctx, int32_t* queries_per_lane, int32_t* waves_per_cu);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* ENNEMI_HIP_H */
