@@ -348,6 +348,40 @@ def estimate_corr(y, x, lag=0, *, k: int = 3, cond=None, cond_lag=0, mask=None,
 
 def _estimate_mi(y, x, lag, k, cond, cond_lag, mask, discrete_x, discrete_y, preprocess,
                  drop_nan, callback):
+    tasks, indices, nvar = _plan_mi(y, x, lag, k, cond, cond_lag, mask, discrete_x, discrete_y,
+                                    preprocess, drop_nan)
+    values = tasks.run()
+    result = np.empty((len(lag), nvar))
+    for index, res in zip(indices, values):
+        result[index] = res
+    if callback is not None:
+        for (li, v) in indices:
+            callback(v, lag[li])
+    return result
+
+
+def plan_mi(y, x, lag=0, *, k: int = 3, cond=None, cond_lag=0, mask=None,
+            discrete_y: bool = False, discrete_x: bool = False, preprocess: bool = True,
+            drop_nan: bool = False):
+    """Validate and rescale every task of an ``estimate_mi`` call and return
+    its batch of GPU instances without running it (for timing the device
+    work alone).  Returns (batch, task indices [(lag index, x column)])."""
+    x_arr = np.asarray(x)
+    y_arr = np.asarray(y)
+    cond_arr, ncond = (np.column_stack((np.asarray(cond),)), None) if cond is not None else (None, 1)
+    if cond_arr is not None:
+        ncond = cond_arr.shape[1]
+    lag_arr = np.atleast_1d(lag)
+    cond_lag_arr = np.broadcast_to(np.column_stack((cond_lag,)), (lag_arr.shape[0], ncond))
+    mask_arr = np.asarray(mask) if mask is not None else None
+    tasks, indices, _ = _plan_mi(y_arr, x_arr, lag_arr, k, cond_arr, cond_lag_arr, mask_arr,
+                                 discrete_x, discrete_y, preprocess, drop_nan)
+    if tasks.host_values:
+        raise ValueError("plan_mi covers the GPU estimators only (not discrete-discrete tasks)")
+    return tasks.batch, indices
+
+
+def _plan_mi(y, x, lag, k, cond, cond_lag, mask, discrete_x, discrete_y, preprocess, drop_nan):
     _check_parameters(x, y, k, cond, mask)
     min_lag = min(np.min(lag), np.min(cond_lag))
     max_lag = max(np.max(lag), np.max(cond_lag))
@@ -364,15 +398,7 @@ def _estimate_mi(y, x, lag, k, cond, cond_lag, mask, discrete_x, discrete_y, pre
                   discrete_x, discrete_y, preprocess, drop_nan)
         xs, ys, zs = _prepare_lagged(params, noise)
         _add_mi_task(tasks, xs, ys, zs, k, discrete_x, discrete_y)
-    values = tasks.run()
-
-    result = np.empty((len(lag), nvar))
-    for index, res in zip(indices, values):
-        result[index] = res
-    if callback is not None:
-        for (li, v) in indices:
-            callback(v, lag[li])
-    return result
+    return tasks, indices, nvar
 
 
 # ---------------------------------------------------------------------------

@@ -25,6 +25,8 @@ F_FP64 = 0x1
 F_BRUTE = 0x2
 F_GRAPH = 0x4
 F_PER_INSTANCE = 0x8
+F_TIMING = 0x10
+STAGES = ("fill", "sort", "gather", "knn", "count", "combine")
 
 EXPORTED_SYMBOLS = ("enm_init", "enm_shutdown", "enm_last_error", "enm_run_batch",
                     "enm_get_stats", "enm_autotune", "enm_set_tuning", "enm_get_tuning")
@@ -45,10 +47,14 @@ class EnmStats(ctypes.Structure):
     _fields_ = [("knn_pairs", ctypes.c_uint64), ("knn_rechecks", ctypes.c_uint64),
                 ("count_pairs", ctypes.c_uint64), ("count_rechecks", ctypes.c_uint64),
                 ("count_queries_1d", ctypes.c_uint64), ("sort_keys", ctypes.c_uint64),
-                ("n_launches", ctypes.c_uint32), ("graph_replayed", ctypes.c_uint32)]
+                ("n_launches", ctypes.c_uint32), ("graph_replayed", ctypes.c_uint32),
+                ("stage_ms", ctypes.c_float * 6), ("total_ms", ctypes.c_float)]
 
     def as_dict(self) -> dict:
-        return {name: int(getattr(self, name)) for name, _ in self._fields_}
+        d = {name: int(getattr(self, name)) for name, _ in self._fields_[:8]}
+        d["stage_ms"] = {s: float(self.stage_ms[i]) for i, s in enumerate(STAGES)}
+        d["total_ms"] = float(self.total_ms)
+        return d
 
 
 def load_library(path: str = LIB_PATH) -> ctypes.CDLL:

@@ -7,10 +7,13 @@
 //
 // 1-D spaces (the marginals of KSG, Ross and a 1-D condition): the ball is a
 // contiguous run of the sorted key column around the query, because
-// fl64(v_j - v_i) is monotone in v_j on each side.  Each lane gallops out from
-// its own position (steps 1, 2, 4, ...) and binary-searches the boundary with
-// the same FP64 predicate the reference evaluates, so counts are exact.
-// Algorithmic traffic: 8 B radius + 4 B count per query plus the probes.
+// fl64(v_j - v_i) is monotone in v_j on each side.  Each lane searches both
+// boundaries outward from its own position with 8 independent probes per
+// round (exponential, then 8-ary), using the same FP64 predicate the
+// reference evaluates, so counts are exact.  Algorithmic traffic: 8 B radius
+// + 8 B key + 4 B count per query; the probes mostly hit L2.  In every mode
+// (including ENM_F_BRUTE, which only removes window pruning) 1-D counts use
+// this search.
 //
 // Multi-D spaces: a wave takes 64 consecutive queries, bounds the union of
 // their balls on the key axis, binary-searches that window in the sorted key
@@ -59,6 +62,66 @@ __device__ __forceinline__ void finish_item(const CountDev& cd, const Item& it, 
     }
 }
 
+// Search state of one side of a 1-D ball: P(good) holds, P(bad) fails (or
+// bad is the sentinel just outside the scope range), where for the right
+// side P(j) = fl(v[j] - x) <= r and for the left side P(j) = fl(x - v[j]) <= r.
+// Each round probes 8 positions whose loads are independent, so a lane waits
+// on about 5 rounds of memory latency instead of ~20 dependent probes:
+// first an exponential phase (offsets step*2^m, m = 0..7, step growing by
+// 256 while every probe holds), then an 8-ary refinement of (good, bad).
+struct Side {
+    int good, bad;   // positions; direction is +1 (right) or -1 (left)
+    long long step;
+    bool expo;
+};
+
+template <int DIR>
+__device__ __forceinline__ bool side_pred(const double* __restrict__ v, int j, double x, double r) {
+    return DIR > 0 ? (v[j] - x <= r) : (x - v[j] <= r);
+}
+
+template <int DIR>
+__device__ __forceinline__ void side_round(Side& S, const double* __restrict__ v, int lo, int hi,
+                                           double x, double r) {
+    if (S.expo) {
+        int pos[8];
+        bool ok[8];
+#pragma unroll
+        for (int m = 0; m < 8; ++m) {
+            const long long p = (long long)S.good + DIR * S.step * (1ll << m);
+            const bool in = DIR > 0 ? (p < hi) : (p >= lo);
+            pos[m] = in ? (int)p : S.bad;
+            ok[m] = in && side_pred<DIR>(v, pos[m], x, r);
+        }
+        int first_fail = 8;
+#pragma unroll
+        for (int m = 7; m >= 0; --m) if (!ok[m]) first_fail = m;
+        if (first_fail == 8) {
+            S.good = pos[7];
+            S.step <<= 8;
+        } else {
+            if (first_fail > 0) S.good = pos[first_fail - 1];
+            S.bad = pos[first_fail];
+            S.expo = false;
+        }
+        return;
+    }
+    const int L = DIR > 0 ? S.bad - S.good : S.good - S.bad;
+    if (L <= 1) return;
+    int last_ok = S.good, first_fail = S.bad;
+#pragma unroll
+    for (int m = 0; m < 8; ++m) {
+        // Offsets ceil((m+1) L / 9), strictly inside (good, bad).
+        const int off = min(L - 1, (int)(((long long)(m + 1) * L + 8) / 9));
+        const int q = S.good + DIR * off;
+        const bool okm = side_pred<DIR>(v, q, x, r);
+        if (okm && (DIR > 0 ? q > last_ok : q < last_ok)) last_ok = q;
+        if (!okm && (DIR > 0 ? q < first_fail : q > first_fail)) first_fail = q;
+    }
+    S.good = last_ok;
+    S.bad = first_fail;
+}
+
 __global__ __launch_bounds__(ENM_WAVE)
 void count1d_kernel(CountArgs A, const double* __restrict__ col64, double* __restrict__ res64,
                     int* __restrict__ res32, unsigned long long* __restrict__ stats) {
@@ -76,28 +139,17 @@ void count1d_kernel(CountArgs A, const double* __restrict__ col64, double* __res
             const double r = query_radius(cd, A, res64, i);
             if (!(r < 0.0)) {
                 const double x = v[i];
-                // Right boundary: last j in [i, hi) with fl(v[j] - x) <= r.
-                int good = i, step = 1, bad = it.hi;
-                while (good + step < it.hi) {
-                    if (v[good + step] - x <= r) { good += step; step <<= 1; }
-                    else { bad = good + step; break; }
+                Side R{i, it.hi, 1, true};
+                Side Lf{i, it.lo - 1, 1, true};
+                // Both sides advance in the same round, so their loads overlap.
+                for (;;) {
+                    const bool rdone = !R.expo && R.bad - R.good <= 1;
+                    const bool ldone = !Lf.expo && Lf.good - Lf.bad <= 1;
+                    if (rdone && ldone) break;
+                    if (!rdone) side_round<1>(R, v, it.lo, it.hi, x, r);
+                    if (!ldone) side_round<-1>(Lf, v, it.lo, it.hi, x, r);
                 }
-                while (bad - good > 1) {
-                    const int mid = good + ((bad - good) >> 1);
-                    if (v[mid] - x <= r) good = mid; else bad = mid;
-                }
-                const int right = good;
-                // Left boundary: first j in [lo, i] with fl(x - v[j]) <= r.
-                good = i; step = 1; bad = it.lo - 1;
-                while (good - step >= it.lo) {
-                    if (x - v[good - step] <= r) { good -= step; step <<= 1; }
-                    else { bad = good - step; break; }
-                }
-                while (good - bad > 1) {
-                    const int mid = good - ((good - bad) >> 1);
-                    if (x - v[mid] <= r) good = mid; else bad = mid;
-                }
-                cnt = right - good + 1;
+                cnt = R.good - Lf.good + 1;
             }
         }
         finish_item(cd, it, active, cnt, res32, res64, i);
@@ -266,7 +318,7 @@ hipError_t enm_launch_count(const Item* items, int n_items, unsigned* counter,
                             int grid, hipStream_t st) {
     if (n_items <= 0) return hipSuccess;
     CountArgs a{items, n_items, counter, descs, orders, maxabs, perm, brute ? 1 : 0};
-    if (dclass == 1 && !brute) {
+    if (dclass == 1) {
         hipLaunchKernelGGL(count1d_kernel, dim3(grid), dim3(ENM_WAVE), 0, st, a, col64, res64, res32, stats);
         return hipGetLastError();
     }

@@ -103,7 +103,10 @@ struct Plan {
     hipEvent_t upload_done = nullptr;
     hipGraphExec_t graph = nullptr;
     int n_launch = 0;
+    bool timing = false;       // events recorded by the last enqueue
+    hipEvent_t ev[ENM_N_STAGES + 1] = {};
     ~Plan() {
+        for (hipEvent_t e : ev) if (e) (void)hipEventDestroy(e);
         if (graph) (void)hipGraphExecDestroy(graph);
         if (upload_done) (void)hipEventDestroy(upload_done);
         if (pinned) (void)hipHostFree(pinned);
@@ -536,9 +539,10 @@ int grid_for(enm_ctx* ctx, int n_items) {
     return std::max(1, std::min(n_items, g));
 }
 
-// Enqueue the whole launch sequence of a plan on `st`.
+// Enqueue the whole launch sequence of a plan on `st`.  With `timing`, an
+// event is recorded on `st` before the first and after every stage.
 int enqueue(enm_ctx* ctx, Plan& P, const double* arena, double* out, double* d_eps, int* d_counts,
-            hipStream_t st) {
+            hipStream_t st, bool timing) {
     hipError_t e;
     int nl = 0;
     auto chk = [&](hipError_t er, const char* what) -> int {
@@ -546,16 +550,28 @@ int enqueue(enm_ctx* ctx, Plan& P, const double* arena, double* out, double* d_e
         return ENM_OK;
     };
     int rc;
+    P.timing = timing;
+    if (timing) {
+        for (hipEvent_t& ev : P.ev)
+            if (!ev && (e = hipEventCreate(&ev)) != hipSuccess) return hip_fail(ctx, e, "hipEventCreate");
+    }
+    auto mark = [&](int stage) -> int {
+        if (!timing) return ENM_OK;
+        return chk(hipEventRecord(P.ev[stage], st), "hipEventRecord");
+    };
+    if ((rc = mark(0))) return rc;
     unsigned long long* stats = P.stats.as<unsigned long long>();
     if ((rc = chk(enm_launch_fill_u64(stats, ST_N, 0ull, st), "fill"))) return rc;
     if ((rc = chk(enm_launch_fill_u64(P.maxabs.as<unsigned long long>(), P.n_inst, 0ull, st), "fill"))) return rc;
     if ((rc = chk(enm_launch_fill_i32(P.counters.as<int>(), (int64_t)P.groups.size(), 0, st), "fill"))) return rc;
     nl += 3;
+    if ((rc = mark(1))) return rc;
     int fb = 0;
     if ((rc = chk(enm_launch_sort(tab<SortSeg>(P, P.off_segs), (int)P.segs.size(), P.ntiles, P.maxlen, arena,
                                   P.sk0.as<double>(), P.si0.as<int>(), P.sk1.as<double>(), P.si1.as<int>(),
                                   &fb, &nl, st), "sort")))
         return rc;
+    if ((rc = mark(2))) return rc;
     const double* skey = fb ? P.sk1.as<double>() : P.sk0.as<double>();
     const int* sidx = fb ? P.si1.as<int>() : P.si0.as<int>();
     if ((rc = chk(enm_launch_gather(tab<GatherItem>(P, P.off_gitems), (int)P.gitems.size(),
@@ -563,6 +579,7 @@ int enqueue(enm_ctx* ctx, Plan& P, const double* arena, double* out, double* d_e
                                     sidx, P.col64.as<double>(), P.col32.as<float>(), P.perm.as<int>(),
                                     P.maxabs.as<unsigned long long>(), &nl, st), "gather")))
         return rc;
+    if ((rc = mark(3))) return rc;
     const OrderDev* orders = tab<OrderDev>(P, P.off_orders);
     const unsigned long long* maxabs = P.maxabs.as<unsigned long long>();
     unsigned* counters = P.counters.as<unsigned>();
@@ -594,11 +611,15 @@ int enqueue(enm_ctx* ctx, Plan& P, const double* arena, double* out, double* d_e
                 return rc;
             ++nl;
         }
+        if ((rc = mark(4))) return rc;
+        if ((rc = mark(5))) return rc;
     } else {
         for (const Group& g : P.groups)
             if (!g.is_count && (rc = run_group(g))) return rc;
+        if ((rc = mark(4))) return rc;
         for (const Group& g : P.groups)
             if (g.is_count && (rc = run_group(g))) return rc;
+        if ((rc = mark(5))) return rc;
         if ((rc = chk(enm_launch_combine(tab<InstDev>(P, P.off_insts), tab<double>(P, P.off_cls), 0, P.n_inst,
                                          P.res64.as<double>(), P.res32.as<int>(), out, st), "combine")))
             return rc;
@@ -626,6 +647,7 @@ int enqueue(enm_ctx* ctx, Plan& P, const double* arena, double* out, double* d_e
             }
         }
     }
+    if ((rc = mark(6))) return rc;
     P.n_launch = nl;
     return ENM_OK;
 }
@@ -707,8 +729,8 @@ int enm_run_batch(enm_ctx* ctx, const enm_instance* inst, int32_t n_inst, const 
     }
     ctx->last_stream = st;
     const bool use_graph = (flags & ENM_F_GRAPH) != 0;
-    const std::string key = plan_key(inst, n_inst, seg_len, d_arena, d_out, d_eps, d_counts, flags,
-                                     ctx->q_per_lane, ctx->waves_per_cu);
+    const std::string key = plan_key(inst, n_inst, seg_len, d_arena, d_out, d_eps, d_counts,
+                                     flags & ~(uint32_t)ENM_F_TIMING, ctx->q_per_lane, ctx->waves_per_cu);
     // Reuse a cached plan for an identical call.
     for (auto it = ctx->cache.begin(); it != ctx->cache.end(); ++it) {
         if ((*it)->key == key) {
@@ -716,12 +738,14 @@ int enm_run_batch(enm_ctx* ctx, const enm_instance* inst, int32_t n_inst, const 
             if (it != ctx->cache.begin()) ctx->cache.splice(ctx->cache.begin(), ctx->cache, it);
             ctx->last = P;
             if (use_graph && P->graph) {
+                P->timing = false;
                 e = hipGraphLaunch(P->graph, st);
                 if (e != hipSuccess) return hip_fail(ctx, e, "hipGraphLaunch");
                 P->n_launch = 0;
                 return ENM_OK;
             }
-            int rc = enqueue(ctx, *P, d_arena, d_out, d_eps, d_counts, st);
+            P->timing = false;
+            int rc = enqueue(ctx, *P, d_arena, d_out, d_eps, d_counts, st, (flags & ENM_F_TIMING) != 0);
             return rc;
         }
     }
@@ -745,7 +769,7 @@ int enm_run_batch(enm_ctx* ctx, const enm_instance* inst, int32_t n_inst, const 
         hipGraph_t graph = nullptr;
         e = hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal);
         if (e != hipSuccess) return hip_fail(ctx, e, "hipStreamBeginCapture");
-        rc = enqueue(ctx, P, d_arena, d_out, d_eps, d_counts, st);
+        rc = enqueue(ctx, P, d_arena, d_out, d_eps, d_counts, st, false);
         hipError_t e2 = hipStreamEndCapture(st, &graph);
         if (rc) { if (graph) (void)hipGraphDestroy(graph); return rc; }
         if (e2 != hipSuccess) return hip_fail(ctx, e2, "hipStreamEndCapture");
@@ -755,7 +779,7 @@ int enm_run_batch(enm_ctx* ctx, const enm_instance* inst, int32_t n_inst, const 
         e = hipGraphLaunch(P.graph, st);
         if (e != hipSuccess) return hip_fail(ctx, e, "hipGraphLaunch");
     } else {
-        rc = enqueue(ctx, P, d_arena, d_out, d_eps, d_counts, st);
+        rc = enqueue(ctx, P, d_arena, d_out, d_eps, d_counts, st, (flags & ENM_F_TIMING) != 0);
         if (rc) return rc;
     }
     ctx->last = up.get();
@@ -780,6 +804,18 @@ int enm_get_stats(enm_ctx* ctx, enm_stats* out) {
     out->sort_keys = (uint64_t)ctx->last->sort_elems;
     out->n_launches = (uint32_t)ctx->last->n_launch;
     out->graph_replayed = ctx->last->graph != nullptr && ctx->last->n_launch == 0;
+    if (ctx->last->timing) {
+        for (int s = 0; s < ENM_N_STAGES; ++s) {
+            float ms = 0.f;
+            e = hipEventElapsedTime(&ms, ctx->last->ev[s], ctx->last->ev[s + 1]);
+            if (e != hipSuccess) return hip_fail(ctx, e, "hipEventElapsedTime");
+            out->stage_ms[s] = ms;
+        }
+        float tot = 0.f;
+        e = hipEventElapsedTime(&tot, ctx->last->ev[0], ctx->last->ev[ENM_N_STAGES]);
+        if (e != hipSuccess) return hip_fail(ctx, e, "hipEventElapsedTime");
+        out->total_ms = tot;
+    }
     return ENM_OK;
 }
 

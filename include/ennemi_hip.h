@@ -54,6 +54,16 @@ extern "C" {
 #define ENM_F_BRUTE         0x2u  /* scan every candidate (no sorted-window pruning)  */
 #define ENM_F_GRAPH         0x4u  /* capture the batch in a hipGraph, replay on reuse */
 #define ENM_F_PER_INSTANCE  0x8u  /* one launch sequence per instance (no batching)   */
+#define ENM_F_TIMING       0x10u  /* record HIP events between stages (not in graphs) */
+
+/* Stages timed with ENM_F_TIMING (enm_stats.stage_ms). */
+#define ENM_STAGE_FILL    0
+#define ENM_STAGE_SORT    1
+#define ENM_STAGE_GATHER  2
+#define ENM_STAGE_KNN     3
+#define ENM_STAGE_COUNT   4
+#define ENM_STAGE_COMBINE 5
+#define ENM_N_STAGES      6
 
 /* Error codes. */
 #define ENM_OK            0
@@ -103,6 +113,8 @@ typedef struct enm_stats {
     uint64_t sort_keys;       /* keys sorted                                        */
     uint32_t n_launches;      /* kernel launches (0 when a graph was replayed)      */
     uint32_t graph_replayed;  /* 1 if the batch ran as a hipGraph replay            */
+    float stage_ms[ENM_N_STAGES]; /* per-stage device time with ENM_F_TIMING, else 0  */
+    float total_ms;           /* first to last event with ENM_F_TIMING, else 0     */
 } enm_stats;
 
 /* Create a context on HIP device `device`. */
