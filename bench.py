@@ -98,6 +98,8 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-compare", action="store_true")
     ap.add_argument("--cpu-lags", type=int, default=12, help="lags in the CPU baseline sample")
+    ap.add_argument("--copy-calib", action="store_true",
+                    help="first run a 1 GiB device-to-device copy (PMC byte-counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -108,6 +110,16 @@ def main() -> int:
     import torch
     import ennemi_amd._native as nat
     from ennemi_amd._driver import plan_mi
+
+    if args.copy_calib:
+        # A plain copy of known size: its FETCH_SIZE / WRITE_SIZE per dispatch
+        # give the scale of those counters (KiB per dispatch) on this device.
+        src = torch.ones(1 << 27, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log("copy calibration: 1 GiB read, 1 GiB written")
+        del src, dst
 
     lags = np.arange(-48, 49)
     y, x = lag_scan_data(args.length)

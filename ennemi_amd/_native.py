@@ -123,8 +123,8 @@ class Context:
         self.check(self.lib.enm_get_stats(self.ctx, ctypes.byref(st)), "enm_get_stats")
         return st.as_dict()
 
-    def set_tuning(self, queries_per_lane: int, waves_per_cu: int) -> None:
-        self.check(self.lib.enm_set_tuning(self.ctx, queries_per_lane, waves_per_cu), "enm_set_tuning")
+    def set_tuning(self, strip_points: int, waves_per_cu: int) -> None:
+        self.check(self.lib.enm_set_tuning(self.ctx, strip_points, waves_per_cu), "enm_set_tuning")
 
     def get_tuning(self) -> Tuple[int, int]:
         q = ctypes.c_int32()

@@ -7,13 +7,13 @@
 //
 // 1-D spaces (the marginals of KSG, Ross and a 1-D condition): the ball is a
 // contiguous run of the sorted key column around the query, because
-// fl64(v_j - v_i) is monotone in v_j on each side.  Each lane searches both
-// boundaries outward from its own position with 8 independent probes per
-// round (exponential, then 8-ary), using the same FP64 predicate the
-// reference evaluates, so counts are exact.  Algorithmic traffic: 8 B radius
-// + 8 B key + 4 B count per query; the probes mostly hit L2.  In every mode
-// (including ENM_F_BRUTE, which only removes window pruning) 1-D counts use
-// this search.
+// fl64(v_j - v_i) is monotone in v_j on each side.  Each query binary-searches
+// both boundaries in an LDS copy of the keys around its item, with the same
+// FP64 predicate the reference evaluates, so counts are exact; a boundary
+// beyond the staged keys is finished in global memory with 8 independent
+// probes per round.  Algorithmic traffic: 8 B radius + 8 B key + 4 B count
+// per query.  In every mode (including ENM_F_BRUTE, which only removes window
+// pruning) 1-D counts use this search.
 //
 // Multi-D spaces: a wave takes 64 consecutive queries, bounds the union of
 // their balls on the key axis, binary-searches that window in the sorted key
@@ -122,38 +122,101 @@ __device__ __forceinline__ void side_round(Side& S, const double* __restrict__ v
     S.bad = first_fail;
 }
 
+// Boundary search inside an LDS copy w[0, nw) of the window: last position
+// in [i, nw) with fl(w[j] - x) <= r (right) or first in [0, i] with
+// fl(x - w[j]) <= r (left); position i holds x itself.
+__device__ __forceinline__ int lds_right(const double* w, int i, int nw, double x, double r) {
+    int good = i, bad = nw;
+    while (bad - good > 1) {
+        const int mid = (good + bad) >> 1;
+        if (w[mid] - x <= r) good = mid; else bad = mid;
+    }
+    return good;
+}
+__device__ __forceinline__ int lds_left(const double* w, int i, double x, double r) {
+    int good = i, bad = -1;
+    while (good - bad > 1) {
+        const int mid = (good + bad) >> 1;
+        if (x - w[mid] <= r) good = mid; else bad = mid;
+    }
+    return good;
+}
+
+// Finish a boundary search in global memory from a known state (the 8-probe
+// rounds of side_round).
+template <int DIR>
+__device__ __forceinline__ int global_side(const double* __restrict__ v, int good, int bad, int lo,
+                                           int hi, double x, double r) {
+    Side S{good, bad, 1, true};
+    for (;;) {
+        const bool done = !S.expo && (DIR > 0 ? S.bad - S.good : S.good - S.bad) <= 1;
+        if (done) break;
+        side_round<DIR>(S, v, lo, hi, x, r);
+    }
+    return S.good;
+}
+
+// 1-D counts.  An item is ENM_C1_ITEM consecutive positions of a key-sorted
+// order, ENM_C1_Q per lane.  The wave copies the keys of the item and
+// ENM_C1_HALO positions on each side into LDS with coalesced loads, and each
+// query binary-searches both boundaries there.  A boundary that lies beyond
+// the staged keys (the query's predicate still holds at the window's edge)
+// is finished from that edge in global memory, so a few wide balls do not
+// widen the window of the whole item.
 __global__ __launch_bounds__(ENM_WAVE)
 void count1d_kernel(CountArgs A, const double* __restrict__ col64, double* __restrict__ res64,
                     int* __restrict__ res32, unsigned long long* __restrict__ stats) {
+    __shared__ double win[ENM_C1_CAP];
+    const int lane = threadIdx.x;
     for (;;) {
         const unsigned idx = pop_ticket(A.counter);
         if (idx >= (unsigned)A.n_items) break;
-        const Item it = A.items[idx];
-        const CountDev cd = A.descs[uni(it.desc)];
+        Item it = A.items[idx];
+        it.desc = uni(it.desc); it.tile = uni(it.tile);
+        it.lo = uni(it.lo); it.hi = uni(it.hi); it.t0 = uni(it.t0); it.t1 = uni(it.t1);
+        const CountDev cd = A.descs[it.desc];
         const OrderDev od = A.orders[cd.order];
         const double* v = col64 + od.c64;  // sorted key column
-        const int i = it.t0 + (int)threadIdx.x;
-        const bool active = i < it.t1;
-        int cnt = 0;
-        if (active) {
-            const double r = query_radius(cd, A, res64, i);
-            if (!(r < 0.0)) {
-                const double x = v[i];
-                Side R{i, it.hi, 1, true};
-                Side Lf{i, it.lo - 1, 1, true};
-                // Both sides advance in the same round, so their loads overlap.
-                for (;;) {
-                    const bool rdone = !R.expo && R.bad - R.good <= 1;
-                    const bool ldone = !Lf.expo && Lf.good - Lf.bad <= 1;
-                    if (rdone && ldone) break;
-                    if (!rdone) side_round<1>(R, v, it.lo, it.hi, x, r);
-                    if (!ldone) side_round<-1>(Lf, v, it.lo, it.hi, x, r);
+        const int w0 = max(it.lo, it.t0 - ENM_C1_HALO);
+        const int w1 = min(it.hi, it.t1 + ENM_C1_HALO);
+        const int nw = w1 - w0;
+        for (int t = lane; t < nw; t += ENM_WAVE) win[t] = v[w0 + t];
+        __syncthreads();
+        double ps = 0.0;
+        bool zero = false;
+#pragma unroll
+        for (int q = 0; q < ENM_C1_Q; ++q) {
+            const int i = it.t0 + lane + ENM_WAVE * q;
+            if (i < it.t1) {
+                const double r = query_radius(cd, A, res64, i);
+                int cnt = 0;
+                if (!(r < 0.0)) {
+                    const int p = i - w0;
+                    const double x = win[p];
+                    int right, left;
+                    if (w1 < it.hi && win[nw - 1] - x <= r)
+                        right = global_side<1>(v, w1 - 1, it.hi, it.lo, it.hi, x, r);
+                    else
+                        right = w0 + lds_right(win, p, nw, x, r);
+                    if (w0 > it.lo && x - win[0] <= r)
+                        left = global_side<-1>(v, w0, it.lo - 1, it.lo, it.hi, x, r);
+                    else
+                        left = w0 + lds_left(win, p, x, r);
+                    cnt = right - left + 1;
                 }
-                cnt = R.good - Lf.good + 1;
+                res32[cd.counts + i] = cnt;
+                if (cnt > 0) ps += enm_psi((double)cnt);
+                else zero = true;
             }
         }
-        finish_item(cd, it, active, cnt, res32, res64, i);
-        if (threadIdx.x == 0) atomicAdd(stats + ST_CNT_1D, (unsigned long long)(it.t1 - it.t0));
+        __syncthreads();
+        const double s = wave_sum(ps);
+        const int z = wave_or(zero);
+        if (lane == 0) {
+            res64[cd.part + it.tile] = s;
+            res32[cd.zero + it.tile] = z;
+            atomicAdd(stats + ST_CNT_1D, (unsigned long long)(it.t1 - it.t0));
+        }
     }
 }
 

@@ -98,6 +98,46 @@ __device__ __forceinline__ void list_insert(double (&L)[P], double v) {
     L[0] = fmin(L[0], v);
 }
 
+__device__ __forceinline__ double uni_f64(double v) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readfirstlane((int)(b & 0xffffffff));
+    const int hi = __builtin_amdgcn_readfirstlane((int)(b >> 32));
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
+__device__ __forceinline__ void cex(double& a, double& b) {
+    const double lo = fmin(a, b), hi = fmax(a, b);
+    a = lo;
+    b = hi;
+}
+
+// Sort 8 values ascending (19 compare-exchanges, Batcher's odd-even merge
+// network for n = 8).
+__device__ __forceinline__ void sort8(double (&e)[8]) {
+    cex(e[0], e[1]); cex(e[2], e[3]); cex(e[4], e[5]); cex(e[6], e[7]);
+    cex(e[0], e[2]); cex(e[1], e[3]); cex(e[4], e[6]); cex(e[5], e[7]);
+    cex(e[1], e[2]); cex(e[5], e[6]);
+    cex(e[0], e[4]); cex(e[1], e[5]); cex(e[2], e[6]); cex(e[3], e[7]);
+    cex(e[2], e[4]); cex(e[3], e[5]);
+    cex(e[1], e[2]); cex(e[3], e[4]); cex(e[5], e[6]);
+}
+
+// Keep the P smallest of the ascending list L[0..P) and the ascending
+// values e[0..8) (P = 4 or 8), L ascending afterwards: min(L[t], e[P-1-t])
+// selects them as a bitonic sequence, which a bitonic merge then sorts.
+template <int P>
+__device__ __forceinline__ void list_merge8(double (&L)[P], const double (&e)[8]) {
+    static_assert(P == 4 || P == 8, "merge network for 4 or 8 slots");
+#pragma unroll
+    for (int t = 0; t < P; ++t) L[t] = fmin(L[t], e[P - 1 - t]);
+#pragma unroll
+    for (int half = P / 2; half > 0; half >>= 1) {
+#pragma unroll
+        for (int t = 0; t < P; ++t)
+            if ((t & half) == 0) cex(L[t], L[t + half]);
+    }
+}
+
 // First position in [lo, hi) whose key is >= x (keys ascending), with a
 // wave-uniform key array and bounds: every lane computes the same answer.
 __device__ __forceinline__ int lower_bound_u(const double* __restrict__ v, int lo, int hi, double x) {

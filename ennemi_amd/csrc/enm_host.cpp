@@ -79,8 +79,8 @@ struct ExportInst {
 struct Plan {
     std::string key;
     // host tables
-    std::vector<SortSeg> segs;
-    std::vector<GatherItem> gitems;
+    std::vector<SortSeg> segs, segs2;        // stage 1 (arena keys), stage 2 (grid strips)
+    std::vector<GatherItem> gitems, gitems2;
     std::vector<OrderDev> orders;
     std::vector<KnnDev> kdescs;
     std::vector<CountDev> cdescs;
@@ -89,16 +89,16 @@ struct Plan {
     std::vector<Item> kitems, citems;
     std::vector<Group> groups;
     std::vector<ExportInst> exports;
-    int ntiles = 0;
-    int64_t maxlen = 0;
-    int64_t sort_elems = 0;
+    int ntiles = 0, ntiles2 = 0;
+    int64_t maxlen = 0, maxlen2 = 0;
+    int64_t sort_elems = 0, sort_elems2 = 0;
     int64_t n_col64 = 0, n_col32 = 0, n_perm = 0, n_res64 = 0, n_res32 = 0;
     int n_inst = 0;
     bool f32 = true, brute = false, per_instance = false;
     // device
     DevBuf tables, sk0, si0, sk1, si1, col64, col32, perm, res64, res32, maxabs, counters, stats;
     size_t off_segs = 0, off_gitems = 0, off_orders = 0, off_kdescs = 0, off_cdescs = 0, off_insts = 0,
-           off_cls = 0, off_kitems = 0, off_citems = 0, table_bytes = 0;
+           off_cls = 0, off_kitems = 0, off_citems = 0, off_segs2 = 0, off_gitems2 = 0, table_bytes = 0;
     void* pinned = nullptr;
     hipEvent_t upload_done = nullptr;
     hipGraphExec_t graph = nullptr;
@@ -119,7 +119,7 @@ struct enm_ctx {
     int device = 0;
     int num_cus = 256;
     std::string err;
-    int q_per_lane = 1;
+    int strip = 1024;       // grid strip size of the kNN order (0: 1-D key order)
     int waves_per_cu = 16;
     std::list<std::unique_ptr<Plan>> cache;   // most recent first
     Plan* last = nullptr;
@@ -206,6 +206,7 @@ struct Planner {
         od.c64 = P.n_col64;
         od.c32 = P.n_col32;
         od.perm = P.n_perm;
+        od.base_perm = -1;
         for (size_t c = 0; c < cols_src.size(); ++c) od.src[c] = cols_src[c];
         P.n_col64 += od.ncol * I.n;
         P.n_col32 += od.ncol * I.n;
@@ -223,6 +224,49 @@ struct Planner {
                 gi.off = off;
                 gi.pos0 = (int32_t)group_pos0[g][t];
                 P.gitems.push_back(gi);
+            }
+        }
+        return id;
+    }
+
+    // A grid order over base order `base`: each scope range cut into strips of
+    // S consecutive base positions, each strip sorted by the base order's
+    // column 1 (second-stage sort), all columns gathered in that order.
+    int grid_order(int base, int S) {
+        std::string key = "grid";
+        key += std::string((const char*)&base, sizeof base) + std::string((const char*)&S, sizeof S);
+        auto it = order_ids.find(key);
+        if (it != order_ids.end()) return it->second;
+        const OrderDev bo = P.orders[base];
+        OrderDev od = bo;
+        od.c64 = P.n_col64;
+        od.c32 = P.n_col32;
+        od.perm = P.n_perm;
+        od.base_perm = bo.perm;
+        P.n_col64 += od.ncol * od.n;
+        P.n_col32 += od.ncol * od.n;
+        P.n_perm += od.n;
+        const int id = (int)P.orders.size();
+        P.orders.push_back(od);
+        order_ids[key] = id;
+        order_group.push_back(order_group[base]);
+        for (auto rg : ranges(base)) {
+            for (int st = rg.first; st < rg.second; st += S) {
+                const int len = std::min(S, rg.second - st);
+                SortSeg sg;
+                sg.src = bo.c64 + 1 * bo.n + st;  // column 1 of the base order, in col64
+                sg.sbase = P.sort_elems2;
+                sg.len = len;
+                sg.first_tile = P.ntiles2;
+                sg.p0 = st;
+                sg.pad = 0;
+                P.ntiles2 += (len + ENM_SORT_TILE - 1) / ENM_SORT_TILE;
+                P.sort_elems2 += len;
+                P.maxlen2 = std::max<int64_t>(P.maxlen2, len);
+                const int sid = (int)P.segs2.size();
+                P.segs2.push_back(sg);
+                for (int off = 0; off < len; off += ENM_SORT_TILE)
+                    P.gitems2.push_back(GatherItem{id, sid, off, st});
             }
         }
         return id;
@@ -315,6 +359,10 @@ int plan_batch(enm_ctx* ctx, Plan& P, const enm_instance* inst, int n_inst, cons
             acols.insert(acols.end(), Y.begin(), Y.end());
         }
         A = pl.order(ii, acols, segA);
+        // The kNN runs on a grid order (strips of the key-sorted order, each
+        // sorted by the second column) when there are two or more columns.
+        const bool grid = !P.brute && ctx->strip > 0 && acols.size() >= 2;
+        const int K = grid ? pl.grid_order(A, ctx->strip) : A;
 
         struct SpaceSpec { int order; std::vector<int> cols; bool segmented; };
         std::vector<SpaceSpec> spaces;
@@ -343,13 +391,17 @@ int plan_batch(enm_ctx* ctx, Plan& P, const enm_instance* inst, int n_inst, cons
             spaces.push_back({A, z, true});
             spaces.push_back({B, z, false});
         }
-        for (auto& s : spaces) if (s.order != A) need_orig = true;
+        for (auto& s : spaces) if (s.order != K) need_orig = true;
+        // Entropy sums log radii in point order, whatever the kNN order.
+        if (I.kind == ENM_KIND_KL) need_orig = true;
 
         // kNN descriptor and items.
         KnnDev kd;
         std::memset(&kd, 0, sizeof(kd));
-        kd.order = A;
+        kd.order = K;
         kd.inst = ii;
+        kd.strip = grid ? ctx->strip : 0;
+        kd.base_key = P.orders[A].c64;
         kd.k = I.k;
         kd.kl = I.kind == ENM_KIND_KL;
         kd.eps = P.n_res64;
@@ -365,7 +417,9 @@ int plan_batch(enm_ctx* ctx, Plan& P, const enm_instance* inst, int n_inst, cons
         if (pcl < 0) return fail(ctx, ENM_E_UNSUPPORTED, "k above 31 is not supported by the register lists");
         int ktiles = 0;
         std::vector<PendingItem> mine;
-        for (auto rg : pl.ranges(A)) {
+        for (auto rg : pl.ranges(K)) {
+            // Tiles never straddle a strip: strips start at lo + s*strip and
+            // the strip size is a multiple of the tile size.
             for (int t0 = rg.first; t0 < rg.second; t0 += ENM_TILE) {
                 PendingItem pi;
                 pi.it = Item{kdi, ktiles++, rg.first, rg.second, t0, std::min(t0 + ENM_TILE, rg.second)};
@@ -374,18 +428,12 @@ int plan_batch(enm_ctx* ctx, Plan& P, const enm_instance* inst, int n_inst, cons
                 mine.push_back(pi);
             }
         }
-        if (kd.kl) {
-            kd.part = P.n_res64;
-            P.n_res64 += ktiles;
-            id.nspace = 1;
-            id.part[0] = kd.part;
-            id.zero[0] = -1;
-            id.ntile[0] = ktiles;
-        }
+        id.eps_orig = kd.eps_orig;
+        if (kd.kl) id.nspace = 0;
         P.kdescs.push_back(kd);
         kpend.insert(kpend.end(), mine.begin(), mine.end());
         ex.eps = kd.eps;
-        ex.permA = P.orders[A].perm;
+        ex.permA = P.orders[K].perm;
 
         // Count descriptors and items.
         if (!kd.kl) id.nspace = (int)spaces.size();
@@ -397,7 +445,7 @@ int plan_batch(enm_ctx* ctx, Plan& P, const enm_instance* inst, int n_inst, cons
             cd.inst = ii;
             cd.ncol = (int)sp.cols.size();
             for (int c = 0; c < cd.ncol; ++c) cd.cols[c] = sp.cols[c];
-            cd.same_order = sp.order == A;
+            cd.same_order = sp.order == K;
             cd.eps = cd.same_order ? kd.eps : kd.eps_orig;
             cd.eps_perm = P.orders[sp.order].perm;
             cd.counts = P.n_res32;
@@ -408,10 +456,12 @@ int plan_batch(enm_ctx* ctx, Plan& P, const enm_instance* inst, int n_inst, cons
             std::vector<std::pair<int, int>> rgs;
             if (sp.segmented) rgs = pl.ranges(sp.order);
             else rgs.emplace_back(0, (int)I.n);
+            // 1-D counts take ENM_C1_ITEM positions per item, multi-D ENM_TILE.
+            const int step = ccl == 1 ? ENM_C1_ITEM : ENM_TILE;
             for (auto rg : rgs) {
-                for (int t0 = rg.first; t0 < rg.second; t0 += ENM_TILE) {
+                for (int t0 = rg.first; t0 < rg.second; t0 += step) {
                     PendingItem pi;
-                    pi.it = Item{cdi, ctiles++, rg.first, rg.second, t0, std::min(t0 + ENM_TILE, rg.second)};
+                    pi.it = Item{cdi, ctiles++, rg.first, rg.second, t0, std::min(t0 + step, rg.second)};
                     pi.cost = P.brute ? (int64_t)(rg.second - rg.first) : (int64_t)I.n;
                     pi.inst = ii; pi.dclass = ccl; pi.pclass = 0;
                     cpend.push_back(pi);
@@ -489,6 +539,8 @@ int allocate_and_upload(enm_ctx* ctx, Plan& P, hipStream_t st) {
     P.off_cls = append(blob, P.cls);
     P.off_kitems = append(blob, P.kitems);
     P.off_citems = append(blob, P.citems);
+    P.off_segs2 = append(blob, P.segs2);
+    P.off_gitems2 = append(blob, P.gitems2);
     P.table_bytes = blob.size();
     hipError_t e;
     if (P.upload_done) {
@@ -510,10 +562,11 @@ int allocate_and_upload(enm_ctx* ctx, Plan& P, hipStream_t st) {
     };
     int rc;
     if ((rc = need(P.tables, P.table_bytes, "tables"))) return rc;
-    if ((rc = need(P.sk0, P.sort_elems * 8, "sort keys"))) return rc;
-    if ((rc = need(P.sk1, P.sort_elems * 8, "sort keys"))) return rc;
-    if ((rc = need(P.si0, P.sort_elems * 4, "sort index"))) return rc;
-    if ((rc = need(P.si1, P.sort_elems * 4, "sort index"))) return rc;
+    const int64_t se = std::max(P.sort_elems, P.sort_elems2);
+    if ((rc = need(P.sk0, se * 8, "sort keys"))) return rc;
+    if ((rc = need(P.sk1, se * 8, "sort keys"))) return rc;
+    if ((rc = need(P.si0, se * 4, "sort index"))) return rc;
+    if ((rc = need(P.si1, se * 4, "sort index"))) return rc;
     if ((rc = need(P.col64, P.n_col64 * 8, "columns"))) return rc;
     if ((rc = need(P.col32, P.n_col32 * 4, "columns"))) return rc;
     if ((rc = need(P.perm, P.n_perm * 4, "permutations"))) return rc;
@@ -579,6 +632,22 @@ int enqueue(enm_ctx* ctx, Plan& P, const double* arena, double* out, double* d_e
                                     sidx, P.col64.as<double>(), P.col32.as<float>(), P.perm.as<int>(),
                                     P.maxabs.as<unsigned long long>(), &nl, st), "gather")))
         return rc;
+    if (!P.segs2.empty()) {
+        // Second stage: sort each grid strip by the base order's column 1
+        // (keys read from col64), then gather the grid orders.
+        if ((rc = chk(enm_launch_sort(tab<SortSeg>(P, P.off_segs2), (int)P.segs2.size(), P.ntiles2, P.maxlen2,
+                                      P.col64.as<double>(), P.sk0.as<double>(), P.si0.as<int>(),
+                                      P.sk1.as<double>(), P.si1.as<int>(), &fb, &nl, st), "sort")))
+            return rc;
+        const double* skey2 = fb ? P.sk1.as<double>() : P.sk0.as<double>();
+        const int* sidx2 = fb ? P.si1.as<int>() : P.si0.as<int>();
+        if ((rc = chk(enm_launch_gather(tab<GatherItem>(P, P.off_gitems2), (int)P.gitems2.size(),
+                                        tab<SortSeg>(P, P.off_segs2), tab<OrderDev>(P, P.off_orders), arena,
+                                        skey2, sidx2, P.col64.as<double>(), P.col32.as<float>(),
+                                        P.perm.as<int>(), P.maxabs.as<unsigned long long>(), &nl, st),
+                      "gather")))
+            return rc;
+    }
     if ((rc = mark(3))) return rc;
     const OrderDev* orders = tab<OrderDev>(P, P.off_orders);
     const unsigned long long* maxabs = P.maxabs.as<unsigned long long>();
@@ -730,7 +799,7 @@ int enm_run_batch(enm_ctx* ctx, const enm_instance* inst, int32_t n_inst, const 
     ctx->last_stream = st;
     const bool use_graph = (flags & ENM_F_GRAPH) != 0;
     const std::string key = plan_key(inst, n_inst, seg_len, d_arena, d_out, d_eps, d_counts,
-                                     flags & ~(uint32_t)ENM_F_TIMING, ctx->q_per_lane, ctx->waves_per_cu);
+                                     flags & ~(uint32_t)ENM_F_TIMING, ctx->strip, ctx->waves_per_cu);
     // Reuse a cached plan for an identical call.
     for (auto it = ctx->cache.begin(); it != ctx->cache.end(); ++it) {
         if ((*it)->key == key) {
@@ -801,7 +870,7 @@ int enm_get_stats(enm_ctx* ctx, enm_stats* out) {
     out->count_pairs = h[ST_CNT_PAIRS];
     out->count_rechecks = h[ST_CNT_RECHECK];
     out->count_queries_1d = h[ST_CNT_1D];
-    out->sort_keys = (uint64_t)ctx->last->sort_elems;
+    out->sort_keys = (uint64_t)(ctx->last->sort_elems + ctx->last->sort_elems2);
     out->n_launches = (uint32_t)ctx->last->n_launch;
     out->graph_replayed = ctx->last->graph != nullptr && ctx->last->n_launch == 0;
     if (ctx->last->timing) {
@@ -819,19 +888,19 @@ int enm_get_stats(enm_ctx* ctx, enm_stats* out) {
     return ENM_OK;
 }
 
-int enm_set_tuning(enm_ctx* ctx, int32_t queries_per_lane, int32_t waves_per_cu) {
+int enm_set_tuning(enm_ctx* ctx, int32_t strip_points, int32_t waves_per_cu) {
     if (!ctx) return ENM_E_ARG;
-    if (queries_per_lane != 1 && queries_per_lane != 2)
-        return fail(ctx, ENM_E_ARG, "queries_per_lane must be 1 or 2");
+    if (strip_points < 0 || strip_points > 65536 || strip_points % ENM_TILE != 0)
+        return fail(ctx, ENM_E_ARG, "strip_points must be 0 or a multiple of 64 up to 65536");
     if (waves_per_cu < 1 || waves_per_cu > 64) return fail(ctx, ENM_E_ARG, "waves_per_cu must be in 1..64");
-    ctx->q_per_lane = queries_per_lane;
+    ctx->strip = strip_points;
     ctx->waves_per_cu = waves_per_cu;
     return ENM_OK;
 }
 
-int enm_get_tuning(const enm_ctx* ctx, int32_t* queries_per_lane, int32_t* waves_per_cu) {
+int enm_get_tuning(const enm_ctx* ctx, int32_t* strip_points, int32_t* waves_per_cu) {
     if (!ctx) return ENM_E_ARG;
-    if (queries_per_lane) *queries_per_lane = ctx->q_per_lane;
+    if (strip_points) *strip_points = ctx->strip;
     if (waves_per_cu) *waves_per_cu = ctx->waves_per_cu;
     return ENM_OK;
 }
@@ -841,33 +910,37 @@ int enm_autotune(enm_ctx* ctx, const enm_instance* inst, int32_t n_inst, const i
     if (!ctx) return ENM_E_ARG;
     hipStream_t st = (hipStream_t)stream;
     hipEvent_t a, b;
-    if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess)
-        return fail(ctx, ENM_E_HIP, "hipEventCreate");
-    const int waves[] = {4, 8, 16, 32};
+    if (hipEventCreate(&a) != hipSuccess) return fail(ctx, ENM_E_HIP, "hipEventCreate");
+    if (hipEventCreate(&b) != hipSuccess) { (void)hipEventDestroy(a); return fail(ctx, ENM_E_HIP, "hipEventCreate"); }
+    const int strips[] = {0, 256, 512, 1024, 2048, 4096};
+    const int waves[] = {8, 16, 32};
     float best = 1e30f;
-    int best_w = ctx->waves_per_cu, best_q = ctx->q_per_lane;
-    const uint32_t fl = flags & ~(uint32_t)ENM_F_GRAPH;
-    for (int q = 1; q <= 1; ++q) {
+    int best_s = ctx->strip, best_w = ctx->waves_per_cu;
+    const uint32_t fl = flags & ~(uint32_t)(ENM_F_GRAPH | ENM_F_TIMING);
+    int rc = ENM_OK;
+    for (int sp : strips) {
         for (int w : waves) {
-            ctx->q_per_lane = q;
+            ctx->strip = sp;
             ctx->waves_per_cu = w;
-            int rc = enm_run_batch(ctx, inst, n_inst, seg_len, d_arena, d_out, nullptr, nullptr, fl, stream);
-            if (rc) { (void)hipEventDestroy(a); (void)hipEventDestroy(b); return rc; }
+            // First call plans and allocates; the second is timed.
+            rc = enm_run_batch(ctx, inst, n_inst, seg_len, d_arena, d_out, nullptr, nullptr, fl, stream);
+            if (rc) break;
             (void)hipEventRecord(a, st);
             rc = enm_run_batch(ctx, inst, n_inst, seg_len, d_arena, d_out, nullptr, nullptr, fl, stream);
             (void)hipEventRecord(b, st);
-            if (rc) { (void)hipEventDestroy(a); (void)hipEventDestroy(b); return rc; }
+            if (rc) break;
             (void)hipEventSynchronize(b);
             float ms = 0.f;
             (void)hipEventElapsedTime(&ms, a, b);
-            if (ms < best) { best = ms; best_w = w; best_q = q; }
+            if (ms < best) { best = ms; best_s = sp; best_w = w; }
         }
+        if (rc) break;
     }
-    ctx->q_per_lane = best_q;
+    ctx->strip = best_s;
     ctx->waves_per_cu = best_w;
     (void)hipEventDestroy(a);
     (void)hipEventDestroy(b);
-    return ENM_OK;
+    return rc;
 }
 
 }  // extern "C"

@@ -21,6 +21,10 @@
 #include <stdint.h>
 
 #define ENM_TILE 64            // queries per work item (one per lane of a wave64)
+#define ENM_C1_Q 8             // queries per lane of a 1-D count item
+#define ENM_C1_ITEM (ENM_TILE * ENM_C1_Q)   // positions per 1-D count item
+#define ENM_C1_HALO 1024       // keys staged on each side of a 1-D count item
+#define ENM_C1_CAP (ENM_C1_ITEM + 2 * ENM_C1_HALO)  // LDS window (doubles, 20 KB)
 #define ENM_SORT_TILE 2048     // elements per block-sort tile and merge chunk
 #define ENM_SORT_THREADS 256
 #define ENM_MAX_SPACES 3
@@ -52,18 +56,28 @@ struct OrderDev {
     int32_t ncol;
     int32_t inst;
     int64_t src[16];    // arena offsets of the instance columns gathered (src[0] = key)
+    // Grid orders are sorted in a second stage, within strips of a base
+    // order: their sort payload is a base-order position, mapped to a point
+    // through the base order's permutation at this perm offset (-1: none).
+    int64_t base_perm;
 };
 
 // kNN over one order (its scope ranges are the class segments, or all points).
+// With strip > 0 the order is a grid order: positions [lo + s*strip,
+// lo + (s+1)*strip) of each scope range [lo, hi) form strip s, whose points
+// are consecutive in the base order (sorted by column 0, whose sorted values
+// start at col64 offset base_key) and are sorted by column 1 inside the strip.
 struct KnnDev {
     int32_t order;
     int32_t inst;
     int32_t k;
-    int32_t kl;         // 1: write sum(log eps) partials per tile (entropy)
+    int32_t kl;         // 1: an entropy instance (radii are summed as logs)
     int64_t eps;        // res64 offset: radius per order position
     int64_t eps_orig;   // res64 offset: radius per instance point (or -1)
-    int64_t part;       // res64 offset: per-tile partial sum (kl only, else -1)
-    int64_t pad;
+    int64_t part;       // unused (-1)
+    int64_t base_key;   // col64 offset of the base order's sorted column 0
+    int32_t strip;      // points per strip (0: plain key-sorted order)
+    int32_t pad;
 };
 
 // Ball count in one space (a subset of an order's columns, key first).
@@ -103,6 +117,7 @@ struct InstDev {
     int64_t zero[ENM_MAX_SPACES];  // res32 offsets of per-tile zero flags
     int32_t ntile[ENM_MAX_SPACES];
     int32_t pad1;
+    int64_t eps_orig;    // res64 offset of the radii in point order (entropy)
 };
 
 // Device counters (enm_stats order).

@@ -13,12 +13,18 @@
 // no LDS is needed.  Each lane keeps an ascending FP64 list of its K1 = k+1
 // smallest distances in registers (P slots, the lowest P-K1 pinned at -inf).
 //
-// Pruning.  Candidates are visited from the tile outward in key order,
-// alternating left and right in steps of 64.  A side stops once every lane's
-// key distance to the last visited candidate is >= its current K1-th
-// distance T: every candidate further out on that side has a key distance,
-// and so a Chebyshev distance, >= T and cannot enter the list.  With
-// ENM_F_BRUTE the whole scope range is scanned instead.
+// Pruning.  With two or more columns the order is a grid: the key-sorted
+// points cut into strips, each strip sorted by the second column, tiles of 64
+// y-consecutive points of one strip (compact in both axes).  A tile visits
+// its own strip and then neighbouring strips outward, each only within the
+// y window of its queries widened by the wave's largest current radius, and
+// stops on a side at the first strip that every lane's radius excludes on the
+// key axis (grid_sweep).  With one column, candidates are visited from the
+// tile outward in key order, alternating left and right in steps of 64; a
+// side stops once every lane's key distance to the last visited candidate is
+// >= its current K1-th distance T.  Either way every candidate that could
+// still enter a list is visited once.  With ENM_F_BRUTE the whole scope range
+// is scanned instead.
 //
 // Precision.  In FP64 mode every distance is exact FP64.  In FP32-filter mode
 // the bulk distance is computed from FP32 copies, and a candidate is
@@ -77,9 +83,14 @@ struct KnnTile {
 
     __device__ __forceinline__ void chunk(int j) {
         if (F32) {
+            // The FP64 coordinates of the chunk are loaded with the FP32 ones
+            // (wave-uniform, scalar), so a recheck does not wait on a load.
+            double e64[D][CH];
             float dd[CH];
 #pragma unroll
             for (int c = 0; c < CH; ++c) {
+#pragma unroll
+                for (int d = 0; d < D; ++d) e64[d][c] = c64[d][j + c];
                 float m = fabsf(c32[0][j + c] - q32[0]);
 #pragma unroll
                 for (int d = 1; d < D; ++d) m = fmaxf(m, fabsf(c32[d][j + c] - q32[d]));
@@ -89,9 +100,39 @@ struct KnnTile {
 #pragma unroll
             for (int c = 1; c < CH; ++c) mn = fminf(mn, dd[c]);
             if (wave_any(mn <= thr)) {
+                if constexpr (P <= 8) {
+                    // Exact distances of all 8 candidates, merged into the
+                    // list by a network.  A candidate the filter rejected has
+                    // an FP64 distance > T, so merging it changes nothing.
+                    double e[CH];
 #pragma unroll
-                for (int c = 0; c < CH; ++c)
-                    if (dd[c] <= thr) recheck(j + c);
+                    for (int c = 0; c < CH; ++c) {
+                        double v = fabs(e64[0][c] - q64[0]);
+#pragma unroll
+                        for (int d = 1; d < D; ++d) v = fmax(v, fabs(e64[d][c] - q64[d]));
+                        e[c] = v;
+                        rechecks += dd[c] <= thr;
+                    }
+                    sort8(e);
+                    list_merge8<P>(L, e);
+                    T = L[P - 1];
+                    set_thr();
+                } else {
+#pragma unroll
+                    for (int c = 0; c < CH; ++c) {
+                        if (dd[c] <= thr) {
+                            double e = fabs(e64[0][c] - q64[0]);
+#pragma unroll
+                            for (int d = 1; d < D; ++d) e = fmax(e, fabs(e64[d][c] - q64[d]));
+                            ++rechecks;
+                            if (e < T) {
+                                list_insert<P>(L, e);
+                                T = L[P - 1];
+                                set_thr();
+                            }
+                        }
+                    }
+                }
             }
         } else {
             double dd[CH];
@@ -106,11 +147,17 @@ struct KnnTile {
 #pragma unroll
             for (int c = 1; c < CH; ++c) mn = fmin(mn, dd[c]);
             if (wave_any(mn < T)) {
+                if constexpr (P <= 8) {
+                    sort8(dd);
+                    list_merge8<P>(L, dd);
+                    T = L[P - 1];
+                } else {
 #pragma unroll
-                for (int c = 0; c < CH; ++c) {
-                    if (dd[c] < T) {
-                        list_insert<P>(L, dd[c]);
-                        T = L[P - 1];
+                    for (int c = 0; c < CH; ++c) {
+                        if (dd[c] < T) {
+                            list_insert<P>(L, dd[c]);
+                            T = L[P - 1];
+                        }
                     }
                 }
             }
@@ -141,6 +188,91 @@ struct KnnTile {
         for (; j < b; ++j) single(j);
     }
 };
+
+// Positions [a, b) of the strip [s0, s1) of a y-sorted column whose values
+// lie within R of [ylo, yhi], widened by a relative 2^-50 so the range is a
+// superset of every candidate with fl(|y_j - y_i|) < R for y_i in [ylo, yhi]
+// (each rounding of the bounds is far below the widening).
+__device__ __forceinline__ void y_window(const double* __restrict__ yc, int s0, int s1, double ylo,
+                                         double yhi, double R, int& a, int& b) {
+    const double slack = 0x1p-50 * (fmax(fabs(ylo), fabs(yhi)) + R);
+    const double lb = uni_f64((ylo - R * (1.0 + 0x1p-50)) - slack);
+    const double ub = uni_f64((yhi + R * (1.0 + 0x1p-50)) + slack);
+    a = uni(lower_bound_u(yc, s0, s1, lb));
+    b = uni(upper_bound_u(yc, a, s1, ub));
+}
+
+// Grid sweep of one tile (already seeded with its own 64 points): the rest
+// of the tile's strip near the tile in y order, then whole strips outward on
+// both sides.  A strip is visited only within the y window of the tile's
+// y range widened by R = the largest current radius of the wave's lanes; a
+// side stops at the first strip whose x bound is at least every lane's
+// radius away (all strips beyond it are further in x).  Each candidate is
+// visited at most once, and every candidate that could still enter a lane's
+// list is visited.  Returns the number of candidates visited.
+template <int D, int P, bool F32>
+__device__ int grid_sweep(KnnTile<D, P, F32>& w, const KnnDev& kd, const double* __restrict__ col64,
+                          int lo, int hi, int t0, int t1, bool active) {
+    constexpr int SEED = 64;
+    const int S = kd.strip;
+    const double* xk = col64 + kd.base_key;  // base order: column 0 sorted over [lo, hi)
+    const double* yc = w.c64[1];             // column 1, sorted inside each strip
+    const int s = (t0 - lo) / S;
+    const int nstrip = (hi - lo + S - 1) / S;
+    const int ss = lo + s * S, se = min(hi, ss + S);
+    int visited = 0;
+    // Seed: up to SEED neighbours in y on each side of the tile.
+    const int a0 = max(ss, t0 - SEED), b0 = min(se, t1 + SEED);
+    w.range(a0, t0);
+    w.range(t1, b0);
+    visited += (t0 - a0) + (b0 - t1);
+    const double ylo = yc[t0], yhi = yc[t1 - 1];
+    {
+        const double R = wave_max(active ? w.T : -INFINITY);
+        int a, b;
+        y_window(yc, ss, se, ylo, yhi, R, a, b);
+        if (a < a0) { w.range(a, a0); visited += a0 - a; }
+        if (b > b0) { w.range(b0, b); visited += b - b0; }
+    }
+    bool ldone = s == 0, rdone = s == nstrip - 1;
+    for (int d = 1; !(ldone && rdone); ++d) {
+        if (!ldone) {
+            const int sl = s - d;
+            if (sl < 0) {
+                ldone = true;
+            } else {
+                const int le = lo + (sl + 1) * S;  // strip sl is [le - S, le)
+                const double xb = xk[le - 1];      // largest x of strip sl
+                ldone = wave_all(!(w.q64[0] - xb < w.T));
+                if (!ldone) {
+                    const double R = wave_max(active ? w.T : -INFINITY);
+                    int a, b;
+                    y_window(yc, le - S, le, ylo, yhi, R, a, b);
+                    w.range(a, b);
+                    visited += b - a;
+                }
+            }
+        }
+        if (!rdone) {
+            const int sr = s + d;
+            if (sr >= nstrip) {
+                rdone = true;
+            } else {
+                const int rs = lo + sr * S;
+                const double xb = xk[rs];          // smallest x of strip sr
+                rdone = wave_all(!(xb - w.q64[0] < w.T));
+                if (!rdone) {
+                    const double R = wave_max(active ? w.T : -INFINITY);
+                    int a, b;
+                    y_window(yc, rs, min(hi, rs + S), ylo, yhi, R, a, b);
+                    w.range(a, b);
+                    visited += b - a;
+                }
+            }
+        }
+    }
+    return visited;
+}
 
 template <int D, int P, bool F32>
 __device__ void knn_item(const KnnArgs& A, const Item& it, const double* __restrict__ col64,
@@ -177,7 +309,15 @@ __device__ void knn_item(const KnnArgs& A, const Item& it, const double* __restr
     const int lo = it.lo, hi = it.hi, t0 = it.t0, t1 = it.t1;
     int visited = t1 - t0;
     w.range(t0, t1);
-    if (A.brute) {
+    bool swept = false;
+    if constexpr (D >= 2) {
+        if (kd.strip > 0 && !A.brute) {
+            visited += grid_sweep(w, kd, col64, lo, hi, t0, t1, active);
+            swept = true;
+        }
+    }
+    if (swept) {
+    } else if (A.brute) {
         w.range(lo, t0);
         w.range(t1, hi);
         visited = hi - lo;
@@ -207,10 +347,6 @@ __device__ void knn_item(const KnnArgs& A, const Item& it, const double* __restr
     if (active) {
         res64[kd.eps + i] = eps;
         if (kd.eps_orig >= 0) res64[kd.eps_orig + A.perm[od.perm + i]] = eps;
-    }
-    if (kd.kl) {
-        const double s = wave_sum(active ? log(eps) : 0.0);
-        if (lane == 0) res64[kd.part + it.tile] = s;
     }
     const unsigned long long nact = (unsigned long long)(t1 - t0);
     const unsigned long long rc = wave_sum_u64(w.rechecks);

@@ -4,7 +4,8 @@
 //
 // One wave per instance sums the partials of each space with a fixed lane
 // assignment and butterfly, so the result does not depend on the order in
-// which the work queue ran the tiles.  The formulas and the psi(0) = +inf
+// which the work queue ran the tiles, nor (as count spaces live in key-sorted
+// orders and entropy radii are read in point order) on the kNN tiling.  The formulas and the psi(0) = +inf
 // broadcasting follow ennemi/_entropy_estimators.py:
 //   KSG   :113  psi(N) + psi(k) - mean(psi(nx) + psi(ny))
 //   CMI   :156  psi(k) - mean(psi(nxz) + psi(nyz) - psi(nz))
@@ -38,6 +39,14 @@ void combine_kernel(const InstDev* __restrict__ insts, const double* __restrict_
         }
         sum[s] = wave_sum(acc);
         zero[s] = wave_or(z);
+    }
+    // Entropy: mean of log radii over the points in point order (lane-strided
+    // then butterfly, so the sum does not depend on the kNN order or tuning).
+    double logsum = 0.0;
+    if (in.kind == ENM_KIND_KL) {
+        double acc = 0.0;
+        for (int64_t i = lane; i < in.n; i += ENM_WAVE) acc += log(res64[in.eps_orig + i]);
+        logsum = wave_sum(acc);
     }
     if (lane != 0) return;
     const double N = (double)in.n;
@@ -73,7 +82,7 @@ void combine_kernel(const InstDev* __restrict__ insts, const double* __restrict_
         }
         case ENM_KIND_KL:
         default: {
-            value = psi_int(in.n) - psik + (double)in.dx * (sum[0] / N + log(2.0));
+            value = psi_int(in.n) - psik + (double)in.dx * (logsum / N + log(2.0));
             break;
         }
     }

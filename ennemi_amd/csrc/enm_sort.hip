@@ -151,7 +151,10 @@ void merge_pass_kernel(const SortSeg* __restrict__ segs, int nsegs, int64_t widt
 
 // Lay one sorted chunk out into an order: permutation, key column and the
 // other gathered columns (FP64 and FP32), and fold |value| into the
-// instance's maximum magnitude (for the FP32 guard band).
+// instance's maximum magnitude (for the FP32 guard band).  For a grid order
+// the sort payload is a base-order position: the point is read through the
+// base permutation and every column, key included, is gathered from the
+// arena.
 __global__ __launch_bounds__(ENM_SORT_THREADS)
 void gather_kernel(const GatherItem* __restrict__ items,
                    const SortSeg* __restrict__ segs,
@@ -159,25 +162,31 @@ void gather_kernel(const GatherItem* __restrict__ items,
                    const double* __restrict__ arena,
                    const double* __restrict__ skey, const int* __restrict__ sidx,
                    double* __restrict__ col64, float* __restrict__ col32,
-                   int* __restrict__ perm, unsigned long long* __restrict__ maxabs) {
+                   int* perm, unsigned long long* __restrict__ maxabs) {
     __shared__ double red[ENM_SORT_THREADS / ENM_WAVE];
     const GatherItem it = items[blockIdx.x];
     const SortSeg sg = segs[it.sseg];
     const OrderDev od = orders[it.order];
     const int len = min(ENM_SORT_TILE, sg.len - it.off);
     const int64_t n = od.n;
+    const bool indirect = od.base_perm >= 0;
     double m = 0.0;
     for (int t = threadIdx.x; t < len; t += ENM_SORT_THREADS) {
         const int64_t sp = sg.sbase + it.off + t;
         const int64_t p = it.pos0 + it.off + t;
-        const double key = skey[sp];
         const int idx = sidx[sp];
-        perm[od.perm + p] = idx;
-        col64[od.c64 + p] = key;
-        col32[od.c32 + p] = (float)key;
-        m = fmax(m, fabs(key));
-        for (int c = 1; c < od.ncol; ++c) {
-            const double v = arena[od.src[c] + idx];
+        const int pt = indirect ? perm[od.base_perm + idx] : idx;
+        perm[od.perm + p] = pt;
+        int c0 = 0;
+        if (!indirect) {
+            const double key = skey[sp];
+            col64[od.c64 + p] = key;
+            col32[od.c32 + p] = (float)key;
+            m = fmax(m, fabs(key));
+            c0 = 1;
+        }
+        for (int c = c0; c < od.ncol; ++c) {
+            const double v = arena[od.src[c] + pt];
             col64[od.c64 + c * n + p] = v;
             col32[od.c32 + c * n + p] = (float)v;
             m = fmax(m, fabs(v));

@@ -152,18 +152,20 @@ int enm_run_batch(enm_ctx* ctx,
 int enm_get_stats(enm_ctx* ctx, enm_stats* out);
 
 /*
- * Autotuning.  Tile shapes of the kNN and count kernels are picked per
- * (joint dimension, list depth) class.  enm_autotune times the candidate
- * shapes on the given batch (a call of enm_run_batch per shape) and keeps
- * the fastest in the context; enm_set_tuning sets one explicitly.
- * queries_per_lane in {1,2}, waves_per_cu in {1..8}.
+ * Autotuning.  The kNN runs on a grid order: the key-sorted points cut into
+ * strips of `strip_points` consecutive points, each strip sorted by the
+ * second column (0 = plain key-sorted order); `waves_per_cu` sizes the
+ * persistent grids.  enm_autotune times strip sizes {0, 256, ..., 4096} x
+ * waves {8, 16, 32} on the given batch (two enm_run_batch calls per shape)
+ * and keeps the fastest in the context; enm_set_tuning sets one explicitly
+ * (strip a multiple of 64 up to 65536, waves in 1..64).
  */
 int enm_autotune(enm_ctx* ctx,
                  const enm_instance* inst, int32_t n_inst,
                  const int64_t* seg_len, const double* d_arena,
                  double* d_out, uint32_t flags, void* stream);
-int enm_set_tuning(enm_ctx* ctx, int32_t queries_per_lane, int32_t waves_per_cu);
-int enm_get_tuning(const enm_ctx* ctx, int32_t* queries_per_lane, int32_t* waves_per_cu);
+int enm_set_tuning(enm_ctx* ctx, int32_t strip_points, int32_t waves_per_cu);
+int enm_get_tuning(const enm_ctx* ctx, int32_t* strip_points, int32_t* waves_per_cu);
 
 #ifdef __cplusplus
 }
