@@ -49,29 +49,6 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def lag_scan_data(length: int, seed: int = 303):
-    """Eight coupled AR(1) series with NaNs in the x columns."""
-    rng = np.random.default_rng(seed)
-    phi = np.linspace(0.3, 0.9, 8)
-    burn = 1000
-    total = length + burn
-    noise = rng.normal(size=(total, 8))
-    s = np.zeros((total, 8))
-    for t in range(1, total):
-        prev = s[t - 1]
-        cur = phi * prev + noise[t]
-        cur[:7] += 0.2 * prev[1:]
-        s[t] = cur
-    s = s[burn:]
-    y = s[:, 0].copy()
-    x = s.copy()
-    for j in range(8):
-        frac = rng.uniform(0.0, 0.3)
-        rows = rng.choice(length, int(frac * length), replace=False)
-        x[rows, j] = np.nan
-    return y, x
-
-
 def time_batch(db, steps: int, warmup: int, stream):
     """Run warmup + timed steps; return (ms/step wall, per-step stats list)."""
     import torch
@@ -89,6 +66,61 @@ def time_batch(db, steps: int, warmup: int, stream):
     return wall, stats
 
 
+def config_batch(name: str):
+    """(batch, description) of a non-headline configuration, planned through
+    the same host code as the public API."""
+    import ennemi_amd._native as nat
+    from ennemi_amd._batch import Batch
+    from ennemi_amd._driver import plan_mi, plan_pairwise_mi
+    from ennemi_amd import _estimators as est
+    from tests import workloads as wl
+    if name == "c1":
+        x, y = wl.c1()
+        return plan_mi(y, x, k=3)[0], "KSG N=10,000 rho=0.6 k=3"
+    if name == "c2":
+        x, y, z = wl.c2_cmi()
+        b = plan_mi(y, x, k=3, cond=z)[0]
+        e, _ = wl.c2_entropy()
+        b.add(nat.KIND_KL, 3, [np.ascontiguousarray(e[:, c]) for c in range(4)])
+        return b, "CMI N=100,000 z 3-D k=3 + KL entropy 4-D N=100,000"
+    if name == "c3":
+        y, x = wl.lag_scan(200_000)
+        return plan_mi(y, x, lag=np.arange(-48, 49), k=3, drop_nan=True)[0], \
+            "masked lag scan, series length 200,000 (776 KSG estimates)"
+    if name == "c4":
+        return plan_pairwise_mi(wl.c4(), k=3)[0], "pairwise_mi 64 vars x 50,000 (2,016 pairs)"
+    if name == "c5a":
+        x, y, z = wl.c5a()
+        return plan_mi(y, x, k=5, cond=z)[0], "CMI N=2,000,000 z 2-D k=5"
+    if name == "c5b":
+        x, yc = wl.c5b()
+        values, counts, order, grouped = est._grouped(yc, [np.ascontiguousarray(x[:, c]) for c in range(3)])
+        b = Batch()
+        b.add(nat.KIND_ROSS, 3, grouped, class_sizes=counts)
+        return b, "Ross N=1,000,000 x 3-D, 16 Zipf(1.1) classes, k=3 (estimator level)"
+    raise ValueError(name)
+
+
+def run_config(name: str, steps: int) -> dict:
+    import ennemi_amd._native as nat
+    t0 = time.perf_counter()
+    batch, desc = config_batch(name)
+    prep = time.perf_counter() - t0
+    out = {"config": name, "workload": desc, "n_estimates": len(batch), "host_prep_s": prep}
+    for mode, flags in (("fp32_filter", 0), ("fp64", nat.F_FP64)):
+        db = batch.device(flags | nat.F_TIMING)
+        w, st = time_batch(db, steps, 1, db.stream)
+        s = st[-1]
+        out[mode] = {"ms_per_step": round(w, 3), "estimates_per_s": len(batch) / (w * 1e-3),
+                     "stage_ms": {k: round(float(np.mean([x["stage_ms"][k] for x in st])), 3)
+                                  for k in nat.STAGES},
+                     "knn_pairs": s["knn_pairs"], "count_pairs": s["count_pairs"],
+                     "count_queries_1d": s["count_queries_1d"], "sort_keys": s["sort_keys"],
+                     "values": db.results()[:4].tolist()}
+        del db
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -98,6 +130,9 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-compare", action="store_true")
     ap.add_argument("--cpu-lags", type=int, default=12, help="lags in the CPU baseline sample")
+    ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--config", default=None,
+                    help="time another configuration instead (c1 c2 c3 c4 c5a c5b); one JSON line each")
     ap.add_argument("--copy-calib", action="store_true",
                     help="first run a 1 GiB device-to-device copy (PMC byte-counter calibration)")
     args = ap.parse_args()
@@ -111,6 +146,11 @@ def main() -> int:
     import ennemi_amd._native as nat
     from ennemi_amd._driver import plan_mi
 
+    if args.config:
+        for name in args.config.split(","):
+            print(json.dumps(run_config(name, max(1, args.steps))), flush=True)
+        return 0
+
     if args.copy_calib:
         # A plain copy of known size: its FETCH_SIZE / WRITE_SIZE per dispatch
         # give the scale of those counters (KiB per dispatch) on this device.
@@ -121,8 +161,9 @@ def main() -> int:
         log("copy calibration: 1 GiB read, 1 GiB written")
         del src, dst
 
+    from tests.workloads import lag_scan
     lags = np.arange(-48, 49)
-    y, x = lag_scan_data(args.length)
+    y, x = lag_scan(args.length)
     t0 = time.perf_counter()
     batch, indices = plan_mi(y, x, lag=lags, k=3, drop_nan=True)
     host_prep_s = time.perf_counter() - t0
@@ -134,6 +175,11 @@ def main() -> int:
     t0 = time.perf_counter()
     db = batch.device(nat.F_TIMING)
     upload_s = time.perf_counter() - t0
+    tuning = db.ctx.get_tuning()
+    if not args.no_autotune:
+        t0 = time.perf_counter()
+        tuning = db.autotune()
+        log(f"autotuned strip={tuning[0]} waves/CU={tuning[1]} in {time.perf_counter() - t0:.1f} s")
     wall, stats = time_batch(db, args.steps, args.warmup, db.stream)
     values = db.results()
     st = stats[-1]
@@ -234,7 +280,8 @@ def main() -> int:
                                "lags -48..48, k=3, drop_nan (776 KSG estimates, N~1e5)",
                    "n_estimates": n_est, "n_min": int(ns.min()), "n_max": int(ns.max()),
                    "mode": "batched persistent work queue, FP32 filter + FP64 recheck, "
-                           "sorted-window pruning"},
+                           "grid-window pruning",
+                   "tuning": {"strip_points": tuning[0], "waves_per_cu": tuning[1]}},
         "roofline": roofline,
         "cpu_baseline": cpu,
         "comparators": comparators,

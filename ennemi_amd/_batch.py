@@ -131,6 +131,15 @@ class DeviceBatch:
         with self.ctx.lock:
             return self.ctx.stats()
 
+    def autotune(self) -> Tuple[int, int]:
+        """Time the library's tile shapes on this batch and keep the fastest
+        (in the context, for every later batch); returns (strip, waves/CU)."""
+        with self.ctx.lock:
+            self.ctx.autotune(self.inst_arr, self.n_inst, self.seg_arr, self.arena.data_ptr(),
+                              self.out.data_ptr(), self.flags, self.stream.cuda_stream)
+            self.stream.synchronize()
+            return self.ctx.get_tuning()
+
     def results(self):
         self.stream.synchronize()
         values = self.out.cpu().numpy().copy()

@@ -450,6 +450,34 @@ def pairwise_corr(data, *, k: int = 3, cond=None, mask=None, preprocess: bool = 
 
 
 def _pairwise_mi(data, k, cond, preprocess, drop_nan, mask, discrete, callback):
+    tasks, indices, nvar = _plan_pairwise(data, k, cond, preprocess, drop_nan, mask, discrete)
+    values = tasks.run()
+    result = np.full((nvar, nvar), np.nan)
+    for (i, j), res in zip(indices, values):
+        result[i, j] = res
+        result[j, i] = res
+    if callback is not None:
+        for (i, j) in indices:
+            callback(i, j)
+    return result
+
+
+def plan_pairwise_mi(data, *, k: int = 3, cond=None, mask=None, discrete=False,
+                     preprocess: bool = True, drop_nan: bool = False):
+    """Validate and rescale every pair of a ``pairwise_mi`` call and return
+    its batch of GPU instances without running it.  Returns (batch, pairs)."""
+    data_arr = np.asarray(data)
+    cond_arr = np.column_stack((np.asarray(cond),)) if cond is not None else None
+    mask_arr = np.asarray(mask) if mask is not None else None
+    discrete_arr = np.broadcast_to(discrete, data_arr.shape[1])
+    tasks, indices, _ = _plan_pairwise(data_arr, k, cond_arr, preprocess, drop_nan, mask_arr,
+                                       discrete_arr)
+    if tasks.host_values:
+        raise ValueError("plan_pairwise_mi covers the GPU estimators only")
+    return tasks.batch, indices
+
+
+def _plan_pairwise(data, k, cond, preprocess, drop_nan, mask, discrete):
     _check_parameters(data, None, k, cond, mask)
     if (not (np.all(discrete) or np.all(~discrete))) and (cond is not None):
         raise ValueError("Conditioning is not supported with mixed discrete and continuous data. " +
@@ -500,16 +528,7 @@ def _pairwise_mi(data, k, cond, preprocess, drop_nan, mask, discrete, callback):
             _add_mi_task(tasks, xs, ys, zs, k, dx, dy)
             continue
         _add_mi_task(tasks, xs, ys, None, k, dx, dy)
-    values = tasks.run()
-
-    result = np.full((nvar, nvar), np.nan)
-    for (i, j), res in zip(indices, values):
-        result[i, j] = res
-        result[j, i] = res
-    if callback is not None:
-        for (i, j) in indices:
-            callback(i, j)
-    return result
+    return tasks, indices, nvar
 
 
 # ---------------------------------------------------------------------------

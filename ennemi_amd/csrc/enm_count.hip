@@ -15,13 +15,18 @@
 // per query.  In every mode (including ENM_F_BRUTE, which only removes window
 // pruning) 1-D counts use this search.
 //
-// Multi-D spaces: a wave takes 64 consecutive queries, bounds the union of
-// their balls on the key axis, binary-searches that window in the sorted key
-// column (wave-uniform), and scans it with wave-uniform candidate loads.  In
-// FP32-filter mode each lane counts candidates whose FP32 distance is <= the
-// radius rounded down minus the guard (certainly inside) and <= the radius
-// rounded up plus the guard (possibly inside); if any lane saw a candidate in
-// between, the wave rescans its window and settles those in FP64.
+// Multi-D spaces: on a grid order (the space's first two columns are the
+// order's columns 0 and 1) a wave takes 64 queries of one strip and scans its
+// own strip and the strips outward, each within the y window of the tile
+// widened by the wave's largest radius, in FP64 (count_grid).  Otherwise the
+// wave bounds the union of its 64 balls on the key axis and scans that window
+// of the key-sorted order with wave-uniform candidate loads; in FP32-filter
+// mode each lane counts candidates whose FP32 distance is <= the radius
+// rounded down minus the guard (certainly inside) and <= the radius rounded
+// up plus the guard (possibly inside), and if any lane saw a candidate in
+// between the wave rescans its window and settles those in FP64.  (On this
+// ISA FP32 and FP64 compares issue at the same rate, so the grid path counts
+// in FP64 directly: the band bookkeeping would cost more than it saves.)
 //
 // Each item also writes the sum over its 64 queries of psi(n_i) and a flag
 // for n_i == 0 (psi(0) = +inf in the reference, :338-339).
@@ -51,8 +56,13 @@ __device__ __forceinline__ double query_radius(const CountDev& cd, const CountAr
 
 __device__ __forceinline__ void finish_item(const CountDev& cd, const Item& it, bool active,
                                             int cnt, int* __restrict__ res32,
-                                            double* __restrict__ res64, int i) {
+                                            double* __restrict__ res64, int i, const int* perm) {
     if (active) res32[cd.counts + i] = cnt;
+    if (cd.psi_pt >= 0) {
+        // Point order; a zero count is +inf, as psi(0) poisons the mean.
+        if (active) res64[cd.psi_pt + perm[cd.eps_perm + i]] = cnt > 0 ? enm_psi((double)cnt) : INFINITY;
+        return;
+    }
     const double p = (active && cnt > 0) ? enm_psi((double)cnt) : 0.0;
     const double s = wave_sum(p);
     const int z = wave_or(active && cnt == 0);
@@ -220,6 +230,99 @@ void count1d_kernel(CountArgs A, const double* __restrict__ col64, double* __res
     }
 }
 
+// Count of candidates j in [a, b) with max_d fl64(|c_d[j] - q_d|) <= r, for
+// one lane (wave-uniform candidate range).
+template <int D>
+__device__ __forceinline__ int count_range64(const double* const (&c64)[D], const double (&q64)[D],
+                                             double r, int a, int b) {
+    int cnt = 0;
+    int j = a;
+    for (; j + CH <= b; j += CH) {
+#pragma unroll
+        for (int c = 0; c < CH; ++c) {
+            double m = fabs(c64[0][j + c] - q64[0]);
+#pragma unroll
+            for (int d = 1; d < D; ++d) m = fmax(m, fabs(c64[d][j + c] - q64[d]));
+            cnt += (m <= r);
+        }
+    }
+    for (; j < b; ++j) {
+        double m = fabs(c64[0][j] - q64[0]);
+#pragma unroll
+        for (int d = 1; d < D; ++d) m = fmax(m, fabs(c64[d][j] - q64[d]));
+        cnt += (m <= r);
+    }
+    return cnt;
+}
+
+// Grid count: the tile's own strip and then strips outward, each within the
+// y window of the tile's column-1 range widened by the wave's largest radius
+// R.  A side stops at the first strip whose column-0 bound is more than every
+// live lane's radius away (strictly: a count includes distance == r), and
+// all strips beyond it are further still.  Returns candidates visited.
+template <int D>
+__device__ int count_grid(const CountDev& cd, const double* __restrict__ col64,
+                          const double* const (&c64)[D], const double (&q64)[D], double r, bool live,
+                          int lo, int hi, int t0, int t1, int& cnt) {
+    const int S = cd.strip;
+    const double* xk = col64 + cd.base_key;
+    const double* yc = c64[1];
+    const int s = (t0 - lo) / S;
+    const int nstrip = (hi - lo + S - 1) / S;
+    const int ss = lo + s * S, se = min(hi, ss + S);
+    const double R = uni_f64(wave_max(live ? r : -INFINITY));
+    if (!(R >= 0.0)) return 0;  // no live lane
+    const double ylo = yc[t0], yhi = yc[t1 - 1];
+    int visited = 0;
+    int a, b;
+    {
+        const double slack = 0x1p-50 * (fmax(fabs(ylo), fabs(yhi)) + R);
+        const double lb = uni_f64((ylo - R * (1.0 + 0x1p-50)) - slack);
+        const double ub = uni_f64((yhi + R * (1.0 + 0x1p-50)) + slack);
+        a = uni(lower_bound_u(yc, ss, se, lb));
+        b = uni(upper_bound_u(yc, a, se, ub));
+        cnt += count_range64<D>(c64, q64, r, a, b);
+        visited += b - a;
+        bool ldone = s == 0, rdone = s == nstrip - 1;
+        for (int d = 1; !(ldone && rdone); ++d) {
+            if (!ldone) {
+                const int sl = s - d;
+                if (sl < 0) {
+                    ldone = true;
+                } else {
+                    const int le = lo + (sl + 1) * S;
+                    const double xb = xk[le - 1];
+                    ldone = wave_all(!live || (q64[0] - xb > r));
+                    if (!ldone) {
+                        a = uni(lower_bound_u(yc, le - S, le, lb));
+                        b = uni(upper_bound_u(yc, a, le, ub));
+                        cnt += count_range64<D>(c64, q64, r, a, b);
+                        visited += b - a;
+                    }
+                }
+            }
+            if (!rdone) {
+                const int sr = s + d;
+                if (sr >= nstrip) {
+                    rdone = true;
+                } else {
+                    const int rs = lo + sr * S;
+                    const int re = min(hi, rs + S);
+                    const double xb = xk[rs];
+                    rdone = wave_all(!live || (xb - q64[0] > r));
+                    if (!rdone) {
+                        a = uni(lower_bound_u(yc, rs, re, lb));
+                        b = uni(upper_bound_u(yc, a, re, ub));
+                        cnt += count_range64<D>(c64, q64, r, a, b);
+                        visited += b - a;
+                    }
+                }
+            }
+        }
+    }
+    return visited;
+}
+
 template <int D, bool F32>
 __device__ void countnd_item(const CountArgs& A, const Item& it, const CountDev& cd,
                              const double* __restrict__ col64, const float* __restrict__ col32,
@@ -247,6 +350,17 @@ __device__ void countnd_item(const CountArgs& A, const Item& it, const CountDev&
     double r = active ? query_radius(cd, A, res64, i) : -1.0;
     const bool live = active && !(r < 0.0);
     if (!live) r = -INFINITY;
+
+    if constexpr (D >= 2) {
+        if (cd.strip > 0) {
+            int cnt = 0;
+            const int visited = count_grid<D>(cd, col64, c64, q64, r, live, it.lo, it.hi, it.t0, it.t1, cnt);
+            finish_item(cd, it, active, cnt, res32, res64, i, A.perm);
+            if (threadIdx.x == 0)
+                atomicAdd(stats + ST_CNT_PAIRS, (unsigned long long)(it.t1 - it.t0) * (unsigned long long)visited);
+            return;
+        }
+    }
 
     // Window on the key axis (a superset of every live lane's ball).
     int w0 = it.lo, w1 = it.hi;
@@ -323,7 +437,7 @@ __device__ void countnd_item(const CountArgs& A, const Item& it, const CountDev&
             cnt += (m <= r);
         }
     }
-    finish_item(cd, it, active, cnt, res32, res64, i);
+    finish_item(cd, it, active, cnt, res32, res64, i, A.perm);
     if (threadIdx.x == 0)
         atomicAdd(stats + ST_CNT_PAIRS, (unsigned long long)(it.t1 - it.t0) * (unsigned long long)(w1 - w0));
 }

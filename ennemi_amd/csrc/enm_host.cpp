@@ -85,20 +85,25 @@ struct Plan {
     std::vector<KnnDev> kdescs;
     std::vector<CountDev> cdescs;
     std::vector<InstDev> insts;
+    std::vector<RedDev> rdescs;
+    std::vector<RedItem> ritems;
+    std::vector<std::pair<int, int>> inst_ritems;   // per instance: first item, count
     std::vector<double> cls;
     std::vector<Item> kitems, citems;
     std::vector<Group> groups;
     std::vector<ExportInst> exports;
     int ntiles = 0, ntiles2 = 0;
+    int tile2 = ENM_SORT_TILE;   // block-sort tile of the second stage
     int64_t maxlen = 0, maxlen2 = 0;
     int64_t sort_elems = 0, sort_elems2 = 0;
     int64_t n_col64 = 0, n_col32 = 0, n_perm = 0, n_res64 = 0, n_res32 = 0;
     int n_inst = 0;
     bool f32 = true, brute = false, per_instance = false;
     // device
-    DevBuf tables, sk0, si0, sk1, si1, col64, col32, perm, res64, res32, maxabs, counters, stats;
+    DevBuf tables, sk0, si0, sk1, si1, split, col64, col32, perm, res64, res32, maxabs, counters, stats;
     size_t off_segs = 0, off_gitems = 0, off_orders = 0, off_kdescs = 0, off_cdescs = 0, off_insts = 0,
-           off_cls = 0, off_kitems = 0, off_citems = 0, off_segs2 = 0, off_gitems2 = 0, table_bytes = 0;
+           off_cls = 0, off_kitems = 0, off_citems = 0, off_segs2 = 0, off_gitems2 = 0, off_rdescs = 0,
+           off_ritems = 0, table_bytes = 0;
     void* pinned = nullptr;
     hipEvent_t upload_done = nullptr;
     hipGraphExec_t graph = nullptr;
@@ -120,7 +125,7 @@ struct enm_ctx {
     int num_cus = 256;
     std::string err;
     int strip = 1024;       // grid strip size of the kNN order (0: 1-D key order)
-    int waves_per_cu = 16;
+    int waves_per_cu = 32;
     std::list<std::unique_ptr<Plan>> cache;   // most recent first
     Plan* last = nullptr;
     hipStream_t last_stream = nullptr;
@@ -207,6 +212,8 @@ struct Planner {
         od.c32 = P.n_col32;
         od.perm = P.n_perm;
         od.base_perm = -1;
+        od.base_c64 = -1;
+        od.base_c32 = -1;
         for (size_t c = 0; c < cols_src.size(); ++c) od.src[c] = cols_src[c];
         P.n_col64 += od.ncol * I.n;
         P.n_col32 += od.ncol * I.n;
@@ -243,6 +250,8 @@ struct Planner {
         od.c32 = P.n_col32;
         od.perm = P.n_perm;
         od.base_perm = bo.perm;
+        od.base_c64 = bo.c64;
+        od.base_c32 = bo.c32;
         P.n_col64 += od.ncol * od.n;
         P.n_col32 += od.ncol * od.n;
         P.n_perm += od.n;
@@ -250,6 +259,9 @@ struct Planner {
         P.orders.push_back(od);
         order_ids[key] = id;
         order_group.push_back(order_group[base]);
+        // Strips up to 2048 keys are sorted by one block each, with the
+        // smallest tile that holds a strip.
+        P.tile2 = S <= 512 ? 512 : S <= 1024 ? 1024 : ENM_SORT_TILE;
         for (auto rg : ranges(base)) {
             for (int st = rg.first; st < rg.second; st += S) {
                 const int len = std::min(S, rg.second - st);
@@ -260,7 +272,7 @@ struct Planner {
                 sg.first_tile = P.ntiles2;
                 sg.p0 = st;
                 sg.pad = 0;
-                P.ntiles2 += (len + ENM_SORT_TILE - 1) / ENM_SORT_TILE;
+                P.ntiles2 += (len + P.tile2 - 1) / P.tile2;
                 P.sort_elems2 += len;
                 P.maxlen2 = std::max<int64_t>(P.maxlen2, len);
                 const int sid = (int)P.segs2.size();
@@ -283,6 +295,24 @@ struct Planner {
         return r;
     }
 };
+
+// Chunked point-order sum of n values at res64 offset src; returns the
+// partials' offset and sets the chunk count.
+int64_t add_reduction(Plan& P, int64_t src, int64_t n, int log_values, int* nchunks) {
+    RedDev rd;
+    std::memset(&rd, 0, sizeof(rd));
+    rd.src = src;
+    rd.n = n;
+    rd.log = log_values;
+    rd.part = P.n_res64;
+    const int chunks = (int)((n + ENM_RED_CHUNK - 1) / ENM_RED_CHUNK);
+    P.n_res64 += chunks;
+    const int d = (int)P.rdescs.size();
+    P.rdescs.push_back(rd);
+    for (int c = 0; c < chunks; ++c) P.ritems.push_back(RedItem{d, c});
+    *nchunks = chunks;
+    return rd.part;
+}
 
 struct PendingItem {
     Item it;
@@ -341,55 +371,71 @@ int plan_batch(enm_ctx* ctx, Plan& P, const enm_instance* inst, int n_inst, cons
         ex.n = I.n;
         point_off += I.n;
 
-        // Orders and spaces of this kind (column layouts documented in the header).
+        // Orders and spaces of this kind.  The kNN order's columns put the
+        // condition's first two columns first (CMI, conditional Ross), so the
+        // key and the grid's second axis are shared by every count space;
+        // each space lists its columns in ascending order (the max-norm does
+        // not depend on column order), so it starts with columns 0 and 1
+        // whenever it contains both.
         int A = -1;
         std::vector<int64_t> acols;
-        bool segA = classes;
-        if (I.kind == ENM_KIND_CMI) {
-            acols.push_back(Z[0]);
-            acols.insert(acols.end(), X.begin(), X.end());
-            acols.insert(acols.end(), Y.begin(), Y.end());
-            acols.insert(acols.end(), Z.begin() + 1, Z.end());
-        } else if (I.kind == ENM_KIND_CROSS) {
-            acols.push_back(Z[0]);
-            acols.insert(acols.end(), X.begin(), X.end());
-            acols.insert(acols.end(), Z.begin() + 1, Z.end());
+        std::vector<int> posX, posY, posZ;
+        const bool segA = classes;
+        auto put = [&](std::vector<int>& pos, int64_t col) { pos.push_back((int)acols.size()); acols.push_back(col); };
+        if (I.kind == ENM_KIND_CMI || I.kind == ENM_KIND_CROSS) {
+            put(posZ, Z[0]);
+            if (dz >= 2) put(posZ, Z[1]);
+            for (int64_t c : X) put(posX, c);
+            for (int64_t c : Y) put(posY, c);
+            for (int t = 2; t < dz; ++t) put(posZ, Z[t]);
         } else {
-            acols = X;
-            acols.insert(acols.end(), Y.begin(), Y.end());
+            for (int64_t c : X) put(posX, c);
+            for (int64_t c : Y) put(posY, c);
         }
         A = pl.order(ii, acols, segA);
         // The kNN runs on a grid order (strips of the key-sorted order, each
         // sorted by the second column) when there are two or more columns.
-        const bool grid = !P.brute && ctx->strip > 0 && acols.size() >= 2;
-        const int K = grid ? pl.grid_order(A, ctx->strip) : A;
+        const int S = ctx->strip;
+        const bool grid = !P.brute && S > 0 && acols.size() >= 2;
+        const int K = grid ? pl.grid_order(A, S) : A;
 
         struct SpaceSpec { int order; std::vector<int> cols; bool segmented; };
         std::vector<SpaceSpec> spaces;
         auto iota = [](int a, int b) { std::vector<int> v; for (int t = a; t < b; ++t) v.push_back(t); return v; };
+        auto join = [](std::vector<int> a, const std::vector<int>& b) {
+            a.insert(a.end(), b.begin(), b.end());
+            std::sort(a.begin(), a.end());
+            return a;
+        };
         bool need_orig = false;
         if (I.kind == ENM_KIND_KSG) {
-            spaces.push_back({A, iota(0, dx), false});
+            spaces.push_back({A, posX, false});
             const int Yo = pl.order(ii, Y, false);
             spaces.push_back({Yo, iota(0, dy), false});
         } else if (I.kind == ENM_KIND_CMI) {
-            std::vector<int> xz{0}, yz{0}, z{0};
-            for (int t = 1; t <= dx; ++t) xz.push_back(t);
-            for (int t = dx + 1; t <= dx + dy; ++t) yz.push_back(t);
-            for (int t = dx + dy + 1; t < ncols; ++t) { xz.push_back(t); yz.push_back(t); z.push_back(t); }
-            spaces.push_back({A, xz, false});
-            spaces.push_back({A, yz, false});
-            spaces.push_back({A, z, false});
+            spaces.push_back({A, join(posX, posZ), false});
+            spaces.push_back({A, join(posY, posZ), false});
+            spaces.push_back({A, posZ, false});
         } else if (I.kind == ENM_KIND_ROSS) {
             const int B = I.n_segs > 1 ? pl.order(ii, X, false) : A;
             spaces.push_back({B, iota(0, dx), false});
         } else if (I.kind == ENM_KIND_CROSS) {
             const int B = I.n_segs > 1 ? pl.order(ii, acols, false) : A;
-            std::vector<int> z{0};
-            for (int t = dx + 1; t < ncols - dy; ++t) z.push_back(t);
             spaces.push_back({B, iota(0, (int)acols.size()), false});
-            spaces.push_back({A, z, true});
-            spaces.push_back({B, z, false});
+            spaces.push_back({A, posZ, true});
+            spaces.push_back({B, posZ, false});
+        }
+        // A multi-dimensional space whose first two columns are the order's
+        // columns 0 and 1 counts on the grid order over that order.
+        std::vector<int> space_strip(spaces.size(), 0);
+        std::vector<int64_t> space_base_key(spaces.size(), -1);
+        for (size_t t = 0; t < spaces.size(); ++t) {
+            SpaceSpec& sp = spaces[t];
+            if (grid && sp.cols.size() >= 2 && sp.cols[0] == 0 && sp.cols[1] == 1) {
+                space_base_key[t] = P.orders[sp.order].c64;
+                sp.order = pl.grid_order(sp.order, S);
+                space_strip[t] = S;
+            }
         }
         for (auto& s : spaces) if (s.order != K) need_orig = true;
         // Entropy sums log radii in point order, whatever the kNN order.
@@ -428,8 +474,15 @@ int plan_batch(enm_ctx* ctx, Plan& P, const enm_instance* inst, int n_inst, cons
                 mine.push_back(pi);
             }
         }
-        id.eps_orig = kd.eps_orig;
-        if (kd.kl) id.nspace = 0;
+        const int red_first = (int)P.ritems.size();
+        if (kd.kl) {
+            // Entropy: log radii summed in point order.
+            int chunks = 0;
+            id.nspace = 1;
+            id.part[0] = add_reduction(P, kd.eps_orig, I.n, 1, &chunks);
+            id.zero[0] = -1;
+            id.ntile[0] = chunks;
+        }
         P.kdescs.push_back(kd);
         kpend.insert(kpend.end(), mine.begin(), mine.end());
         ex.eps = kd.eps;
@@ -441,6 +494,8 @@ int plan_batch(enm_ctx* ctx, Plan& P, const enm_instance* inst, int n_inst, cons
             const SpaceSpec& sp = spaces[s];
             CountDev cd;
             std::memset(&cd, 0, sizeof(cd));
+            cd.strip = space_strip[s];
+            cd.base_key = space_base_key[s];
             cd.order = sp.order;
             cd.inst = ii;
             cd.ncol = (int)sp.cols.size();
@@ -452,6 +507,7 @@ int plan_batch(enm_ctx* ctx, Plan& P, const enm_instance* inst, int n_inst, cons
             P.n_res32 += I.n;
             const int cdi = (int)P.cdescs.size();
             const int ccl = enm_count_dim_class(cd.ncol);
+            cd.psi_pt = -1;
             int ctiles = 0;
             std::vector<std::pair<int, int>> rgs;
             if (sp.segmented) rgs = pl.ranges(sp.order);
@@ -467,18 +523,32 @@ int plan_batch(enm_ctx* ctx, Plan& P, const enm_instance* inst, int n_inst, cons
                     cpend.push_back(pi);
                 }
             }
-            cd.part = P.n_res64;
-            P.n_res64 += ctiles;
-            cd.zero = P.n_res32;
-            P.n_res32 += ctiles;
-            id.part[s] = cd.part;
-            id.zero[s] = cd.zero;
-            id.ntile[s] = ctiles;
+            if (ccl == 1) {
+                // 1-D spaces sum per tile of their key-sorted order.
+                cd.part = P.n_res64;
+                P.n_res64 += ctiles;
+                cd.zero = P.n_res32;
+                P.n_res32 += ctiles;
+                id.part[s] = cd.part;
+                id.zero[s] = cd.zero;
+                id.ntile[s] = ctiles;
+            } else {
+                // Multi-dimensional spaces write psi in point order.
+                int chunks = 0;
+                cd.psi_pt = P.n_res64;
+                P.n_res64 += I.n;
+                cd.part = -1;
+                cd.zero = -1;
+                id.part[s] = add_reduction(P, cd.psi_pt, I.n, 0, &chunks);
+                id.zero[s] = -1;
+                id.ntile[s] = chunks;
+            }
             ex.sp[s].counts = cd.counts;
             ex.sp[s].perm = cd.eps_perm;
             P.cdescs.push_back(cd);
         }
         ex.nspace = (int)spaces.size();
+        P.inst_ritems.emplace_back(red_first, (int)P.ritems.size() - red_first);
         if (I.kind == ENM_KIND_ROSS) {
             id.seg_len = (int64_t)P.cls.size();
             for (int s = 0; s < I.n_segs; ++s) P.cls.push_back((double)seg_len[I.seg_index + s]);
@@ -541,6 +611,8 @@ int allocate_and_upload(enm_ctx* ctx, Plan& P, hipStream_t st) {
     P.off_citems = append(blob, P.citems);
     P.off_segs2 = append(blob, P.segs2);
     P.off_gitems2 = append(blob, P.gitems2);
+    P.off_rdescs = append(blob, P.rdescs);
+    P.off_ritems = append(blob, P.ritems);
     P.table_bytes = blob.size();
     hipError_t e;
     if (P.upload_done) {
@@ -567,6 +639,7 @@ int allocate_and_upload(enm_ctx* ctx, Plan& P, hipStream_t st) {
     if ((rc = need(P.sk1, se * 8, "sort keys"))) return rc;
     if ((rc = need(P.si0, se * 4, "sort index"))) return rc;
     if ((rc = need(P.si1, se * 4, "sort index"))) return rc;
+    if ((rc = need(P.split, (size_t)std::max(P.ntiles, P.ntiles2) * 4 + 4, "merge splits"))) return rc;
     if ((rc = need(P.col64, P.n_col64 * 8, "columns"))) return rc;
     if ((rc = need(P.col32, P.n_col32 * 4, "columns"))) return rc;
     if ((rc = need(P.perm, P.n_perm * 4, "permutations"))) return rc;
@@ -620,9 +693,10 @@ int enqueue(enm_ctx* ctx, Plan& P, const double* arena, double* out, double* d_e
     nl += 3;
     if ((rc = mark(1))) return rc;
     int fb = 0;
-    if ((rc = chk(enm_launch_sort(tab<SortSeg>(P, P.off_segs), (int)P.segs.size(), P.ntiles, P.maxlen, arena,
+    if ((rc = chk(enm_launch_sort(tab<SortSeg>(P, P.off_segs), (int)P.segs.size(), P.ntiles, P.maxlen,
+                                  ENM_SORT_TILE, arena,
                                   P.sk0.as<double>(), P.si0.as<int>(), P.sk1.as<double>(), P.si1.as<int>(),
-                                  &fb, &nl, st), "sort")))
+                                  P.split.as<int>(), &fb, &nl, st), "sort")))
         return rc;
     if ((rc = mark(2))) return rc;
     const double* skey = fb ? P.sk1.as<double>() : P.sk0.as<double>();
@@ -636,8 +710,9 @@ int enqueue(enm_ctx* ctx, Plan& P, const double* arena, double* out, double* d_e
         // Second stage: sort each grid strip by the base order's column 1
         // (keys read from col64), then gather the grid orders.
         if ((rc = chk(enm_launch_sort(tab<SortSeg>(P, P.off_segs2), (int)P.segs2.size(), P.ntiles2, P.maxlen2,
-                                      P.col64.as<double>(), P.sk0.as<double>(), P.si0.as<int>(),
-                                      P.sk1.as<double>(), P.si1.as<int>(), &fb, &nl, st), "sort")))
+                                      P.tile2, P.col64.as<double>(), P.sk0.as<double>(), P.si0.as<int>(),
+                                      P.sk1.as<double>(), P.si1.as<int>(), P.split.as<int>(), &fb, &nl, st),
+                      "sort")))
             return rc;
         const double* skey2 = fb ? P.sk1.as<double>() : P.sk0.as<double>();
         const int* sidx2 = fb ? P.si1.as<int>() : P.si0.as<int>();
@@ -675,6 +750,12 @@ int enqueue(enm_ctx* ctx, Plan& P, const double* arena, double* out, double* d_e
                 if (g.inst == ii && !g.is_count && (rc = run_group(g))) return rc;
             for (const Group& g : P.groups)
                 if (g.inst == ii && g.is_count && (rc = run_group(g))) return rc;
+            const auto ri = P.inst_ritems[ii];
+            if ((rc = chk(enm_launch_reduce_pt(tab<RedItem>(P, P.off_ritems) + ri.first, ri.second,
+                                               tab<RedDev>(P, P.off_rdescs), P.res64.as<double>(), st),
+                          "reduce")))
+                return rc;
+            if (ri.second > 0) ++nl;
             if ((rc = chk(enm_launch_combine(tab<InstDev>(P, P.off_insts), tab<double>(P, P.off_cls), ii, 1,
                                              P.res64.as<double>(), P.res32.as<int>(), out, st), "combine")))
                 return rc;
@@ -688,6 +769,11 @@ int enqueue(enm_ctx* ctx, Plan& P, const double* arena, double* out, double* d_e
         if ((rc = mark(4))) return rc;
         for (const Group& g : P.groups)
             if (g.is_count && (rc = run_group(g))) return rc;
+        if ((rc = chk(enm_launch_reduce_pt(tab<RedItem>(P, P.off_ritems), (int)P.ritems.size(),
+                                           tab<RedDev>(P, P.off_rdescs), P.res64.as<double>(), st),
+                      "reduce")))
+            return rc;
+        if (!P.ritems.empty()) ++nl;
         if ((rc = mark(5))) return rc;
         if ((rc = chk(enm_launch_combine(tab<InstDev>(P, P.off_insts), tab<double>(P, P.off_cls), 0, P.n_inst,
                                          P.res64.as<double>(), P.res32.as<int>(), out, st), "combine")))

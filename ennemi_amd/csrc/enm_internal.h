@@ -57,9 +57,12 @@ struct OrderDev {
     int32_t inst;
     int64_t src[16];    // arena offsets of the instance columns gathered (src[0] = key)
     // Grid orders are sorted in a second stage, within strips of a base
-    // order: their sort payload is a base-order position, mapped to a point
-    // through the base order's permutation at this perm offset (-1: none).
+    // order: their sort payload is a base-order position; the point and its
+    // columns are read from the base order (perm offset base_perm, col64 /
+    // col32 offsets base_c64 / base_c32), inside the same strip.  -1: none.
     int64_t base_perm;
+    int64_t base_c64;
+    int64_t base_c32;
 };
 
 // kNN over one order (its scope ranges are the class segments, or all points).
@@ -92,6 +95,29 @@ struct CountDev {
     int64_t counts;      // res32 offset: count per order position
     int64_t part;        // res64 offset: per-tile sum of psi(count)
     int64_t zero;        // res32 offset: per-tile "some count is 0" flag
+    // With strip > 0 the order is a grid order (as for KnnDev) and the space
+    // starts with the order's columns 0 and 1.
+    int64_t base_key;
+    int32_t strip;
+    int32_t pad;
+    // >= 0: res64 offset of a point-order array receiving psi(count) (+inf
+    // for a count of 0) instead of per-tile partial sums; a reduction in
+    // point order then makes the estimate independent of this order.
+    int64_t psi_pt;
+};
+
+// Fixed-chunk sum of a point-order array (psi values, or log radii).
+#define ENM_RED_CHUNK 4096
+struct RedDev {
+    int64_t src;    // res64 offset of n values in point order
+    int64_t n;
+    int64_t part;   // res64 offset of one partial per chunk
+    int32_t log;    // 1: sum log(value)
+    int32_t pad;
+};
+struct RedItem {
+    int32_t desc;
+    int32_t chunk;
 };
 
 // One work item: a tile of up to 64 consecutive order positions [t0, t1)
@@ -117,7 +143,6 @@ struct InstDev {
     int64_t zero[ENM_MAX_SPACES];  // res32 offsets of per-tile zero flags
     int32_t ntile[ENM_MAX_SPACES];
     int32_t pad1;
-    int64_t eps_orig;    // res64 offset of the radii in point order (entropy)
 };
 
 // Device counters (enm_stats order).

@@ -27,10 +27,14 @@
 // is scanned instead.
 //
 // Precision.  In FP64 mode every distance is exact FP64.  In FP32-filter mode
-// the bulk distance is computed from FP32 copies, and a candidate is
-// re-evaluated in FP64 only when its FP32 distance is <= fl32_up(T + g), with
-// the per-instance guard g bounding |d32 - d64|.  T only decreases, so one
-// pass is exact: radii are bit-identical to the FP64 mode.
+// the bulk distance is computed from FP32 copies, and a chunk of 8 candidates
+// is settled in FP64 only when some lane's FP32 distance is <= fl32_up(T + g),
+// with the per-instance guard g bounding |d32 - d64|.  T only decreases, so
+// one pass is exact: radii are bit-identical to the FP64 mode.  While most
+// chunks improve some lane's list (early in a sweep, and in the compact grid
+// windows) the wave skips the filter and evaluates chunks in FP64 directly.
+// List updates for up to 8 slots are branch-free: the 8 exact distances are
+// sorted by a 19-comparator network and merged into the list.
 //
 // Work distribution.  A persistent grid pops items from an atomic counter in
 // the (largest-first) order the host laid them out.
@@ -64,6 +68,7 @@ struct KnnTile {
     float thr;
     double guard;
     unsigned rechecks;
+    bool dense;   // FP32 mode: evaluating chunks directly in FP64
 
     __device__ __forceinline__ void set_thr() {
         if (F32) thr = f32_up(T + guard);
@@ -81,87 +86,78 @@ struct KnnTile {
         }
     }
 
-    __device__ __forceinline__ void chunk(int j) {
-        if (F32) {
-            // The FP64 coordinates of the chunk are loaded with the FP32 ones
-            // (wave-uniform, scalar), so a recheck does not wait on a load.
-            double e64[D][CH];
-            float dd[CH];
+    // FP64 chunk: exact distances; merge when some lane improves.  Returns
+    // whether any lane could improve its list.
+    __device__ __forceinline__ bool chunk64(int j) {
+        double dd[CH];
+#pragma unroll
+        for (int c = 0; c < CH; ++c) {
+            double m = fabs(c64[0][j + c] - q64[0]);
+#pragma unroll
+            for (int d = 1; d < D; ++d) m = fmax(m, fabs(c64[d][j + c] - q64[d]));
+            dd[c] = m;
+        }
+        double mn = dd[0];
+#pragma unroll
+        for (int c = 1; c < CH; ++c) mn = fmin(mn, dd[c]);
+        if (!wave_any(mn < T)) return false;
+        bool merged = false;
+        if constexpr (P <= 8) {
+            // After a chunk that improved some list (dense regime) most of
+            // the 8 candidates matter to some lane: one network merge.
+            if (dense) {
+                sort8(dd);
+                list_merge8<P>(L, dd);
+                T = L[P - 1];
+                merged = true;
+            }
+        }
+        if (!merged) {
+            // Sparse regime: only the few candidates some lane needs.
 #pragma unroll
             for (int c = 0; c < CH; ++c) {
-#pragma unroll
-                for (int d = 0; d < D; ++d) e64[d][c] = c64[d][j + c];
-                float m = fabsf(c32[0][j + c] - q32[0]);
-#pragma unroll
-                for (int d = 1; d < D; ++d) m = fmaxf(m, fabsf(c32[d][j + c] - q32[d]));
-                dd[c] = m;
-            }
-            float mn = dd[0];
-#pragma unroll
-            for (int c = 1; c < CH; ++c) mn = fminf(mn, dd[c]);
-            if (wave_any(mn <= thr)) {
-                if constexpr (P <= 8) {
-                    // Exact distances of all 8 candidates, merged into the
-                    // list by a network.  A candidate the filter rejected has
-                    // an FP64 distance > T, so merging it changes nothing.
-                    double e[CH];
-#pragma unroll
-                    for (int c = 0; c < CH; ++c) {
-                        double v = fabs(e64[0][c] - q64[0]);
-#pragma unroll
-                        for (int d = 1; d < D; ++d) v = fmax(v, fabs(e64[d][c] - q64[d]));
-                        e[c] = v;
-                        rechecks += dd[c] <= thr;
-                    }
-                    sort8(e);
-                    list_merge8<P>(L, e);
+                if (wave_any(dd[c] < T) && dd[c] < T) {
+                    list_insert<P>(L, dd[c]);
                     T = L[P - 1];
-                    set_thr();
-                } else {
-#pragma unroll
-                    for (int c = 0; c < CH; ++c) {
-                        if (dd[c] <= thr) {
-                            double e = fabs(e64[0][c] - q64[0]);
-#pragma unroll
-                            for (int d = 1; d < D; ++d) e = fmax(e, fabs(e64[d][c] - q64[d]));
-                            ++rechecks;
-                            if (e < T) {
-                                list_insert<P>(L, e);
-                                T = L[P - 1];
-                                set_thr();
-                            }
-                        }
-                    }
-                }
-            }
-        } else {
-            double dd[CH];
-#pragma unroll
-            for (int c = 0; c < CH; ++c) {
-                double m = fabs(c64[0][j + c] - q64[0]);
-#pragma unroll
-                for (int d = 1; d < D; ++d) m = fmax(m, fabs(c64[d][j + c] - q64[d]));
-                dd[c] = m;
-            }
-            double mn = dd[0];
-#pragma unroll
-            for (int c = 1; c < CH; ++c) mn = fmin(mn, dd[c]);
-            if (wave_any(mn < T)) {
-                if constexpr (P <= 8) {
-                    sort8(dd);
-                    list_merge8<P>(L, dd);
-                    T = L[P - 1];
-                } else {
-#pragma unroll
-                    for (int c = 0; c < CH; ++c) {
-                        if (dd[c] < T) {
-                            list_insert<P>(L, dd[c]);
-                            T = L[P - 1];
-                        }
-                    }
                 }
             }
         }
+        if (F32) set_thr();
+        return true;
+    }
+
+    // FP32-filter chunk: FP32 distances; if some lane's minimum passes the
+    // guard-widened threshold, the chunk is settled exactly in FP64.
+    // Returns whether that happened.
+    __device__ __forceinline__ bool chunk32(int j) {
+        float dd[CH];
+#pragma unroll
+        for (int c = 0; c < CH; ++c) {
+            float m = fabsf(c32[0][j + c] - q32[0]);
+#pragma unroll
+            for (int d = 1; d < D; ++d) m = fmaxf(m, fabsf(c32[d][j + c] - q32[d]));
+            dd[c] = m;
+        }
+        float mn = dd[0];
+#pragma unroll
+        for (int c = 1; c < CH; ++c) mn = fminf(mn, dd[c]);
+        if (!wave_any(mn <= thr)) return false;
+#pragma unroll
+        for (int c = 0; c < CH; ++c) rechecks += dd[c] <= thr;
+        // A candidate the filter rejected has an FP64 distance > T, so
+        // settling the whole chunk in FP64 gives the same list.
+        (void)chunk64(j);
+        return true;
+    }
+
+    // One chunk of 8 candidates.  The wave adapts to how often chunks
+    // improve some lane's list: after such a chunk ("dense") it evaluates the
+    // next in FP64 directly and updates lists with the merge network; after a
+    // chunk that improved none it filters in FP32 (FP32-filter mode) and
+    // inserts the few passing candidates one by one.
+    __device__ __forceinline__ void chunk(int j) {
+        if (F32 && !dense) dense = chunk32(j);
+        else dense = chunk64(j);
     }
 
     __device__ __forceinline__ void single(int j) {
@@ -305,6 +301,7 @@ __device__ void knn_item(const KnnArgs& A, const Item& it, const double* __restr
     w.T = active ? INFINITY : -INFINITY;
     w.thr = active ? INFINITY : -INFINITY;
     w.rechecks = 0;
+    w.dense = true;   // a tile starts with its nearest candidates
 
     const int lo = it.lo, hi = it.hi, t0 = it.t0, t1 = it.t1;
     int visited = t1 - t0;

@@ -9,8 +9,8 @@
 hipError_t enm_launch_fill_u64(unsigned long long* p, int64_t n, unsigned long long v, hipStream_t st);
 hipError_t enm_launch_fill_i32(int* p, int64_t n, int v, hipStream_t st);
 
-hipError_t enm_launch_sort(const SortSeg* d_segs, int nsegs, int ntiles, int64_t maxlen,
-                           const double* arena, double* k0, int* i0, double* k1, int* i1,
+hipError_t enm_launch_sort(const SortSeg* d_segs, int nsegs, int ntiles, int64_t maxlen, int tile,
+                           const double* arena, double* k0, int* i0, double* k1, int* i1, int* split,
                            int* final_buf, int* n_launch, hipStream_t st);
 hipError_t enm_launch_gather(const GatherItem* d_items, int nitems, const SortSeg* d_segs,
                              const OrderDev* d_orders, const double* arena,
@@ -37,5 +37,7 @@ hipError_t enm_launch_count(const Item* items, int n_items, unsigned* counter,
 
 hipError_t enm_launch_combine(const InstDev* insts, const double* cls, int first, int count,
                               const double* res64, const int* res32, double* out, hipStream_t st);
+hipError_t enm_launch_reduce_pt(const RedItem* items, int n_items, const RedDev* descs, double* res64,
+                                hipStream_t st);
 hipError_t enm_launch_export_eps(const double* src, const int* perm, int64_t n, double* dst, hipStream_t st);
 hipError_t enm_launch_export_cnt(const int* src, const int* perm, int64_t n, int* dst, hipStream_t st);

@@ -4,8 +4,10 @@
 //
 // One wave per instance sums the partials of each space with a fixed lane
 // assignment and butterfly, so the result does not depend on the order in
-// which the work queue ran the tiles, nor (as count spaces live in key-sorted
-// orders and entropy radii are read in point order) on the kNN tiling.  The formulas and the psi(0) = +inf
+// which the work queue ran the tiles, nor on the kNN and count tilings: 1-D
+// count spaces sum per tile of a key-sorted order (which no tuning changes),
+// multi-dimensional spaces and entropy log radii are summed in point order by
+// reduce_pt_kernel.  The formulas and the psi(0) = +inf
 // broadcasting follow ennemi/_entropy_estimators.py:
 //   KSG   :113  psi(N) + psi(k) - mean(psi(nx) + psi(ny))
 //   CMI   :156  psi(k) - mean(psi(nxz) + psi(nyz) - psi(nz))
@@ -40,14 +42,10 @@ void combine_kernel(const InstDev* __restrict__ insts, const double* __restrict_
         sum[s] = wave_sum(acc);
         zero[s] = wave_or(z);
     }
-    // Entropy: mean of log radii over the points in point order (lane-strided
-    // then butterfly, so the sum does not depend on the kNN order or tuning).
-    double logsum = 0.0;
-    if (in.kind == ENM_KIND_KL) {
-        double acc = 0.0;
-        for (int64_t i = lane; i < in.n; i += ENM_WAVE) acc += log(res64[in.eps_orig + i]);
-        logsum = wave_sum(acc);
-    }
+    // A space summed in point order flags a zero count as a +inf sum.
+#pragma unroll
+    for (int s = 0; s < ENM_MAX_SPACES; ++s)
+        if (s < in.nspace && in.kind != ENM_KIND_KL && sum[s] == INFINITY) zero[s] = 1;
     if (lane != 0) return;
     const double N = (double)in.n;
     const double psik = enm_psi((double)in.k);
@@ -82,11 +80,30 @@ void combine_kernel(const InstDev* __restrict__ insts, const double* __restrict_
         }
         case ENM_KIND_KL:
         default: {
-            value = psi_int(in.n) - psik + (double)in.dx * (logsum / N + log(2.0));
+            value = psi_int(in.n) - psik + (double)in.dx * (sum[0] / N + log(2.0));
             break;
         }
     }
     out[ii] = value;
+}
+
+// Chunked sums of point-order arrays: one wave per chunk of ENM_RED_CHUNK
+// points, lane-strided then butterfly, so each partial (and the instance's
+// estimate) depends only on the values in point order.
+__global__ __launch_bounds__(ENM_WAVE)
+void reduce_pt_kernel(const RedItem* __restrict__ items, const RedDev* __restrict__ descs,
+                      double* __restrict__ res64) {
+    const RedItem it = items[blockIdx.x];
+    const RedDev rd = descs[it.desc];
+    const int64_t a = (int64_t)it.chunk * ENM_RED_CHUNK;
+    const int64_t b = min(rd.n, a + ENM_RED_CHUNK);
+    double acc = 0.0;
+    for (int64_t i = a + threadIdx.x; i < b; i += ENM_WAVE) {
+        const double v = res64[rd.src + i];
+        acc += rd.log ? log(v) : v;
+    }
+    acc = wave_sum(acc);
+    if (threadIdx.x == 0) res64[rd.part + it.chunk] = acc;
 }
 
 // Export one order-positioned array into instance numbering.
@@ -114,6 +131,13 @@ hipError_t enm_launch_combine(const InstDev* insts, const double* cls, int first
     if (count <= 0) return hipSuccess;
     hipLaunchKernelGGL(combine_kernel, dim3(count), dim3(ENM_WAVE), 0, st, insts, cls, first, res64,
                        res32, out);
+    return hipGetLastError();
+}
+
+hipError_t enm_launch_reduce_pt(const RedItem* items, int n_items, const RedDev* descs, double* res64,
+                                hipStream_t st) {
+    if (n_items <= 0) return hipSuccess;
+    hipLaunchKernelGGL(reduce_pt_kernel, dim3(n_items), dim3(ENM_WAVE), 0, st, items, descs, res64);
     return hipGetLastError();
 }
 

@@ -8,10 +8,10 @@
 //
 // Algorithm: every segment is cut into ENM_SORT_TILE chunks; a 256-thread
 // block sorts one chunk in LDS with a bitonic network, then merge passes of
-// doubling run width merge neighbouring runs inside each segment.  A merge
-// block owns ENM_SORT_TILE outputs: two lanes find the merge-path split of the
-// block's first and last output by binary search, the block stages the two
-// input slices in LDS, and each thread merges 8 outputs.  Keys compare as
+// doubling run width merge neighbouring runs inside each segment.  Per pass,
+// one thread per chunk finds the merge-path split of the chunk's first output
+// (co_rank_kernel), then a merge block owns ENM_SORT_TILE outputs: it stages
+// its two input slices in LDS and each thread merges 8 outputs.  Keys compare as
 // (value, index) pairs, so all keys are distinct and the order is total.
 // Traffic per pass is 12 B read + 12 B written per key (HBM-bound).
 #include "enm_device.h"
@@ -33,18 +33,19 @@ __device__ __forceinline__ int seg_of_tile(const SortSeg* __restrict__ segs, int
     return lo;
 }
 
+template <int TILE>
 __global__ __launch_bounds__(ENM_SORT_THREADS)
 void block_sort_kernel(const SortSeg* __restrict__ segs, int nsegs,
                        const double* __restrict__ arena,
                        double* __restrict__ kout, int* __restrict__ iout) {
-    __shared__ double sk[ENM_SORT_TILE];
-    __shared__ int si[ENM_SORT_TILE];
+    __shared__ double sk[TILE];
+    __shared__ int si[TILE];
     const int tile = blockIdx.x;
     const int s = seg_of_tile(segs, nsegs, tile);
     const SortSeg sg = segs[s];
-    const int off = (tile - sg.first_tile) * ENM_SORT_TILE;
-    const int len = min(ENM_SORT_TILE, sg.len - off);
-    for (int t = threadIdx.x; t < ENM_SORT_TILE; t += ENM_SORT_THREADS) {
+    const int off = (tile - sg.first_tile) * TILE;
+    const int len = min(TILE, sg.len - off);
+    for (int t = threadIdx.x; t < TILE; t += ENM_SORT_THREADS) {
         if (t < len) {
             sk[t] = arena[sg.src + off + t];
             si[t] = sg.p0 + off + t;
@@ -54,9 +55,9 @@ void block_sort_kernel(const SortSeg* __restrict__ segs, int nsegs,
         }
     }
     __syncthreads();
-    for (int size = 2; size <= ENM_SORT_TILE; size <<= 1) {
+    for (int size = 2; size <= TILE; size <<= 1) {
         for (int stride = size >> 1; stride > 0; stride >>= 1) {
-            for (int q = threadIdx.x; q < ENM_SORT_TILE / 2; q += ENM_SORT_THREADS) {
+            for (int q = threadIdx.x; q < TILE / 2; q += ENM_SORT_THREADS) {
                 const int i = 2 * q - (q & (stride - 1));
                 const int j = i + stride;
                 const bool asc = (i & size) == 0;
@@ -88,24 +89,53 @@ __device__ int co_rank(int d, const double* ka, const int* ia, int na,
     return lo;
 }
 
-__global__ __launch_bounds__(ENM_SORT_THREADS)
-void merge_pass_kernel(const SortSeg* __restrict__ segs, int nsegs, int64_t width,
-                       const double* __restrict__ kin, const int* __restrict__ iin,
-                       double* __restrict__ kout, int* __restrict__ iout) {
-    __shared__ double sk[ENM_SORT_TILE];
-    __shared__ int si[ENM_SORT_TILE];
-    __shared__ int split[2];
-    const int tile = blockIdx.x;
-    const int s = seg_of_tile(segs, nsegs, tile);
-    const SortSeg sg = segs[s];
+// Geometry of chunk `tile` in a merge pass of run width `width`: its pair of
+// runs A = [base, base+na), B = [base+width, base+width+nb) in the sort
+// buffer and its output range [d0, d1) within the pair.
+struct MergeChunk {
+    int64_t base;
+    int na, nb, d0, d1;
+};
+
+__device__ __forceinline__ MergeChunk merge_chunk(const SortSeg& sg, int tile, int64_t width) {
+    MergeChunk m;
     const int64_t off = (int64_t)(tile - sg.first_tile) * ENM_SORT_TILE;
     const int64_t len = sg.len;
     const int64_t pb = (off / (2 * width)) * (2 * width);
-    const int na = (int)min(width, len - pb);
-    const int nb = (int)max((int64_t)0, min(2 * width, len - pb) - width);
-    const int d0 = (int)(off - pb);
-    const int d1 = min(d0 + ENM_SORT_TILE, na + nb);
-    const int64_t base = sg.sbase + pb;
+    m.na = (int)min(width, len - pb);
+    m.nb = (int)max((int64_t)0, min(2 * width, len - pb) - width);
+    m.d0 = (int)(off - pb);
+    m.d1 = min(m.d0 + ENM_SORT_TILE, m.na + m.nb);
+    m.base = sg.sbase + pb;
+    return m;
+}
+
+// Merge-path split of every chunk's first output, one thread per chunk, all
+// in parallel (each a binary search of ~17 dependent loads).
+__global__ __launch_bounds__(ENM_SORT_THREADS)
+void co_rank_kernel(const SortSeg* __restrict__ segs, int nsegs, int ntiles, int64_t width,
+                    const double* __restrict__ kin, const int* __restrict__ iin, int* __restrict__ split) {
+    const int tile = blockIdx.x * ENM_SORT_THREADS + threadIdx.x;
+    if (tile >= ntiles) return;
+    const SortSeg sg = segs[seg_of_tile(segs, nsegs, tile)];
+    const MergeChunk m = merge_chunk(sg, tile, width);
+    split[tile] = m.nb == 0 ? m.d0
+                            : co_rank(m.d0, kin + m.base, iin + m.base, m.na, kin + m.base + width,
+                                      iin + m.base + width, m.nb);
+}
+
+__global__ __launch_bounds__(ENM_SORT_THREADS)
+void merge_pass_kernel(const SortSeg* __restrict__ segs, int nsegs, int64_t width,
+                       const double* __restrict__ kin, const int* __restrict__ iin,
+                       const int* __restrict__ split,
+                       double* __restrict__ kout, int* __restrict__ iout) {
+    __shared__ double sk[ENM_SORT_TILE];
+    __shared__ int si[ENM_SORT_TILE];
+    const int tile = blockIdx.x;
+    const SortSeg sg = segs[seg_of_tile(segs, nsegs, tile)];
+    const MergeChunk m = merge_chunk(sg, tile, width);
+    const int64_t base = m.base;
+    const int na = m.na, nb = m.nb, d0 = m.d0, d1 = m.d1;
     if (nb == 0) {
         for (int t = d0 + threadIdx.x; t < d1; t += ENM_SORT_THREADS) {
             kout[base + t] = kin[base + t];
@@ -117,12 +147,9 @@ void merge_pass_kernel(const SortSeg* __restrict__ segs, int nsegs, int64_t widt
     const int* ia = iin + base;
     const double* kb = kin + base + width;
     const int* ib = iin + base + width;
-    if (threadIdx.x < 2) {
-        const int d = threadIdx.x == 0 ? d0 : d1;
-        split[threadIdx.x] = co_rank(d, ka, ia, na, kb, ib, nb);
-    }
-    __syncthreads();
-    const int i0 = split[0], i1 = split[1];
+    // The next chunk's split is this chunk's end split, unless the pair ends.
+    const int i0 = split[tile];
+    const int i1 = d1 == na + nb ? na : split[tile + 1];
     const int j0 = d0 - i0, j1 = d1 - i1;
     const int la = i1 - i0, lb = j1 - j0;
     for (int t = threadIdx.x; t < la + lb; t += ENM_SORT_THREADS) {
@@ -152,16 +179,16 @@ void merge_pass_kernel(const SortSeg* __restrict__ segs, int nsegs, int64_t widt
 // Lay one sorted chunk out into an order: permutation, key column and the
 // other gathered columns (FP64 and FP32), and fold |value| into the
 // instance's maximum magnitude (for the FP32 guard band).  For a grid order
-// the sort payload is a base-order position: the point is read through the
-// base permutation and every column, key included, is gathered from the
-// arena.
+// the sort payload is a base-order position in the same strip: the point and
+// every column are copied from the base order (strip-local reads; the base
+// order's magnitudes are already folded in).
 __global__ __launch_bounds__(ENM_SORT_THREADS)
 void gather_kernel(const GatherItem* __restrict__ items,
                    const SortSeg* __restrict__ segs,
                    const OrderDev* __restrict__ orders,
                    const double* __restrict__ arena,
                    const double* __restrict__ skey, const int* __restrict__ sidx,
-                   double* __restrict__ col64, float* __restrict__ col32,
+                   double* col64, float* col32,
                    int* perm, unsigned long long* __restrict__ maxabs) {
     __shared__ double red[ENM_SORT_THREADS / ENM_WAVE];
     const GatherItem it = items[blockIdx.x];
@@ -175,18 +202,22 @@ void gather_kernel(const GatherItem* __restrict__ items,
         const int64_t sp = sg.sbase + it.off + t;
         const int64_t p = it.pos0 + it.off + t;
         const int idx = sidx[sp];
-        const int pt = indirect ? perm[od.base_perm + idx] : idx;
-        perm[od.perm + p] = pt;
-        int c0 = 0;
-        if (!indirect) {
-            const double key = skey[sp];
-            col64[od.c64 + p] = key;
-            col32[od.c32 + p] = (float)key;
-            m = fmax(m, fabs(key));
-            c0 = 1;
+        if (indirect) {
+            // idx is a base-order position in the same strip: read locally.
+            perm[od.perm + p] = perm[od.base_perm + idx];
+            for (int c = 0; c < od.ncol; ++c) {
+                col64[od.c64 + c * n + p] = col64[od.base_c64 + c * n + idx];
+                col32[od.c32 + c * n + p] = col32[od.base_c32 + c * n + idx];
+            }
+            continue;
         }
-        for (int c = c0; c < od.ncol; ++c) {
-            const double v = arena[od.src[c] + pt];
+        perm[od.perm + p] = idx;
+        const double key = skey[sp];
+        col64[od.c64 + p] = key;
+        col32[od.c32 + p] = (float)key;
+        m = fmax(m, fabs(key));
+        for (int c = 1; c < od.ncol; ++c) {
+            const double v = arena[od.src[c] + idx];
             col64[od.c64 + c * n + p] = v;
             col32[od.c32 + c * n + p] = (float)v;
             m = fmax(m, fabs(v));
@@ -218,13 +249,25 @@ hipError_t enm_launch_fill_u64(unsigned long long* p, int64_t n, unsigned long l
     return hipGetLastError();
 }
 
-hipError_t enm_launch_sort(const SortSeg* d_segs, int nsegs, int ntiles, int64_t maxlen,
-                           const double* arena, double* k0, int* i0, double* k1, int* i1,
+hipError_t enm_launch_sort(const SortSeg* d_segs, int nsegs, int ntiles, int64_t maxlen, int tile,
+                           const double* arena, double* k0, int* i0, double* k1, int* i1, int* split,
                            int* final_buf, int* n_launch, hipStream_t st) {
     *final_buf = 0;
     if (ntiles <= 0) return hipSuccess;
-    hipLaunchKernelGGL(block_sort_kernel, dim3(ntiles), dim3(ENM_SORT_THREADS), 0, st,
-                       d_segs, nsegs, arena, k0, i0);
+    // Segments no longer than the tile are sorted by the block sort alone;
+    // longer ones (always with 2048-key tiles) continue with merge passes.
+    if (tile == 512)
+        hipLaunchKernelGGL(block_sort_kernel<512>, dim3(ntiles), dim3(ENM_SORT_THREADS), 0, st,
+                           d_segs, nsegs, arena, k0, i0);
+    else if (tile == 1024)
+        hipLaunchKernelGGL(block_sort_kernel<1024>, dim3(ntiles), dim3(ENM_SORT_THREADS), 0, st,
+                           d_segs, nsegs, arena, k0, i0);
+    else if (tile == ENM_SORT_TILE)
+        hipLaunchKernelGGL(block_sort_kernel<ENM_SORT_TILE>, dim3(ntiles), dim3(ENM_SORT_THREADS), 0, st,
+                           d_segs, nsegs, arena, k0, i0);
+    else
+        return hipErrorInvalidValue;
+    if (tile != ENM_SORT_TILE && maxlen > tile) return hipErrorInvalidValue;
     ++*n_launch;
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
@@ -234,9 +277,11 @@ hipError_t enm_launch_sort(const SortSeg* d_segs, int nsegs, int ntiles, int64_t
         const int* iin = cur ? i1 : i0;
         double* kout = cur ? k0 : k1;
         int* iout = cur ? i0 : i1;
+        hipLaunchKernelGGL(co_rank_kernel, dim3((ntiles + ENM_SORT_THREADS - 1) / ENM_SORT_THREADS),
+                           dim3(ENM_SORT_THREADS), 0, st, d_segs, nsegs, ntiles, w, kin, iin, split);
         hipLaunchKernelGGL(merge_pass_kernel, dim3(ntiles), dim3(ENM_SORT_THREADS), 0, st,
-                           d_segs, nsegs, w, kin, iin, kout, iout);
-        ++*n_launch;
+                           d_segs, nsegs, w, kin, iin, split, kout, iout);
+        *n_launch += 2;
         e = hipGetLastError();
         if (e != hipSuccess) return e;
         cur ^= 1;

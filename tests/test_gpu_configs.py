@@ -1,0 +1,133 @@
+"""The benchmark configurations at full size on the GPU.
+
+Where the oracle finishes the whole estimate in seconds (C1, C2) the GPU
+matches it exactly (radii, counts) and within 1e-12 (values).  At the large
+sizes (C4, C5a, C5b) two size-independent checks: the radii and counts of a
+random sample of 2,000 points equal cKDTree's bit for bit (trees over all
+points, exactly as ennemi queries them), and each estimate recomputed on the
+host from all exported per-point counts (the oracle's psi and formula)
+equals the device's within 1e-12.
+"""
+
+import numpy as np
+import pytest
+from scipy.spatial import cKDTree
+
+from oracle import ennemi_oracle as orc
+from tests import workloads as wl
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-12
+SAMPLE = 2000
+
+
+def _cols(a):
+    a = np.column_stack((a,))
+    return [np.ascontiguousarray(a[:, c]) for c in range(a.shape[1])]
+
+
+def _sample_check(pts_knn, k, eps, spaces, rng):
+    """pts_knn: joint points; spaces: list of (points, counts) for the ball
+    counts, all in the same point numbering as eps."""
+    n = len(eps)
+    idx = rng.choice(n, min(SAMPLE, n), replace=False)
+    ref_eps = cKDTree(pts_knn).query(pts_knn[idx], k=[k + 1], p=np.inf)[0].ravel()
+    np.testing.assert_array_equal(eps[idx], ref_eps)
+    for pts, cnt in spaces:
+        ref = cKDTree(pts).query_ball_point(pts[idx], ref_eps - 1e-12, p=np.inf, return_length=True)
+        np.testing.assert_array_equal(cnt[idx], ref)
+
+
+def test_c1_full_parity():
+    import ennemi_amd._native as nat
+    from ennemi_amd._batch import Batch
+    x, y = wl.c1()
+    b = Batch()
+    b.add(nat.KIND_KSG, 3, [x], [y])
+    vals, eps, cnts = b.run(export=True)
+    w, ref = orc.estimate_single_mi(x, y, 3, backend="kdtree", detail=True)
+    np.testing.assert_array_equal(eps[0], ref["eps"])
+    np.testing.assert_array_equal(cnts[0][0], ref["nx"])
+    np.testing.assert_array_equal(cnts[0][1], ref["ny"])
+    assert abs(vals[0] - w) <= TOL
+
+
+def test_c2_cmi_full_parity():
+    from ennemi_amd import _estimators as est
+    x, y, z = wl.c2_cmi()
+    v, d = est.estimate_conditional_mi(x, y, z, 3, detail=True)
+    w, ref = orc.estimate_conditional_mi(x, y, z, 3, backend="kdtree", detail=True)
+    np.testing.assert_array_equal(d["eps"], ref["eps"])
+    for key in ("nxz", "nyz", "nz"):
+        np.testing.assert_array_equal(d[key], ref[key], err_msg=key)
+    assert abs(v - w) <= TOL
+
+
+def test_c2_entropy_full_parity():
+    from ennemi_amd import _estimators as est
+    x, cov = wl.c2_entropy()
+    v, d = est.estimate_single_entropy(x, 3, detail=True)
+    w, ref = orc.estimate_single_entropy(x, 3, backend="kdtree", detail=True)
+    np.testing.assert_array_equal(d["eps"], ref["eps"])
+    assert abs(v - w) <= TOL
+    # Sanity: the analytic entropy of the Gaussian.
+    assert abs(v - 0.5 * np.log(np.linalg.det(2 * np.pi * np.e * cov))) < 0.05
+
+
+def test_c4_pairwise_sampled_pairs():
+    import ennemi_amd
+    data = wl.c4()
+    out = ennemi_amd.pairwise_mi(data, k=3)
+    assert out.shape == (64, 64) and np.isnan(np.diag(out)).all()
+    np.testing.assert_array_equal(out, out.T)
+    rng = np.random.default_rng(4)
+    pairs = [(0, 10), (10, 20), (3, 50), (17, 33), (40, 60)] + \
+        [tuple(sorted(rng.choice(64, 2, replace=False))) for _ in range(5)]
+    for i, j in pairs:
+        p = (data[:, i], data[:, j], 0, 0, 0, 3, None, None, np.asarray(0), False, False, True, False)
+        w, _ = orc.lagged_task(p, backend="kdtree")
+        assert abs(out[i, j] - w) <= TOL, (i, j, out[i, j], w)
+
+
+def test_c5a_large_cmi_sampled():
+    import ennemi_amd._native as nat
+    from ennemi_amd._batch import Batch
+    x, y, z = wl.c5a()
+    b = Batch()
+    b.add(nat.KIND_CMI, 5, [x], [y], _cols(z))
+    vals, eps, cnts = b.run(export=True)
+    eps, c = eps[0], cnts[0]
+    xyz = np.column_stack([x, y, z])
+    xz = np.column_stack([x, z])
+    yz = np.column_stack([y, z])
+    _sample_check(xyz, 5, eps, [(xz, c[0]), (yz, c[1]), (z, c[2])], np.random.default_rng(5))
+    ref = float(orc.psi(5) - np.mean(orc.psi(c[0]) + orc.psi(c[1]) - orc.psi(c[2])))
+    assert abs(vals[0] - ref) <= TOL
+
+
+def test_c5b_ross_sampled():
+    import ennemi_amd._native as nat
+    from ennemi_amd import _estimators as est
+    from ennemi_amd._batch import Batch
+    x, y = wl.c5b()
+    values, counts, order, grouped = est._grouped(y, _cols(x))
+    b = Batch()
+    b.add(nat.KIND_ROSS, 3, grouped, class_sizes=counts)
+    vals, eps, cnts = b.run(export=True)
+    inv = np.empty_like(order)
+    inv[order] = np.arange(len(order))
+    eps_pts = eps[0][inv]
+    n_full = cnts[0][0][inv]
+    rng = np.random.default_rng(6)
+    idx = rng.choice(len(y), SAMPLE, replace=False)
+    for cls in np.unique(y[idx]):
+        members = np.flatnonzero(y == cls)
+        sub = idx[y[idx] == cls]
+        ref = cKDTree(x[members]).query(x[sub], k=[4], p=np.inf)[0].ravel()
+        np.testing.assert_array_equal(eps_pts[sub], ref)
+    ref_cnt = cKDTree(x).query_ball_point(x[idx], eps_pts[idx] - 1e-12, p=np.inf, return_length=True)
+    np.testing.assert_array_equal(n_full[idx], ref_cnt)
+    n = len(y)
+    ref = float(orc.psi(n) + orc.psi(3) - np.mean(orc.psi(n_full.astype(float)))
+                - np.sum(np.dot(orc.psi(counts), counts / n)))
+    assert abs(vals[0] - ref) <= TOL
