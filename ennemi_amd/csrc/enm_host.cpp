@@ -704,7 +704,7 @@ int enqueue(enm_ctx* ctx, Plan& P, const double* arena, double* out, double* d_e
     if ((rc = chk(enm_launch_gather(tab<GatherItem>(P, P.off_gitems), (int)P.gitems.size(),
                                     tab<SortSeg>(P, P.off_segs), tab<OrderDev>(P, P.off_orders), arena, skey,
                                     sidx, P.col64.as<double>(), P.col32.as<float>(), P.perm.as<int>(),
-                                    P.maxabs.as<unsigned long long>(), &nl, st), "gather")))
+                                    P.maxabs.as<unsigned long long>(), P.f32 ? 1 : 0, &nl, st), "gather")))
         return rc;
     if (!P.segs2.empty()) {
         // Second stage: sort each grid strip by the base order's column 1
@@ -719,7 +719,7 @@ int enqueue(enm_ctx* ctx, Plan& P, const double* arena, double* out, double* d_e
         if ((rc = chk(enm_launch_gather(tab<GatherItem>(P, P.off_gitems2), (int)P.gitems2.size(),
                                         tab<SortSeg>(P, P.off_segs2), tab<OrderDev>(P, P.off_orders), arena,
                                         skey2, sidx2, P.col64.as<double>(), P.col32.as<float>(),
-                                        P.perm.as<int>(), P.maxabs.as<unsigned long long>(), &nl, st),
+                                        P.perm.as<int>(), P.maxabs.as<unsigned long long>(), P.f32 ? 1 : 0, &nl, st),
                       "gather")))
             return rc;
     }

@@ -16,7 +16,7 @@ hipError_t enm_launch_gather(const GatherItem* d_items, int nitems, const SortSe
                              const OrderDev* d_orders, const double* arena,
                              const double* skey, const int* sidx,
                              double* col64, float* col32, int* perm,
-                             unsigned long long* maxabs, int* n_launch, hipStream_t st);
+                             unsigned long long* maxabs, int with32, int* n_launch, hipStream_t st);
 
 int enm_knn_dim_class(int d);
 int enm_knn_list_class(int k1);

@@ -177,7 +177,8 @@ void merge_pass_kernel(const SortSeg* __restrict__ segs, int nsegs, int64_t widt
 }
 
 // Lay one sorted chunk out into an order: permutation, key column and the
-// other gathered columns (FP64 and FP32), and fold |value| into the
+// other gathered columns (FP64, and FP32 copies when a kernel will read
+// them: with32), and fold |value| into the
 // instance's maximum magnitude (for the FP32 guard band).  For a grid order
 // the sort payload is a base-order position in the same strip: the point and
 // every column are copied from the base order (strip-local reads; the base
@@ -189,7 +190,7 @@ void gather_kernel(const GatherItem* __restrict__ items,
                    const double* __restrict__ arena,
                    const double* __restrict__ skey, const int* __restrict__ sidx,
                    double* col64, float* col32,
-                   int* perm, unsigned long long* __restrict__ maxabs) {
+                   int* perm, unsigned long long* __restrict__ maxabs, int with32) {
     __shared__ double red[ENM_SORT_THREADS / ENM_WAVE];
     const GatherItem it = items[blockIdx.x];
     const SortSeg sg = segs[it.sseg];
@@ -207,19 +208,19 @@ void gather_kernel(const GatherItem* __restrict__ items,
             perm[od.perm + p] = perm[od.base_perm + idx];
             for (int c = 0; c < od.ncol; ++c) {
                 col64[od.c64 + c * n + p] = col64[od.base_c64 + c * n + idx];
-                col32[od.c32 + c * n + p] = col32[od.base_c32 + c * n + idx];
+                if (with32) col32[od.c32 + c * n + p] = col32[od.base_c32 + c * n + idx];
             }
             continue;
         }
         perm[od.perm + p] = idx;
         const double key = skey[sp];
         col64[od.c64 + p] = key;
-        col32[od.c32 + p] = (float)key;
+        if (with32) col32[od.c32 + p] = (float)key;
         m = fmax(m, fabs(key));
         for (int c = 1; c < od.ncol; ++c) {
             const double v = arena[od.src[c] + idx];
             col64[od.c64 + c * n + p] = v;
-            col32[od.c32 + c * n + p] = (float)v;
+            if (with32) col32[od.c32 + c * n + p] = (float)v;
             m = fmax(m, fabs(v));
         }
     }
@@ -294,10 +295,10 @@ hipError_t enm_launch_gather(const GatherItem* d_items, int nitems, const SortSe
                              const OrderDev* d_orders, const double* arena,
                              const double* skey, const int* sidx,
                              double* col64, float* col32, int* perm,
-                             unsigned long long* maxabs, int* n_launch, hipStream_t st) {
+                             unsigned long long* maxabs, int with32, int* n_launch, hipStream_t st) {
     if (nitems <= 0) return hipSuccess;
     hipLaunchKernelGGL(gather_kernel, dim3(nitems), dim3(ENM_SORT_THREADS), 0, st,
-                       d_items, d_segs, d_orders, arena, skey, sidx, col64, col32, perm, maxabs);
+                       d_items, d_segs, d_orders, arena, skey, sidx, col64, col32, perm, maxabs, with32);
     ++*n_launch;
     return hipGetLastError();
 }
