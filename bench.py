@@ -131,6 +131,9 @@ def main() -> int:
     ap.add_argument("--no-compare", action="store_true")
     ap.add_argument("--cpu-lags", type=int, default=12, help="lags in the CPU baseline sample")
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--fp64", action="store_true", help="headline batch in FP64 mode (no FP32 filter)")
+    ap.add_argument("--tuning", default=None,
+                    help="strip_points,waves_per_cu to use instead of the autotuner")
     ap.add_argument("--config", default=None,
                     help="time another configuration instead (c1 c2 c3 c4 c5a c5b); one JSON line each")
     ap.add_argument("--copy-calib", action="store_true",
@@ -173,10 +176,13 @@ def main() -> int:
 
     # Headline: batched, FP32 filter + FP64 recheck, sorted-window pruning.
     t0 = time.perf_counter()
-    db = batch.device(nat.F_TIMING)
+    db = batch.device(nat.F_TIMING | (nat.F_FP64 if args.fp64 else 0))
     upload_s = time.perf_counter() - t0
+    if args.tuning:
+        sp, wv = (int(v) for v in args.tuning.split(","))
+        db.ctx.set_tuning(sp, wv)
     tuning = db.ctx.get_tuning()
-    if not args.no_autotune:
+    if not (args.no_autotune or args.tuning):
         t0 = time.perf_counter()
         tuning = db.autotune()
         log(f"autotuned strip={tuning[0]} waves/CU={tuning[1]} in {time.perf_counter() - t0:.1f} s")

@@ -9,11 +9,11 @@
 // contiguous run of the sorted key column around the query, because
 // fl64(v_j - v_i) is monotone in v_j on each side.  Each query binary-searches
 // both boundaries in an LDS copy of the keys around its item, with the same
-// FP64 predicate the reference evaluates, so counts are exact; a boundary
-// beyond the staged keys is finished in global memory with 8 independent
-// probes per round.  Algorithmic traffic: 8 B radius + 8 B key + 4 B count
-// per query.  In every mode (including ENM_F_BRUTE, which only removes window
-// pruning) 1-D counts use this search.
+// FP64 predicate the reference evaluates, so counts are exact; the few balls
+// wider than the staged keys are searched afterwards in global memory.
+// Algorithmic traffic: 8 B radius + 8 B key + 4 B count per query.  In every
+// mode (including ENM_F_BRUTE, which only removes window pruning) 1-D counts
+// use this search.
 //
 // Multi-D spaces: on a grid order (the space's first two columns are the
 // order's columns 0 and 1) a wave takes 64 queries of one strip and scans its
@@ -28,10 +28,14 @@
 // ISA FP32 and FP64 compares issue at the same rate, so the grid path counts
 // in FP64 directly: the band bookkeeping would cost more than it saves.)
 //
-// Each item also writes the sum over its 64 queries of psi(n_i) and a flag
-// for n_i == 0 (psi(0) = +inf in the reference, :338-339).
+// psi(n_i) comes from a table filled once per launch (psi_tab[c], +inf for
+// c = 0: psi(0) = +inf in the reference, :338-339).  Multi-D spaces write it
+// in point order for a fixed-order reduction; 1-D spaces are summed per
+// ENM_PSI_TILE positions of their sorted order with a flag for n_i == 0.
 #include "enm_device.h"
 #include "enm_launch.h"
+
+#include <algorithm>
 
 namespace {
 
@@ -39,12 +43,13 @@ constexpr int CH = 8;
 
 struct CountArgs {
     const Item* items;
-    int n_items;
-    unsigned* counter;
+    const int* qoff;    // ENM_NQ + 1 queue offsets into items
+    unsigned* counter;  // ENM_NQ ticket counters
     const CountDev* descs;
     const OrderDev* orders;
     const unsigned long long* maxabs;
     const int* perm;
+    const double* psi_tab;  // psi_tab[c] = psi(c), psi_tab[0] = +inf
     int brute;
 };
 
@@ -56,14 +61,16 @@ __device__ __forceinline__ double query_radius(const CountDev& cd, const CountAr
 
 __device__ __forceinline__ void finish_item(const CountDev& cd, const Item& it, bool active,
                                             int cnt, int* __restrict__ res32,
-                                            double* __restrict__ res64, int i, const int* perm) {
+                                            double* __restrict__ res64, int i, const int* perm,
+                                            const double* __restrict__ psi_tab) {
     if (active) res32[cd.counts + i] = cnt;
     if (cd.psi_pt >= 0) {
-        // Point order; a zero count is +inf, as psi(0) poisons the mean.
-        if (active) res64[cd.psi_pt + perm[cd.eps_perm + i]] = cnt > 0 ? enm_psi((double)cnt) : INFINITY;
+        // Point order; a zero count is +inf (psi_tab[0]), as psi(0) poisons
+        // the mean.
+        if (active) res64[cd.psi_pt + perm[cd.eps_perm + i]] = psi_tab[cnt];
         return;
     }
-    const double p = (active && cnt > 0) ? enm_psi((double)cnt) : 0.0;
+    const double p = (active && cnt > 0) ? psi_tab[cnt] : 0.0;
     const double s = wave_sum(p);
     const int z = wave_or(active && cnt == 0);
     if (threadIdx.x == 0) {
@@ -72,69 +79,10 @@ __device__ __forceinline__ void finish_item(const CountDev& cd, const Item& it, 
     }
 }
 
-// Search state of one side of a 1-D ball: P(good) holds, P(bad) fails (or
-// bad is the sentinel just outside the scope range), where for the right
-// side P(j) = fl(v[j] - x) <= r and for the left side P(j) = fl(x - v[j]) <= r.
-// Each round probes 8 positions whose loads are independent, so a lane waits
-// on about 5 rounds of memory latency instead of ~20 dependent probes:
-// first an exponential phase (offsets step*2^m, m = 0..7, step growing by
-// 256 while every probe holds), then an 8-ary refinement of (good, bad).
-struct Side {
-    int good, bad;   // positions; direction is +1 (right) or -1 (left)
-    long long step;
-    bool expo;
-};
-
-template <int DIR>
-__device__ __forceinline__ bool side_pred(const double* __restrict__ v, int j, double x, double r) {
-    return DIR > 0 ? (v[j] - x <= r) : (x - v[j] <= r);
-}
-
-template <int DIR>
-__device__ __forceinline__ void side_round(Side& S, const double* __restrict__ v, int lo, int hi,
-                                           double x, double r) {
-    if (S.expo) {
-        int pos[8];
-        bool ok[8];
-#pragma unroll
-        for (int m = 0; m < 8; ++m) {
-            const long long p = (long long)S.good + DIR * S.step * (1ll << m);
-            const bool in = DIR > 0 ? (p < hi) : (p >= lo);
-            pos[m] = in ? (int)p : S.bad;
-            ok[m] = in && side_pred<DIR>(v, pos[m], x, r);
-        }
-        int first_fail = 8;
-#pragma unroll
-        for (int m = 7; m >= 0; --m) if (!ok[m]) first_fail = m;
-        if (first_fail == 8) {
-            S.good = pos[7];
-            S.step <<= 8;
-        } else {
-            if (first_fail > 0) S.good = pos[first_fail - 1];
-            S.bad = pos[first_fail];
-            S.expo = false;
-        }
-        return;
-    }
-    const int L = DIR > 0 ? S.bad - S.good : S.good - S.bad;
-    if (L <= 1) return;
-    int last_ok = S.good, first_fail = S.bad;
-#pragma unroll
-    for (int m = 0; m < 8; ++m) {
-        // Offsets ceil((m+1) L / 9), strictly inside (good, bad).
-        const int off = min(L - 1, (int)(((long long)(m + 1) * L + 8) / 9));
-        const int q = S.good + DIR * off;
-        const bool okm = side_pred<DIR>(v, q, x, r);
-        if (okm && (DIR > 0 ? q > last_ok : q < last_ok)) last_ok = q;
-        if (!okm && (DIR > 0 ? q < first_fail : q > first_fail)) first_fail = q;
-    }
-    S.good = last_ok;
-    S.bad = first_fail;
-}
-
 // Boundary search inside an LDS copy w[0, nw) of the window: last position
 // in [i, nw) with fl(w[j] - x) <= r (right) or first in [0, i] with
-// fl(x - w[j]) <= r (left); position i holds x itself.
+// fl(x - w[j]) <= r (left); position i holds x itself.  fl(v - x) is monotone
+// in v, so the predicate holds on a prefix (right) or suffix (left).
 __device__ __forceinline__ int lds_right(const double* w, int i, int nw, double x, double r) {
     int good = i, bad = nw;
     while (bad - good > 1) {
@@ -152,82 +100,166 @@ __device__ __forceinline__ int lds_left(const double* w, int i, double x, double
     return good;
 }
 
-// Finish a boundary search in global memory from a known state (the 8-probe
-// rounds of side_round).
-template <int DIR>
-__device__ __forceinline__ int global_side(const double* __restrict__ v, int good, int bad, int lo,
-                                           int hi, double x, double r) {
-    Side S{good, bad, 1, true};
-    for (;;) {
-        const bool done = !S.expo && (DIR > 0 ? S.bad - S.good : S.good - S.bad) <= 1;
-        if (done) break;
-        side_round<DIR>(S, v, lo, hi, x, r);
+// The same searches over the whole scope range [lo, hi) in global memory.
+__device__ __forceinline__ int global_right(const double* __restrict__ v, int i, int hi, double x, double r) {
+    int good = i, bad = hi;
+    while (bad - good > 1) {
+        const int mid = (good + bad) >> 1;
+        if (v[mid] - x <= r) good = mid; else bad = mid;
     }
-    return S.good;
+    return good;
+}
+__device__ __forceinline__ int global_left(const double* __restrict__ v, int lo, int i, double x, double r) {
+    int good = i, bad = lo - 1;
+    while (good - bad > 1) {
+        const int mid = (good + bad) >> 1;
+        if (x - v[mid] <= r) good = mid; else bad = mid;
+    }
+    return good;
 }
 
 // 1-D counts.  An item is ENM_C1_ITEM consecutive positions of a key-sorted
-// order, ENM_C1_Q per lane.  The wave copies the keys of the item and
-// ENM_C1_HALO positions on each side into LDS with coalesced loads, and each
-// query binary-searches both boundaries there.  A boundary that lies beyond
-// the staged keys (the query's predicate still holds at the window's edge)
-// is finished from that edge in global memory, so a few wide balls do not
-// widen the window of the whole item.
-__global__ __launch_bounds__(ENM_WAVE)
-void count1d_kernel(CountArgs A, const double* __restrict__ col64, double* __restrict__ res64,
-                    int* __restrict__ res32, unsigned long long* __restrict__ stats) {
+// order, ENM_C1_Q per thread of a 256-thread workgroup.  The workgroup
+// copies the keys of the item and ENM_C1_HALO positions on each side into
+// LDS with coalesced loads (32 KB, shared by its four waves), and each
+// query binary-searches both boundaries there.  A query whose ball reaches
+// past the staged keys on either side (its predicate still holds at the
+// window's edge, about 1 % of queries on Gaussian-like data, but in almost
+// every item) is deferred: its code (item << ENM_C1_OFF_BITS | offset) goes
+// to a list that count1d_deferred_kernel then searches over the whole scope
+// range with every lane busy, instead of stalling the rest of its wave.  The
+// list has ENM_NDEF segments with a counter each (workgroup b appends to
+// segment b % ENM_NDEF, one atomic per wave and round), so appends do not
+// queue on one address; a full segment makes the query search in place.
+// Counts only: the psi sums of 1-D spaces are taken afterwards per
+// ENM_PSI_TILE positions (psi_tile_kernel), in a fixed order.
+__global__ __launch_bounds__(ENM_C1_THREADS)
+void count1d_kernel(CountArgs A, const double* __restrict__ col64, const double* __restrict__ res64,
+                    int* __restrict__ res32, unsigned long long* __restrict__ stats,
+                    unsigned* __restrict__ deferred, unsigned* __restrict__ n_deferred, unsigned seg_cap) {
     __shared__ double win[ENM_C1_CAP];
-    const int lane = threadIdx.x;
-    for (;;) {
-        const unsigned idx = pop_ticket(A.counter);
-        if (idx >= (unsigned)A.n_items) break;
+    __shared__ unsigned s_ticket;
+    const int tid = threadIdx.x;
+    const int seg = (int)(blockIdx.x % ENM_NDEF);
+    unsigned* dcount = n_deferred + seg;
+    unsigned* dlist = deferred + (size_t)seg * seg_cap;
+    unsigned long long queries = 0;
+    for_each_block_item(A.qoff, A.counter, &s_ticket, [&](int idx) {
         Item it = A.items[idx];
-        it.desc = uni(it.desc); it.tile = uni(it.tile);
+        it.desc = uni(it.desc);
         it.lo = uni(it.lo); it.hi = uni(it.hi); it.t0 = uni(it.t0); it.t1 = uni(it.t1);
-        const CountDev cd = A.descs[it.desc];
-        const OrderDev od = A.orders[cd.order];
-        const double* v = col64 + od.c64;  // sorted key column
+        const CountDev& cd = A.descs[it.desc];
+        const double* v = col64 + A.orders[cd.order].c64;  // sorted key column
+        const int64_t counts = cd.counts;
         const int w0 = max(it.lo, it.t0 - ENM_C1_HALO);
         const int w1 = min(it.hi, it.t1 + ENM_C1_HALO);
         const int nw = w1 - w0;
-        for (int t = lane; t < nw; t += ENM_WAVE) win[t] = v[w0 + t];
+        for (int t = tid; t < nw; t += ENM_C1_THREADS) win[t] = v[w0 + t];
         __syncthreads();
-        double ps = 0.0;
-        bool zero = false;
-#pragma unroll
+        const double wlo = win[0], whi = win[nw - 1];
+#pragma unroll 2
         for (int q = 0; q < ENM_C1_Q; ++q) {
-            const int i = it.t0 + lane + ENM_WAVE * q;
-            if (i < it.t1) {
-                const double r = query_radius(cd, A, res64, i);
-                int cnt = 0;
+            const int i = it.t0 + tid + ENM_C1_THREADS * q;
+            const bool active = i < it.t1;
+            int cnt = 0;
+            bool defer = false;
+            double r = 0.0;
+            if (active) {
+                r = query_radius(cd, A, res64, i);
                 if (!(r < 0.0)) {
                     const int p = i - w0;
                     const double x = win[p];
-                    int right, left;
-                    if (w1 < it.hi && win[nw - 1] - x <= r)
-                        right = global_side<1>(v, w1 - 1, it.hi, it.lo, it.hi, x, r);
-                    else
-                        right = w0 + lds_right(win, p, nw, x, r);
-                    if (w0 > it.lo && x - win[0] <= r)
-                        left = global_side<-1>(v, w0, it.lo - 1, it.lo, it.hi, x, r);
-                    else
-                        left = w0 + lds_left(win, p, x, r);
-                    cnt = right - left + 1;
+                    defer = (w1 < it.hi && whi - x <= r) || (w0 > it.lo && x - wlo <= r);
+                    if (!defer) cnt = lds_right(win, p, nw, x, r) - lds_left(win, p, x, r) + 1;
                 }
-                res32[cd.counts + i] = cnt;
-                if (cnt > 0) ps += enm_psi((double)cnt);
-                else zero = true;
             }
+            const unsigned long long m = __builtin_amdgcn_ballot_w64(defer);
+            if (m != 0) {
+                // One atomic per wave: every lane adds, lane 0 the wave's
+                // total, the others 0; lane 0's old value is the base.
+                const unsigned tot = (unsigned)__popcll(m);
+                const unsigned base = (unsigned)__builtin_amdgcn_readfirstlane(
+                    (int)atomicAdd(dcount, (tid & (ENM_WAVE - 1)) == 0 ? tot : 0u));
+                const unsigned rank = __builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32),
+                                                                __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
+                if (defer) {
+                    const unsigned slot = base + rank;
+                    if (slot < seg_cap) {
+                        dlist[slot] = ((unsigned)idx << ENM_C1_OFF_BITS) | (unsigned)(i - it.t0);
+                    } else {
+                        const double x = v[i];
+                        cnt = global_right(v, i, it.hi, x, r) - global_left(v, it.lo, i, x, r) + 1;
+                        defer = false;
+                    }
+                }
+            }
+            if (active && !defer) res32[counts + i] = cnt;
         }
-        __syncthreads();
-        const double s = wave_sum(ps);
-        const int z = wave_or(zero);
-        if (lane == 0) {
-            res64[cd.part + it.tile] = s;
-            res32[cd.zero + it.tile] = z;
-            atomicAdd(stats + ST_CNT_1D, (unsigned long long)(it.t1 - it.t0));
+        queries += (unsigned long long)(it.t1 - it.t0);
+        __syncthreads();  // the window is reloaded by the next item
+    });
+    if (tid == 0) atomicAdd(stats + ST_CNT_1D, queries);
+}
+
+// Deferred 1-D queries: one per thread, both boundaries by binary search
+// over the whole scope range (L2-resident key column).
+__global__ __launch_bounds__(256)
+void count1d_deferred_kernel(CountArgs A, const double* __restrict__ col64, const double* __restrict__ res64,
+                             int* __restrict__ res32, const unsigned* __restrict__ deferred,
+                             const unsigned* __restrict__ n_deferred, unsigned seg_cap) {
+    const unsigned stride = gridDim.x * blockDim.x;
+    for (int s = 0; s < ENM_NDEF; ++s) {
+        const unsigned n = min(n_deferred[s], seg_cap);
+        const unsigned* list = deferred + (size_t)s * seg_cap;
+        for (unsigned e = blockIdx.x * blockDim.x + threadIdx.x; e < n; e += stride) {
+            const unsigned code = list[e];
+            const Item it = A.items[code >> ENM_C1_OFF_BITS];
+            const CountDev& cd = A.descs[it.desc];
+            const double* v = col64 + A.orders[cd.order].c64;
+            const int i = it.t0 + (int)(code & ((1u << ENM_C1_OFF_BITS) - 1));
+            const double r = query_radius(cd, A, res64, i);
+            const double x = v[i];
+            res32[cd.counts + i] = global_right(v, i, it.hi, x, r) - global_left(v, it.lo, i, x, r) + 1;
         }
     }
+}
+
+// psi sums of 1-D spaces: one wave per ENM_PSI_TILE positions of a space's
+// key-sorted order, 8 per lane in lane-major order, then a butterfly sum;
+// psi values come from the table psi_tab[c] = psi(c).  A zero count sets the
+// tile's flag (psi(0) poisons the reference's mean).
+__global__ __launch_bounds__(ENM_WAVE)
+void psi_tile_kernel(const Item* __restrict__ items, int n_items, const CountDev* __restrict__ descs,
+                     const double* __restrict__ psi_tab, const int* __restrict__ counts_in,
+                     double* __restrict__ res64, int* __restrict__ res32) {
+    const int item = blockIdx.x;
+    if (item >= n_items) return;
+    const Item it = items[item];
+    const CountDev& cd = descs[it.desc];
+    const int lane = threadIdx.x;
+    double ps = 0.0;
+    bool zero = false;
+#pragma unroll
+    for (int q = 0; q < ENM_PSI_TILE / ENM_WAVE; ++q) {
+        const int i = it.t0 + lane + ENM_WAVE * q;
+        if (i < it.t1) {
+            const int c = counts_in[cd.counts + i];
+            if (c > 0) ps += psi_tab[c];
+            else zero = true;
+        }
+    }
+    const double s = wave_sum(ps);
+    const int z = wave_or(zero);
+    if (lane == 0) {
+        res64[cd.part + it.tile] = s;
+        res32[cd.zero + it.tile] = z;
+    }
+}
+
+// psi_tab[c] = psi(c) for c in [1, n], psi_tab[0] = +inf.
+__global__ void psi_table_kernel(double* __restrict__ tab, int64_t n) {
+    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c <= n) tab[c] = c == 0 ? INFINITY : enm_psi((double)c);
 }
 
 // Count of candidates j in [a, b) with max_d fl64(|c_d[j] - q_d|) <= r, for
@@ -327,7 +359,7 @@ template <int D, bool F32>
 __device__ void countnd_item(const CountArgs& A, const Item& it, const CountDev& cd,
                              const double* __restrict__ col64, const float* __restrict__ col32,
                              double* __restrict__ res64, int* __restrict__ res32,
-                             unsigned long long* __restrict__ stats) {
+                             unsigned long long& pairs, unsigned long long& rechecks) {
     const OrderDev od = A.orders[cd.order];
     const int64_t n = od.n;
     const int i = it.t0 + (int)threadIdx.x;
@@ -355,9 +387,8 @@ __device__ void countnd_item(const CountArgs& A, const Item& it, const CountDev&
         if (cd.strip > 0) {
             int cnt = 0;
             const int visited = count_grid<D>(cd, col64, c64, q64, r, live, it.lo, it.hi, it.t0, it.t1, cnt);
-            finish_item(cd, it, active, cnt, res32, res64, i, A.perm);
-            if (threadIdx.x == 0)
-                atomicAdd(stats + ST_CNT_PAIRS, (unsigned long long)(it.t1 - it.t0) * (unsigned long long)visited);
+            finish_item(cd, it, active, cnt, res32, res64, i, A.perm, A.psi_tab);
+            pairs += (unsigned long long)(it.t1 - it.t0) * (unsigned long long)visited;
             return;
         }
     }
@@ -417,8 +448,7 @@ __device__ void countnd_item(const CountArgs& A, const Item& it, const CountDev&
                 }
             }
         }
-        const unsigned long long rcs = wave_sum_u64(rc);
-        if (threadIdx.x == 0) atomicAdd(stats + ST_CNT_RECHECK, rcs);
+        rechecks += wave_sum_u64(rc);
     } else {
         int j = w0;
         for (; j + CH <= w1; j += CH) {
@@ -437,9 +467,8 @@ __device__ void countnd_item(const CountArgs& A, const Item& it, const CountDev&
             cnt += (m <= r);
         }
     }
-    finish_item(cd, it, active, cnt, res32, res64, i, A.perm);
-    if (threadIdx.x == 0)
-        atomicAdd(stats + ST_CNT_PAIRS, (unsigned long long)(it.t1 - it.t0) * (unsigned long long)(w1 - w0));
+    finish_item(cd, it, active, cnt, res32, res64, i, A.perm, A.psi_tab);
+    pairs += (unsigned long long)(it.t1 - it.t0) * (unsigned long long)(w1 - w0);
 }
 
 template <int D, bool F32>
@@ -447,9 +476,9 @@ __global__ __launch_bounds__(ENM_WAVE)
 void countnd_kernel(CountArgs A, const double* __restrict__ col64, const float* __restrict__ col32,
                     double* __restrict__ res64, int* __restrict__ res32,
                     unsigned long long* __restrict__ stats) {
-    for (;;) {
-        const unsigned idx = pop_ticket(A.counter);
-        if (idx >= (unsigned)A.n_items) break;
+    // Counters accumulate per wave and go out once (see knn_kernel).
+    unsigned long long pairs = 0, rechecks = 0;
+    for_each_wave_item(A.qoff, A.counter, [&](int idx) {
         Item it = A.items[idx];
         it.desc = uni(it.desc); it.tile = uni(it.tile);
         it.lo = uni(it.lo); it.hi = uni(it.hi); it.t0 = uni(it.t0); it.t1 = uni(it.t1);
@@ -457,11 +486,15 @@ void countnd_kernel(CountArgs A, const double* __restrict__ col64, const float* 
         if (F32) {
             bool ok = true;
             (void)enm_guard(A.maxabs[A.orders[cd.order].inst], &ok);
-            if (ok) countnd_item<D, true>(A, it, cd, col64, col32, res64, res32, stats);
-            else    countnd_item<D, false>(A, it, cd, col64, col32, res64, res32, stats);
+            if (ok) countnd_item<D, true>(A, it, cd, col64, col32, res64, res32, pairs, rechecks);
+            else    countnd_item<D, false>(A, it, cd, col64, col32, res64, res32, pairs, rechecks);
         } else {
-            countnd_item<D, false>(A, it, cd, col64, col32, res64, res32, stats);
+            countnd_item<D, false>(A, it, cd, col64, col32, res64, res32, pairs, rechecks);
         }
+    });
+    if (threadIdx.x == 0) {
+        atomicAdd(stats + ST_CNT_PAIRS, pairs);
+        if (F32) atomicAdd(stats + ST_CNT_RECHECK, rechecks);
     }
 }
 
@@ -487,20 +520,27 @@ int enm_count_dim_class(int d) {
     return -1;
 }
 
-hipError_t enm_launch_count(const Item* items, int n_items, unsigned* counter,
+hipError_t enm_launch_count(const Item* items, const int* qoff, int n_items, unsigned* counter,
                             const CountDev* descs, const OrderDev* orders,
-                            const unsigned long long* maxabs, const int* perm,
+                            const unsigned long long* maxabs, const int* perm, const double* psi_tab,
                             const double* col64, const float* col32, double* res64, int* res32,
                             unsigned long long* stats, int dclass, bool f32, bool brute,
-                            int grid, hipStream_t st) {
+                            int grid, unsigned* deferred, unsigned* n_deferred, unsigned seg_cap,
+                            hipStream_t st) {
     if (n_items <= 0) return hipSuccess;
-    CountArgs a{items, n_items, counter, descs, orders, maxabs, perm, brute ? 1 : 0};
+    CountArgs a{items, qoff, counter, descs, orders, maxabs, perm, psi_tab, brute ? 1 : 0};
     if (dclass == 1) {
-        hipLaunchKernelGGL(count1d_kernel, dim3(grid), dim3(ENM_WAVE), 0, st, a, col64, res64, res32, stats);
+        // `grid` counts waves; a 1-D workgroup has four.
+        const int blocks = std::max(1, std::min(n_items, grid / (ENM_C1_THREADS / ENM_WAVE)));
+        hipLaunchKernelGGL(count1d_kernel, dim3(blocks), dim3(ENM_C1_THREADS), 0, st, a, col64, res64, res32,
+                           stats, deferred, n_deferred, seg_cap);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+        hipLaunchKernelGGL(count1d_deferred_kernel, dim3(std::max(1, grid / 4)), dim3(256), 0, st, a, col64,
+                           res64, res32, deferred, n_deferred, seg_cap);
         return hipGetLastError();
     }
     switch (dclass) {
-        case 1: return launch_nd<1>(a, f32, grid, col64, col32, res64, res32, stats, st);
         case 2: return launch_nd<2>(a, f32, grid, col64, col32, res64, res32, stats, st);
         case 3: return launch_nd<3>(a, f32, grid, col64, col32, res64, res32, stats, st);
         case 4: return launch_nd<4>(a, f32, grid, col64, col32, res64, res32, stats, st);
@@ -511,4 +551,19 @@ hipError_t enm_launch_count(const Item* items, int n_items, unsigned* counter,
         case 16: return launch_nd<16>(a, f32, grid, col64, col32, res64, res32, stats, st);
         default: return hipErrorInvalidValue;
     }
+}
+
+hipError_t enm_launch_psi_table(double* tab, int64_t n, hipStream_t st) {
+    const int threads = 256;
+    const int64_t blocks = (n + threads) / threads;
+    hipLaunchKernelGGL(psi_table_kernel, dim3((unsigned)blocks), dim3(threads), 0, st, tab, n);
+    return hipGetLastError();
+}
+
+hipError_t enm_launch_psi_tiles(const Item* items, int n_items, const CountDev* descs, const double* psi_tab,
+                                const int* counts, double* res64, int* res32, hipStream_t st) {
+    if (n_items <= 0) return hipSuccess;
+    hipLaunchKernelGGL(psi_tile_kernel, dim3(n_items), dim3(ENM_WAVE), 0, st, items, n_items, descs, psi_tab,
+                       counts, res64, res32);
+    return hipGetLastError();
 }

@@ -89,13 +89,28 @@ __device__ __forceinline__ int wave_or(int v) {
     return __builtin_amdgcn_ballot_w64(v != 0) != 0;
 }
 
+// FP64 min / max of values that are never NaN (distances, +-inf).  fmin /
+// fmax make the compiler quiet possible signalling NaNs first (one extra
+// v_max_f64 per operand carried across loop iterations, under the IEEE mode
+// compute kernels run in); these are the bare instructions.
+__device__ __forceinline__ double vmin(double a, double b) {
+    double r;
+    asm("v_min_f64 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b));
+    return r;
+}
+__device__ __forceinline__ double vmax(double a, double b) {
+    double r;
+    asm("v_max_f64 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b));
+    return r;
+}
+
 // Insert v into the ascending list L[0..P) keeping the P smallest.  Slots
 // below P-K1 hold -inf, so L[P-1] is the K1-th smallest value seen.
 template <int P>
 __device__ __forceinline__ void list_insert(double (&L)[P], double v) {
 #pragma unroll
-    for (int t = P - 1; t > 0; --t) L[t] = fmin(L[t], fmax(v, L[t - 1]));
-    L[0] = fmin(L[0], v);
+    for (int t = P - 1; t > 0; --t) L[t] = vmin(L[t], vmax(v, L[t - 1]));
+    L[0] = vmin(L[0], v);
 }
 
 __device__ __forceinline__ double uni_f64(double v) {
@@ -106,7 +121,7 @@ __device__ __forceinline__ double uni_f64(double v) {
 }
 
 __device__ __forceinline__ void cex(double& a, double& b) {
-    const double lo = fmin(a, b), hi = fmax(a, b);
+    const double lo = vmin(a, b), hi = vmax(a, b);
     a = lo;
     b = hi;
 }
@@ -129,7 +144,7 @@ template <int P>
 __device__ __forceinline__ void list_merge8(double (&L)[P], const double (&e)[8]) {
     static_assert(P == 4 || P == 8, "merge network for 4 or 8 slots");
 #pragma unroll
-    for (int t = 0; t < P; ++t) L[t] = fmin(L[t], e[P - 1 - t]);
+    for (int t = 0; t < P; ++t) L[t] = vmin(L[t], e[P - 1 - t]);
 #pragma unroll
     for (int half = P / 2; half > 0; half >>= 1) {
 #pragma unroll
@@ -154,4 +169,49 @@ __device__ __forceinline__ int upper_bound_u(const double* __restrict__ v, int l
         if (v[mid] <= x) lo = mid + 1; else hi = mid;
     }
     return lo;
+}
+
+// Persistent work loop of a wave64 workgroup over ENM_NQ queues.  The host
+// places each instance's items in one queue (large instances split into a
+// few contiguous blocks), balancing the queues by work; a workgroup starts on
+// queue blockIdx.x % ENM_NQ, which the round-robin dispatch of workgroups
+// over the XCDs maps to one XCD, so an instance's columns are read into one
+// XCD's L2 only.  A drained queue sends the workgroup on to the next one, so
+// every queue empties whatever the placement.  Each ticket counter only
+// grows, so a workgroup that found a queue empty never finds work there
+// again: every wave leaves after at most ENM_NQ empty pops.
+template <class F>
+__device__ __forceinline__ void for_each_wave_item(const int* __restrict__ qoff, unsigned* counter, F&& f) {
+    const int home = (int)(blockIdx.x % ENM_NQ);
+    for (int r = 0; r < ENM_NQ; ++r) {
+        const int q = uni((home + r) % ENM_NQ);
+        const int qb = uni(qoff[q]);
+        const int qn = uni(qoff[q + 1]) - qb;
+        for (;;) {
+            const unsigned t = pop_ticket(counter + q);
+            if (t >= (unsigned)qn) break;
+            f(qb + (int)t);
+        }
+    }
+}
+
+// The same for a multi-wave workgroup: thread 0 pops, the ticket is passed
+// through LDS between two barriers and made wave-uniform.
+template <class F>
+__device__ __forceinline__ void for_each_block_item(const int* __restrict__ qoff, unsigned* counter,
+                                                    unsigned* s_ticket, F&& f) {
+    const int home = (int)(blockIdx.x % ENM_NQ);
+    for (int r = 0; r < ENM_NQ; ++r) {
+        const int q = uni((home + r) % ENM_NQ);
+        const int qb = uni(qoff[q]);
+        const int qn = uni(qoff[q + 1]) - qb;
+        for (;;) {
+            if (threadIdx.x == 0) *s_ticket = atomicAdd(counter + q, 1u);
+            __syncthreads();
+            const unsigned t = (unsigned)__builtin_amdgcn_readfirstlane((int)*s_ticket);
+            __syncthreads();
+            if (t >= (unsigned)qn) break;
+            f(qb + (int)t);
+        }
+    }
 }

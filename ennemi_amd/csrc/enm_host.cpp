@@ -58,7 +58,9 @@ struct Group {
     int pclass;
     int first;      // first item in the kernel's item list
     int n_items;
-    int counter;    // slot in the counter array
+    int counter;    // first of its ENM_NQ + ENM_NDEF slots in the counter array
+                    // (queue tickets, then the deferred 1-D query counts)
+    int qoff;       // first of its ENM_NQ + 1 queue offsets in the offset table
     int inst;       // instance (per-instance mode) or -1
 };
 
@@ -90,6 +92,9 @@ struct Plan {
     std::vector<std::pair<int, int>> inst_ritems;   // per instance: first item, count
     std::vector<double> cls;
     std::vector<Item> kitems, citems;
+    std::vector<Item> sitems;                        // psi tiles of 1-D spaces
+    std::vector<std::pair<int, int>> inst_sitems;    // per instance: first item, count
+    std::vector<int> qoffs;                          // queue offsets of every group
     std::vector<Group> groups;
     std::vector<ExportInst> exports;
     int ntiles = 0, ntiles2 = 0;
@@ -97,13 +102,17 @@ struct Plan {
     int64_t maxlen = 0, maxlen2 = 0;
     int64_t sort_elems = 0, sort_elems2 = 0;
     int64_t n_col64 = 0, n_col32 = 0, n_perm = 0, n_res64 = 0, n_res32 = 0;
+    int64_t psi_tab = 0, psi_n = 0;   // res64 offset of psi(0..psi_n)
+    int64_t max_c1_queries = 0;       // 1-D count queries of the largest group
+    unsigned deferred_seg = 0;        // capacity of one deferred 1-D query list segment
     int n_inst = 0;
     bool f32 = true, brute = false, per_instance = false;
     // device
-    DevBuf tables, sk0, si0, sk1, si1, split, col64, col32, perm, res64, res32, maxabs, counters, stats;
+    DevBuf tables, sk0, si0, sk1, si1, split, col64, col32, perm, res64, res32, maxabs, counters, stats,
+        deferred;
     size_t off_segs = 0, off_gitems = 0, off_orders = 0, off_kdescs = 0, off_cdescs = 0, off_insts = 0,
            off_cls = 0, off_kitems = 0, off_citems = 0, off_segs2 = 0, off_gitems2 = 0, off_rdescs = 0,
-           off_ritems = 0, table_bytes = 0;
+           off_ritems = 0, off_sitems = 0, off_qoffs = 0, table_bytes = 0;
     void* pinned = nullptr;
     hipEvent_t upload_done = nullptr;
     hipGraphExec_t graph = nullptr;
@@ -469,12 +478,15 @@ int plan_batch(enm_ctx* ctx, Plan& P, const enm_instance* inst, int n_inst, cons
             for (int t0 = rg.first; t0 < rg.second; t0 += ENM_TILE) {
                 PendingItem pi;
                 pi.it = Item{kdi, ktiles++, rg.first, rg.second, t0, std::min(t0 + ENM_TILE, rg.second)};
-                pi.cost = P.brute ? (int64_t)(rg.second - rg.first) : (int64_t)I.n;
+                // Work per tile: the scope range in brute force, about
+                // constant with window pruning.
+                pi.cost = P.brute ? (int64_t)(rg.second - rg.first) : 1;
                 pi.inst = ii; pi.dclass = dcl; pi.pclass = pcl;
                 mine.push_back(pi);
             }
         }
         const int red_first = (int)P.ritems.size();
+        const int sit_first = (int)P.sitems.size();
         if (kd.kl) {
             // Entropy: log radii summed in point order.
             int chunks = 0;
@@ -512,16 +524,22 @@ int plan_batch(enm_ctx* ctx, Plan& P, const enm_instance* inst, int n_inst, cons
             std::vector<std::pair<int, int>> rgs;
             if (sp.segmented) rgs = pl.ranges(sp.order);
             else rgs.emplace_back(0, (int)I.n);
-            // 1-D counts take ENM_C1_ITEM positions per item, multi-D ENM_TILE.
+            // 1-D counts take ENM_C1_ITEM positions per item (and psi tiles
+            // of ENM_PSI_TILE), multi-D ENM_TILE.
             const int step = ccl == 1 ? ENM_C1_ITEM : ENM_TILE;
             for (auto rg : rgs) {
                 for (int t0 = rg.first; t0 < rg.second; t0 += step) {
                     PendingItem pi;
-                    pi.it = Item{cdi, ctiles++, rg.first, rg.second, t0, std::min(t0 + step, rg.second)};
-                    pi.cost = P.brute ? (int64_t)(rg.second - rg.first) : (int64_t)I.n;
+                    pi.it = Item{cdi, ccl == 1 ? 0 : ctiles++, rg.first, rg.second, t0,
+                                 std::min(t0 + step, rg.second)};
+                    pi.cost = P.brute && ccl != 1 ? (int64_t)(rg.second - rg.first) : (int64_t)step;
                     pi.inst = ii; pi.dclass = ccl; pi.pclass = 0;
                     cpend.push_back(pi);
                 }
+                if (ccl == 1)
+                    for (int t0 = rg.first; t0 < rg.second; t0 += ENM_PSI_TILE)
+                        P.sitems.push_back(Item{cdi, ctiles++, rg.first, rg.second, t0,
+                                                std::min(t0 + ENM_PSI_TILE, rg.second)});
             }
             if (ccl == 1) {
                 // 1-D spaces sum per tile of their key-sorted order.
@@ -549,6 +567,8 @@ int plan_batch(enm_ctx* ctx, Plan& P, const enm_instance* inst, int n_inst, cons
         }
         ex.nspace = (int)spaces.size();
         P.inst_ritems.emplace_back(red_first, (int)P.ritems.size() - red_first);
+        P.inst_sitems.emplace_back(sit_first, (int)P.sitems.size() - sit_first);
+        P.psi_n = std::max<int64_t>(P.psi_n, I.n);
         if (I.kind == ENM_KIND_ROSS) {
             id.seg_len = (int64_t)P.cls.size();
             for (int s = 0; s < I.n_segs; ++s) P.cls.push_back((double)seg_len[I.seg_index + s]);
@@ -557,8 +577,11 @@ int plan_batch(enm_ctx* ctx, Plan& P, const enm_instance* inst, int n_inst, cons
         P.exports.push_back(ex);
     }
 
-    // Work lists: grouped by kernel class, largest first (LPT), or per
-    // instance in per-instance mode.
+    // Work lists: grouped by kernel class (or per instance in per-instance
+    // mode).  Inside a group the items go to ENM_NQ queues, one per XCD: each
+    // instance's items in blocks of consecutive tiles (one block unless the
+    // instance is a large share of the group), blocks placed largest first
+    // on the least-loaded queue.
     auto build = [&](std::vector<PendingItem>& pend, std::vector<Item>& out, int is_count) {
         std::stable_sort(pend.begin(), pend.end(), [&](const PendingItem& a, const PendingItem& b) {
             if (P.per_instance && a.inst != b.inst) return a.inst < b.inst;
@@ -566,11 +589,13 @@ int plan_batch(enm_ctx* ctx, Plan& P, const enm_instance* inst, int n_inst, cons
             if (a.pclass != b.pclass) return a.pclass < b.pclass;
             return a.cost > b.cost;
         });
+        // A deferred 1-D query code holds the item index in its upper bits.
+        const size_t max_group = (size_t)1 << (32 - ENM_C1_OFF_BITS);
         size_t i = 0;
         while (i < pend.size()) {
             size_t j = i;
             while (j < pend.size() && pend[j].dclass == pend[i].dclass && pend[j].pclass == pend[i].pclass &&
-                   (!P.per_instance || pend[j].inst == pend[i].inst))
+                   (!P.per_instance || pend[j].inst == pend[i].inst) && j - i < max_group)
                 ++j;
             Group g;
             g.is_count = is_count;
@@ -578,15 +603,51 @@ int plan_batch(enm_ctx* ctx, Plan& P, const enm_instance* inst, int n_inst, cons
             g.pclass = pend[i].pclass;
             g.first = (int)out.size();
             g.n_items = (int)(j - i);
-            g.counter = (int)P.groups.size();
+            g.counter = (int)P.groups.size() * (ENM_NQ + ENM_NDEF);
             g.inst = P.per_instance ? pend[i].inst : -1;
-            for (size_t t = i; t < j; ++t) out.push_back(pend[t].it);
+            const size_t blk = std::max<size_t>(64, (j - i + 4 * ENM_NQ - 1) / (4 * ENM_NQ));
+            struct Block { int64_t cost; size_t a, b; };
+            std::vector<Block> blocks;
+            for (size_t t = i; t < j;) {
+                size_t u = t;
+                int64_t c = 0;
+                while (u < j && pend[u].inst == pend[t].inst && u - t < blk) c += pend[u++].cost;
+                blocks.push_back(Block{c, t, u});
+                t = u;
+            }
+            std::stable_sort(blocks.begin(), blocks.end(),
+                             [](const Block& a, const Block& b) { return a.cost > b.cost; });
+            std::vector<std::vector<size_t>> queue(ENM_NQ);
+            std::vector<int64_t> load(ENM_NQ, 0);
+            for (const Block& b : blocks) {
+                const int q = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+                load[q] += b.cost;
+                for (size_t t = b.a; t < b.b; ++t) queue[q].push_back(t);
+            }
+            g.qoff = (int)P.qoffs.size();
+            int64_t queries = 0;
+            for (int q = 0; q < ENM_NQ; ++q) {
+                P.qoffs.push_back((int)(out.size() - g.first));
+                for (size_t t : queue[q]) {
+                    out.push_back(pend[t].it);
+                    queries += pend[t].it.t1 - pend[t].it.t0;
+                }
+            }
+            P.qoffs.push_back((int)(out.size() - g.first));
+            if (is_count && g.dclass == 1) P.max_c1_queries = std::max(P.max_c1_queries, queries);
             P.groups.push_back(g);
             i = j;
         }
     };
     build(kpend, P.kitems, 0);
     build(cpend, P.citems, 1);
+    // psi(0..max n) for the count kernels.
+    P.psi_tab = P.n_res64;
+    P.n_res64 += P.psi_n + 1;
+    // Deferred 1-D queries: about 1 % on Gaussian-like data; a full list
+    // makes the rest search in place, so the capacity only bounds memory.
+    P.deferred_seg = (unsigned)std::min<int64_t>(
+        std::max<int64_t>(P.max_c1_queries / 8 / ENM_NDEF, 1024), (1 << 24) / ENM_NDEF);
     return ENM_OK;
 }
 
@@ -613,6 +674,8 @@ int allocate_and_upload(enm_ctx* ctx, Plan& P, hipStream_t st) {
     P.off_gitems2 = append(blob, P.gitems2);
     P.off_rdescs = append(blob, P.rdescs);
     P.off_ritems = append(blob, P.ritems);
+    P.off_sitems = append(blob, P.sitems);
+    P.off_qoffs = append(blob, P.qoffs);
     P.table_bytes = blob.size();
     hipError_t e;
     if (P.upload_done) {
@@ -646,7 +709,8 @@ int allocate_and_upload(enm_ctx* ctx, Plan& P, hipStream_t st) {
     if ((rc = need(P.res64, P.n_res64 * 8, "radii"))) return rc;
     if ((rc = need(P.res32, P.n_res32 * 4, "counts"))) return rc;
     if ((rc = need(P.maxabs, (size_t)P.n_inst * 8, "maxima"))) return rc;
-    if ((rc = need(P.counters, P.groups.size() * 4 + 4, "counters"))) return rc;
+    if ((rc = need(P.counters, P.groups.size() * (ENM_NQ + ENM_NDEF) * 4 + 4, "counters"))) return rc;
+    if ((rc = need(P.deferred, (size_t)P.deferred_seg * ENM_NDEF * 4, "deferred queries"))) return rc;
     if ((rc = need(P.stats, ST_N * 8, "stats"))) return rc;
     e = hipMemcpyAsync(P.tables.p, P.pinned, P.table_bytes, hipMemcpyHostToDevice, st);
     if (e != hipSuccess) return hip_fail(ctx, e, "hipMemcpyAsync(tables)");
@@ -689,8 +753,11 @@ int enqueue(enm_ctx* ctx, Plan& P, const double* arena, double* out, double* d_e
     unsigned long long* stats = P.stats.as<unsigned long long>();
     if ((rc = chk(enm_launch_fill_u64(stats, ST_N, 0ull, st), "fill"))) return rc;
     if ((rc = chk(enm_launch_fill_u64(P.maxabs.as<unsigned long long>(), P.n_inst, 0ull, st), "fill"))) return rc;
-    if ((rc = chk(enm_launch_fill_i32(P.counters.as<int>(), (int64_t)P.groups.size(), 0, st), "fill"))) return rc;
-    nl += 3;
+    if ((rc = chk(enm_launch_fill_i32(P.counters.as<int>(), (int64_t)P.groups.size() * (ENM_NQ + ENM_NDEF), 0, st),
+                  "fill")))
+        return rc;
+    if ((rc = chk(enm_launch_psi_table(P.res64.as<double>() + P.psi_tab, P.psi_n, st), "fill"))) return rc;
+    nl += 4;
     if ((rc = mark(1))) return rc;
     int fb = 0;
     if ((rc = chk(enm_launch_sort(tab<SortSeg>(P, P.off_segs), (int)P.segs.size(), P.ntiles, P.maxlen,
@@ -728,21 +795,33 @@ int enqueue(enm_ctx* ctx, Plan& P, const double* arena, double* out, double* d_e
     const unsigned long long* maxabs = P.maxabs.as<unsigned long long>();
     unsigned* counters = P.counters.as<unsigned>();
     // Per-instance mode interleaves each instance's kNN, counts and combine.
+    const int* qoffs = tab<int>(P, P.off_qoffs);
+    const double* psi_tab = P.res64.as<double>() + P.psi_tab;
     auto run_group = [&](const Group& g) -> int {
         if (!g.is_count) {
-            e = enm_launch_knn(tab<Item>(P, P.off_kitems) + g.first, g.n_items, counters + g.counter,
+            e = enm_launch_knn(tab<Item>(P, P.off_kitems) + g.first, qoffs + g.qoff, g.n_items,
+                               counters + g.counter,
                                tab<KnnDev>(P, P.off_kdescs), orders, maxabs, P.perm.as<int>(),
                                P.col64.as<double>(), P.col32.as<float>(), P.res64.as<double>(), stats,
                                g.dclass, g.pclass, P.f32, P.brute, grid_for(ctx, g.n_items), st);
             ++nl;
             return chk(e, "knn");
         }
-        e = enm_launch_count(tab<Item>(P, P.off_citems) + g.first, g.n_items, counters + g.counter,
-                             tab<CountDev>(P, P.off_cdescs), orders, maxabs, P.perm.as<int>(),
-                             P.col64.as<double>(), P.col32.as<float>(), P.res64.as<double>(),
-                             P.res32.as<int>(), stats, g.dclass, P.f32, P.brute, grid_for(ctx, g.n_items), st);
-        ++nl;
+        e = enm_launch_count(tab<Item>(P, P.off_citems) + g.first, qoffs + g.qoff, g.n_items,
+                             counters + g.counter, tab<CountDev>(P, P.off_cdescs), orders, maxabs,
+                             P.perm.as<int>(), psi_tab, P.col64.as<double>(), P.col32.as<float>(),
+                             P.res64.as<double>(), P.res32.as<int>(), stats, g.dclass, P.f32, P.brute,
+                             grid_for(ctx, g.n_items * (g.dclass == 1 ? ENM_C1_THREADS / 64 : 1)),
+                             P.deferred.as<unsigned>(), counters + g.counter + ENM_NQ, P.deferred_seg, st);
+        nl += g.dclass == 1 ? 2 : 1;
         return chk(e, "count");
+    };
+    auto psi_tiles = [&](int first, int count) -> int {
+        if (count <= 0) return ENM_OK;
+        ++nl;
+        return chk(enm_launch_psi_tiles(tab<Item>(P, P.off_sitems) + first, count, tab<CountDev>(P, P.off_cdescs),
+                                        psi_tab, P.res32.as<int>(), P.res64.as<double>(), P.res32.as<int>(), st),
+                   "psi");
     };
     if (P.per_instance) {
         for (int ii = 0; ii < P.n_inst; ++ii) {
@@ -750,6 +829,7 @@ int enqueue(enm_ctx* ctx, Plan& P, const double* arena, double* out, double* d_e
                 if (g.inst == ii && !g.is_count && (rc = run_group(g))) return rc;
             for (const Group& g : P.groups)
                 if (g.inst == ii && g.is_count && (rc = run_group(g))) return rc;
+            if ((rc = psi_tiles(P.inst_sitems[ii].first, P.inst_sitems[ii].second))) return rc;
             const auto ri = P.inst_ritems[ii];
             if ((rc = chk(enm_launch_reduce_pt(tab<RedItem>(P, P.off_ritems) + ri.first, ri.second,
                                                tab<RedDev>(P, P.off_rdescs), P.res64.as<double>(), st),
@@ -769,6 +849,7 @@ int enqueue(enm_ctx* ctx, Plan& P, const double* arena, double* out, double* d_e
         if ((rc = mark(4))) return rc;
         for (const Group& g : P.groups)
             if (g.is_count && (rc = run_group(g))) return rc;
+        if ((rc = psi_tiles(0, (int)P.sitems.size()))) return rc;
         if ((rc = chk(enm_launch_reduce_pt(tab<RedItem>(P, P.off_ritems), (int)P.ritems.size(),
                                            tab<RedDev>(P, P.off_rdescs), P.res64.as<double>(), st),
                       "reduce")))

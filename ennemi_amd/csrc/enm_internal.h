@@ -21,10 +21,15 @@
 #include <stdint.h>
 
 #define ENM_TILE 64            // queries per work item (one per lane of a wave64)
-#define ENM_C1_Q 8             // queries per lane of a 1-D count item
-#define ENM_C1_ITEM (ENM_TILE * ENM_C1_Q)   // positions per 1-D count item
+#define ENM_NQ 8               // work queues per persistent launch (one per XCD)
+#define ENM_C1_THREADS 256     // threads of a 1-D count workgroup
+#define ENM_C1_Q 8             // queries per thread of a 1-D count item
+#define ENM_C1_ITEM (ENM_C1_THREADS * ENM_C1_Q)   // positions per 1-D count item (2048)
 #define ENM_C1_HALO 1024       // keys staged on each side of a 1-D count item
-#define ENM_C1_CAP (ENM_C1_ITEM + 2 * ENM_C1_HALO)  // LDS window (doubles, 20 KB)
+#define ENM_C1_CAP (ENM_C1_ITEM + 2 * ENM_C1_HALO)  // LDS window (doubles, 32 KB)
+#define ENM_C1_OFF_BITS 11     // position bits of a deferred 1-D query code
+#define ENM_NDEF 64            // segments (and counters) of the deferred 1-D query list
+#define ENM_PSI_TILE 512       // positions per psi partial sum of a 1-D space
 #define ENM_SORT_TILE 2048     // elements per block-sort tile and merge chunk
 #define ENM_SORT_THREADS 256
 #define ENM_MAX_SPACES 3
@@ -120,8 +125,10 @@ struct RedItem {
     int32_t chunk;
 };
 
-// One work item: a tile of up to 64 consecutive order positions [t0, t1)
-// inside the scope range [lo, hi).
+// One work item: a tile of consecutive order positions [t0, t1) inside the
+// scope range [lo, hi) (64 for kNN and multi-D counts, ENM_C1_ITEM for 1-D
+// counts, ENM_PSI_TILE for 1-D psi sums).  A persistent launch takes its
+// items from ENM_NQ queues, queue q being items [qoff[q], qoff[q+1]).
 struct Item {
     int32_t desc;
     int32_t tile;   // tile index within its descriptor (partials slot)

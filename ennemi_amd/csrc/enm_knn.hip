@@ -36,8 +36,8 @@
 // List updates for up to 8 slots are branch-free: the 8 exact distances are
 // sorted by a 19-comparator network and merged into the list.
 //
-// Work distribution.  A persistent grid pops items from an atomic counter in
-// the (largest-first) order the host laid them out.
+// Work distribution.  A persistent grid pops items from ENM_NQ queues, one
+// per XCD (for_each_wave_item), in the order the host laid them out.
 #include "enm_device.h"
 #include "enm_launch.h"
 
@@ -48,8 +48,8 @@ constexpr int SWEEP = 64; // candidates per side per sweep step
 
 struct KnnArgs {
     const Item* items;
-    int n_items;
-    unsigned* counter;
+    const int* qoff;    // ENM_NQ + 1 queue offsets into items
+    unsigned* counter;  // ENM_NQ ticket counters
     const KnnDev* descs;
     const OrderDev* orders;
     const unsigned long long* maxabs;
@@ -86,15 +86,29 @@ struct KnnTile {
         }
     }
 
+    // Candidate coordinates of chunk j (wave-uniform: scalar loads).
+    __device__ __forceinline__ void load64(double (&cx)[D][CH], int j) const {
+#pragma unroll
+        for (int d = 0; d < D; ++d)
+#pragma unroll
+            for (int c = 0; c < CH; ++c) cx[d][c] = c64[d][j + c];
+    }
+
+    __device__ __forceinline__ bool chunk64(int j) {
+        double cx[D][CH];
+        load64(cx, j);
+        return chunk64v(cx);
+    }
+
     // FP64 chunk: exact distances; merge when some lane improves.  Returns
     // whether any lane could improve its list.
-    __device__ __forceinline__ bool chunk64(int j) {
+    __device__ __forceinline__ bool chunk64v(const double (&cx)[D][CH]) {
         double dd[CH];
 #pragma unroll
         for (int c = 0; c < CH; ++c) {
-            double m = fabs(c64[0][j + c] - q64[0]);
+            double m = fabs(cx[0][c] - q64[0]);
 #pragma unroll
-            for (int d = 1; d < D; ++d) m = fmax(m, fabs(c64[d][j + c] - q64[d]));
+            for (int d = 1; d < D; ++d) m = fmax(m, fabs(cx[d][c] - q64[d]));
             dd[c] = m;
         }
         double mn = dd[0];
@@ -177,10 +191,32 @@ struct KnnTile {
         }
     }
 
-    // Process the wave-uniform candidate range [a, b).
+    // Process the wave-uniform candidate range [a, b).  In FP64 mode with
+    // up to two columns the next chunk's coordinates are loaded before the
+    // current chunk is evaluated (software pipelining: the scalar loads of a
+    // chunk often miss the scalar cache, and their L2 latency then overlaps
+    // a chunk of arithmetic instead of stalling the wave); both buffers fit
+    // in SGPRs (2 x 2 x 8 doubles).
     __device__ __forceinline__ void range(int a, int b) {
         int j = a;
-        for (; j + CH <= b; j += CH) chunk(j);
+        if constexpr (!F32 && D <= 2) {
+            if (j + CH <= b) {
+                double cur[D][CH];
+                load64(cur, j);
+                for (; j + CH <= b; j += CH) {
+                    double nxt[D][CH];
+                    // The last chunk reloads itself (in bounds, unused).
+                    load64(nxt, j + 2 * CH <= b ? j + CH : j);
+                    dense = chunk64v(cur);
+#pragma unroll
+                    for (int d = 0; d < D; ++d)
+#pragma unroll
+                        for (int c = 0; c < CH; ++c) cur[d][c] = nxt[d][c];
+                }
+            }
+        } else {
+            for (; j + CH <= b; j += CH) chunk(j);
+        }
         for (; j < b; ++j) single(j);
     }
 };
@@ -273,7 +309,7 @@ __device__ int grid_sweep(KnnTile<D, P, F32>& w, const KnnDev& kd, const double*
 template <int D, int P, bool F32>
 __device__ void knn_item(const KnnArgs& A, const Item& it, const double* __restrict__ col64,
                          const float* __restrict__ col32, double* __restrict__ res64,
-                         unsigned long long* __restrict__ stats) {
+                         unsigned long long& pairs, unsigned long long& rechecks) {
     const KnnDev kd = A.descs[it.desc];
     const OrderDev od = A.orders[kd.order];
     const int lane = threadIdx.x;
@@ -345,21 +381,19 @@ __device__ void knn_item(const KnnArgs& A, const Item& it, const double* __restr
         res64[kd.eps + i] = eps;
         if (kd.eps_orig >= 0) res64[kd.eps_orig + A.perm[od.perm + i]] = eps;
     }
-    const unsigned long long nact = (unsigned long long)(t1 - t0);
-    const unsigned long long rc = wave_sum_u64(w.rechecks);
-    if (lane == 0) {
-        atomicAdd(stats + ST_KNN_PAIRS, nact * (unsigned long long)visited);
-        if (F32) atomicAdd(stats + ST_KNN_RECHECK, rc);
-    }
+    pairs += (unsigned long long)(t1 - t0) * (unsigned long long)visited;
+    if (F32) rechecks += wave_sum_u64(w.rechecks);
 }
 
 template <int D, int P, bool F32>
 __global__ __launch_bounds__(ENM_WAVE)
 void knn_kernel(KnnArgs A, const double* __restrict__ col64, const float* __restrict__ col32,
                 double* __restrict__ res64, unsigned long long* __restrict__ stats) {
-    for (;;) {
-        const unsigned idx = pop_ticket(A.counter);
-        if (idx >= (unsigned)A.n_items) break;
+    // Counters accumulate per wave (the same value in every lane) and go
+    // out once at the end: one atomic per item on one address would queue
+    // a million atomics on a single L2 channel.
+    unsigned long long pairs = 0, rechecks = 0;
+    for_each_wave_item(A.qoff, A.counter, [&](int idx) {
         Item it = A.items[idx];
         it.desc = uni(it.desc); it.tile = uni(it.tile);
         it.lo = uni(it.lo); it.hi = uni(it.hi); it.t0 = uni(it.t0); it.t1 = uni(it.t1);
@@ -367,11 +401,15 @@ void knn_kernel(KnnArgs A, const double* __restrict__ col64, const float* __rest
             const KnnDev& kd = A.descs[it.desc];
             bool ok = true;
             (void)enm_guard(A.maxabs[A.orders[kd.order].inst], &ok);
-            if (ok) knn_item<D, P, true>(A, it, col64, col32, res64, stats);
-            else    knn_item<D, P, false>(A, it, col64, col32, res64, stats);
+            if (ok) knn_item<D, P, true>(A, it, col64, col32, res64, pairs, rechecks);
+            else    knn_item<D, P, false>(A, it, col64, col32, res64, pairs, rechecks);
         } else {
-            knn_item<D, P, false>(A, it, col64, col32, res64, stats);
+            knn_item<D, P, false>(A, it, col64, col32, res64, pairs, rechecks);
         }
+    });
+    if (threadIdx.x == 0) {
+        atomicAdd(stats + ST_KNN_PAIRS, pairs);
+        if (F32) atomicAdd(stats + ST_KNN_RECHECK, rechecks);
     }
 }
 
@@ -417,14 +455,14 @@ int enm_knn_list_class(int k1) {
     return -1;
 }
 
-hipError_t enm_launch_knn(const Item* items, int n_items, unsigned* counter,
+hipError_t enm_launch_knn(const Item* items, const int* qoff, int n_items, unsigned* counter,
                           const KnnDev* descs, const OrderDev* orders,
                           const unsigned long long* maxabs, const int* perm,
                           const double* col64, const float* col32, double* res64,
                           unsigned long long* stats, int dclass, int pclass, bool f32, bool brute,
                           int grid, hipStream_t st) {
     if (n_items <= 0) return hipSuccess;
-    KnnArgs a{items, n_items, counter, descs, orders, maxabs, perm, brute ? 1 : 0};
+    KnnArgs a{items, qoff, counter, descs, orders, maxabs, perm, brute ? 1 : 0};
     switch (dclass) {
         case 1: return launch_d<1>(a, pclass, f32, grid, col64, col32, res64, stats, st);
         case 2: return launch_d<2>(a, pclass, f32, grid, col64, col32, res64, stats, st);

@@ -20,7 +20,7 @@ hipError_t enm_launch_gather(const GatherItem* d_items, int nitems, const SortSe
 
 int enm_knn_dim_class(int d);
 int enm_knn_list_class(int k1);
-hipError_t enm_launch_knn(const Item* items, int n_items, unsigned* counter,
+hipError_t enm_launch_knn(const Item* items, const int* qoff, int n_items, unsigned* counter,
                           const KnnDev* descs, const OrderDev* orders,
                           const unsigned long long* maxabs, const int* perm,
                           const double* col64, const float* col32, double* res64,
@@ -28,12 +28,16 @@ hipError_t enm_launch_knn(const Item* items, int n_items, unsigned* counter,
                           int grid, hipStream_t st);
 
 int enm_count_dim_class(int d);
-hipError_t enm_launch_count(const Item* items, int n_items, unsigned* counter,
+hipError_t enm_launch_count(const Item* items, const int* qoff, int n_items, unsigned* counter,
                             const CountDev* descs, const OrderDev* orders,
-                            const unsigned long long* maxabs, const int* perm,
+                            const unsigned long long* maxabs, const int* perm, const double* psi_tab,
                             const double* col64, const float* col32, double* res64, int* res32,
                             unsigned long long* stats, int dclass, bool f32, bool brute,
-                            int grid, hipStream_t st);
+                            int grid, unsigned* deferred, unsigned* n_deferred, unsigned seg_cap,
+                            hipStream_t st);
+hipError_t enm_launch_psi_table(double* tab, int64_t n, hipStream_t st);
+hipError_t enm_launch_psi_tiles(const Item* items, int n_items, const CountDev* descs, const double* psi_tab,
+                                const int* counts, double* res64, int* res32, hipStream_t st);
 
 hipError_t enm_launch_combine(const InstDev* insts, const double* cls, int first, int count,
                               const double* res64, const int* res32, double* out, hipStream_t st);

@@ -33,11 +33,58 @@ __device__ __forceinline__ int seg_of_tile(const SortSeg* __restrict__ segs, int
     return lo;
 }
 
+// Compare-exchange of two (key, index) pairs.
+__device__ __forceinline__ void cex_pair(double& ka, int& ia, double& kb, int& ib) {
+    const bool sw = kless(kb, ib, ka, ia);
+    const double k0 = sw ? kb : ka, k1 = sw ? ka : kb;
+    const int i0 = sw ? ib : ia, i1 = sw ? ia : ib;
+    ka = k0; ia = i0; kb = k1; ib = i1;
+}
+
+// Sort PER (key, index) pairs held in registers (PER = 2, 4 or 8; Batcher's
+// odd-even merge networks).
+template <int PER>
+__device__ __forceinline__ void sort_regs(double (&k)[PER], int (&x)[PER]) {
+#define ENM_CX(a, b) cex_pair(k[a], x[a], k[b], x[b])
+    if constexpr (PER == 2) {
+        ENM_CX(0, 1);
+    } else if constexpr (PER == 4) {
+        ENM_CX(0, 1); ENM_CX(2, 3); ENM_CX(0, 2); ENM_CX(1, 3); ENM_CX(1, 2);
+    } else {
+        static_assert(PER == 8, "2, 4 or 8 per thread");
+        ENM_CX(0, 1); ENM_CX(2, 3); ENM_CX(4, 5); ENM_CX(6, 7);
+        ENM_CX(0, 2); ENM_CX(1, 3); ENM_CX(4, 6); ENM_CX(5, 7);
+        ENM_CX(1, 2); ENM_CX(5, 6);
+        ENM_CX(0, 4); ENM_CX(1, 5); ENM_CX(2, 6); ENM_CX(3, 7);
+        ENM_CX(2, 4); ENM_CX(3, 5);
+        ENM_CX(1, 2); ENM_CX(3, 4); ENM_CX(5, 6);
+    }
+#undef ENM_CX
+}
+
+// Number of A elements among the first d outputs of merge(A, B).
+__device__ int co_rank(int d, const double* ka, const int* ia, int na,
+                       const double* kb, const int* ib, int nb) {
+    int lo = max(0, d - nb), hi = min(d, na);
+    while (lo < hi) {
+        const int i = (lo + hi) >> 1;
+        const int j = d - i - 1;
+        if (kless(ka[i], ia[i], kb[j], ib[j])) lo = i + 1; else hi = i;
+    }
+    return lo;
+}
+
+// Sort one TILE chunk in a 256-thread block: each thread sorts TILE/256
+// consecutive keys in registers, then merge levels of doubling run width in
+// LDS, each thread producing TILE/256 consecutive outputs of its pair of runs
+// from a merge-path split (outputs kept in registers across a barrier, so the
+// merge works in place).  Padding keys (+inf, INT_MAX) sort last.
 template <int TILE>
 __global__ __launch_bounds__(ENM_SORT_THREADS)
 void block_sort_kernel(const SortSeg* __restrict__ segs, int nsegs,
                        const double* __restrict__ arena,
                        double* __restrict__ kout, int* __restrict__ iout) {
+    constexpr int PER = TILE / ENM_SORT_THREADS;
     __shared__ double sk[TILE];
     __shared__ int si[TILE];
     const int tile = blockIdx.x;
@@ -55,38 +102,44 @@ void block_sort_kernel(const SortSeg* __restrict__ segs, int nsegs,
         }
     }
     __syncthreads();
-    for (int size = 2; size <= TILE; size <<= 1) {
-        for (int stride = size >> 1; stride > 0; stride >>= 1) {
-            for (int q = threadIdx.x; q < TILE / 2; q += ENM_SORT_THREADS) {
-                const int i = 2 * q - (q & (stride - 1));
-                const int j = i + stride;
-                const bool asc = (i & size) == 0;
-                const double ki = sk[i], kj = sk[j];
-                const int ii = si[i], ij = si[j];
-                if (kless(kj, ij, ki, ii) == asc) {
-                    sk[i] = kj; sk[j] = ki;
-                    si[i] = ij; si[j] = ii;
-                }
-            }
-            __syncthreads();
+    const int e0 = threadIdx.x * PER;
+    double k[PER];
+    int x[PER];
+#pragma unroll
+    for (int e = 0; e < PER; ++e) { k[e] = sk[e0 + e]; x[e] = si[e0 + e]; }
+    sort_regs<PER>(k, x);
+    __syncthreads();
+#pragma unroll
+    for (int e = 0; e < PER; ++e) { sk[e0 + e] = k[e]; si[e0 + e] = x[e]; }
+    __syncthreads();
+    for (int run = PER; run < TILE; run <<= 1) {
+        const int pb = (e0 / (2 * run)) * (2 * run);
+        const double* ka = sk + pb;
+        const int* ia = si + pb;
+        const double* kb = ka + run;
+        const int* ib = ia + run;
+        const int d = e0 - pb;
+        int a = co_rank(d, ka, ia, run, kb, ib, run);
+        int b = d - a;
+#pragma unroll
+        for (int e = 0; e < PER; ++e) {
+            // Clamped reads stay inside the pair; the flags decide.
+            const int ac = min(a, run - 1), bc = min(b, run - 1);
+            const bool take_a = b >= run || (a < run && kless(ka[ac], ia[ac], kb[bc], ib[bc]));
+            const int src = take_a ? ac : run + bc;
+            k[e] = ka[src];
+            x[e] = ia[src];
+            if (take_a) ++a; else ++b;
         }
+        __syncthreads();
+#pragma unroll
+        for (int e = 0; e < PER; ++e) { sk[e0 + e] = k[e]; si[e0 + e] = x[e]; }
+        __syncthreads();
     }
     for (int t = threadIdx.x; t < len; t += ENM_SORT_THREADS) {
         kout[sg.sbase + off + t] = sk[t];
         iout[sg.sbase + off + t] = si[t];
     }
-}
-
-// Number of A elements among the first d outputs of merge(A, B).
-__device__ int co_rank(int d, const double* ka, const int* ia, int na,
-                       const double* kb, const int* ib, int nb) {
-    int lo = max(0, d - nb), hi = min(d, na);
-    while (lo < hi) {
-        const int i = (lo + hi) >> 1;
-        const int j = d - i - 1;
-        if (kless(ka[i], ia[i], kb[j], ib[j])) lo = i + 1; else hi = i;
-    }
-    return lo;
 }
 
 // Geometry of chunk `tile` in a merge pass of run width `width`: its pair of
@@ -157,22 +210,41 @@ void merge_pass_kernel(const SortSeg* __restrict__ segs, int nsegs, int64_t widt
         else        { sk[t] = kb[j0 + t - la]; si[t] = ib[j0 + t - la]; }
     }
     __syncthreads();
-    const int per = ENM_SORT_TILE / ENM_SORT_THREADS;
-    const int e0 = threadIdx.x * per;
-    const int e1 = min(e0 + per, la + lb);
-    if (e0 < e1) {
+    // Each thread merges 8 consecutive outputs into registers; they go back
+    // to LDS and out with coalesced stores.
+    constexpr int PER = ENM_SORT_TILE / ENM_SORT_THREADS;
+    const int e0 = threadIdx.x * PER;
+    const int ne = min(PER, la + lb - e0);
+    double ok[PER];
+    int oi[PER];
+    if (ne > 0) {
         int a = co_rank(e0, sk, si, la, sk + la, si + la, lb);
         int b = e0 - a;
-        for (int e = e0; e < e1; ++e) {
-            bool take_a;
-            if (a >= la) take_a = false;
-            else if (b >= lb) take_a = true;
-            else take_a = kless(sk[a], si[a], sk[la + b], si[la + b]);
-            const int src = take_a ? a : la + b;
-            kout[base + d0 + e] = sk[src];
-            iout[base + d0 + e] = si[src];
-            if (take_a) ++a; else ++b;
+#pragma unroll
+        for (int e = 0; e < PER; ++e) {
+            if (e < ne) {
+                // Clamped reads stay inside the staged slices; the flags decide.
+                const int ac = max(0, min(a, la - 1)), bc = la + max(0, min(b, lb - 1));
+                const bool take_a = lb == 0 || b >= lb || (a < la && kless(sk[ac], si[ac], sk[bc], si[bc]));
+                const int src = take_a ? ac : bc;
+                ok[e] = sk[src];
+                oi[e] = si[src];
+                if (take_a) ++a; else ++b;
+            }
         }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int e = 0; e < PER; ++e) {
+        if (e < ne) {
+            sk[e0 + e] = ok[e];
+            si[e0 + e] = oi[e];
+        }
+    }
+    __syncthreads();
+    for (int t = threadIdx.x; t < la + lb; t += ENM_SORT_THREADS) {
+        kout[base + d0 + t] = sk[t];
+        iout[base + d0 + t] = si[t];
     }
 }
 
@@ -194,7 +266,9 @@ void gather_kernel(const GatherItem* __restrict__ items,
     __shared__ double red[ENM_SORT_THREADS / ENM_WAVE];
     const GatherItem it = items[blockIdx.x];
     const SortSeg sg = segs[it.sseg];
-    const OrderDev od = orders[it.order];
+    // Fields read one by one: a local copy of the whole descriptor, indexed
+    // by a loop counter, would live in scratch memory.
+    const OrderDev& od = orders[it.order];
     const int len = min(ENM_SORT_TILE, sg.len - it.off);
     const int64_t n = od.n;
     const bool indirect = od.base_perm >= 0;
