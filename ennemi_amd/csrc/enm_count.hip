@@ -157,15 +157,22 @@ void count1d_kernel(CountArgs A, const double* __restrict__ col64, const double*
         for (int t = tid; t < nw; t += ENM_C1_THREADS) win[t] = v[w0 + t];
         __syncthreads();
         const double wlo = win[0], whi = win[nw - 1];
-#pragma unroll 2
+        // All radii first: their dependent gathers (permutation, then
+        // radius) overlap instead of stalling every query in turn.
+        double rq[ENM_C1_Q];
+#pragma unroll
+        for (int q = 0; q < ENM_C1_Q; ++q) {
+            const int i = it.t0 + tid + ENM_C1_THREADS * q;
+            rq[q] = i < it.t1 ? query_radius(cd, A, res64, i) : -1.0;
+        }
+#pragma unroll
         for (int q = 0; q < ENM_C1_Q; ++q) {
             const int i = it.t0 + tid + ENM_C1_THREADS * q;
             const bool active = i < it.t1;
             int cnt = 0;
             bool defer = false;
-            double r = 0.0;
+            const double r = rq[q];
             if (active) {
-                r = query_radius(cd, A, res64, i);
                 if (!(r < 0.0)) {
                     const int p = i - w0;
                     const double x = win[p];
@@ -201,26 +208,39 @@ void count1d_kernel(CountArgs A, const double* __restrict__ col64, const double*
     if (tid == 0) atomicAdd(stats + ST_CNT_1D, queries);
 }
 
-// Deferred 1-D queries: one per thread, both boundaries by binary search
-// over the whole scope range (L2-resident key column).
+// Deferred 1-D queries: one per thread over all segments at once (entry e
+// of the concatenated segments found through their prefix sums in LDS),
+// both boundaries by binary search over the whole scope range.
 __global__ __launch_bounds__(256)
 void count1d_deferred_kernel(CountArgs A, const double* __restrict__ col64, const double* __restrict__ res64,
                              int* __restrict__ res32, const unsigned* __restrict__ deferred,
                              const unsigned* __restrict__ n_deferred, unsigned seg_cap) {
-    const unsigned stride = gridDim.x * blockDim.x;
-    for (int s = 0; s < ENM_NDEF; ++s) {
-        const unsigned n = min(n_deferred[s], seg_cap);
-        const unsigned* list = deferred + (size_t)s * seg_cap;
-        for (unsigned e = blockIdx.x * blockDim.x + threadIdx.x; e < n; e += stride) {
-            const unsigned code = list[e];
-            const Item it = A.items[code >> ENM_C1_OFF_BITS];
-            const CountDev& cd = A.descs[it.desc];
-            const double* v = col64 + A.orders[cd.order].c64;
-            const int i = it.t0 + (int)(code & ((1u << ENM_C1_OFF_BITS) - 1));
-            const double r = query_radius(cd, A, res64, i);
-            const double x = v[i];
-            res32[cd.counts + i] = global_right(v, i, it.hi, x, r) - global_left(v, it.lo, i, x, r) + 1;
+    __shared__ unsigned pre[ENM_NDEF + 1];
+    if (threadIdx.x == 0) {
+        unsigned acc = 0;
+        for (int s = 0; s < ENM_NDEF; ++s) {
+            pre[s] = acc;
+            acc += min(n_deferred[s], seg_cap);
         }
+        pre[ENM_NDEF] = acc;
+    }
+    __syncthreads();
+    const unsigned total = pre[ENM_NDEF];
+    const unsigned stride = gridDim.x * blockDim.x;
+    for (unsigned e = blockIdx.x * blockDim.x + threadIdx.x; e < total; e += stride) {
+        int lo = 0, hi = ENM_NDEF - 1;   // last segment s with pre[s] <= e
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (pre[mid] <= e) lo = mid; else hi = mid - 1;
+        }
+        const unsigned code = deferred[(size_t)lo * seg_cap + (e - pre[lo])];
+        const Item it = A.items[code >> ENM_C1_OFF_BITS];
+        const CountDev& cd = A.descs[it.desc];
+        const double* v = col64 + A.orders[cd.order].c64;
+        const int i = it.t0 + (int)(code & ((1u << ENM_C1_OFF_BITS) - 1));
+        const double r = query_radius(cd, A, res64, i);
+        const double x = v[i];
+        res32[cd.counts + i] = global_right(v, i, it.hi, x, r) - global_left(v, it.lo, i, x, r) + 1;
     }
 }
 

@@ -641,6 +641,20 @@ int plan_batch(enm_ctx* ctx, Plan& P, const enm_instance* inst, int n_inst, cons
     };
     build(kpend, P.kitems, 0);
     build(cpend, P.citems, 1);
+    // Stage-1 gathers read their columns at random positions of one
+    // instance: interleave the items so that block b (dispatched to XCD
+    // b % ENM_NQ) only serves the instances of queue b % ENM_NQ, and each
+    // instance's columns are cached in one XCD's L2.
+    {
+        std::vector<std::vector<GatherItem>> lists(ENM_NQ);
+        for (const GatherItem& gi : P.gitems) lists[P.orders[gi.order].inst % ENM_NQ].push_back(gi);
+        std::vector<GatherItem> mixed;
+        mixed.reserve(P.gitems.size());
+        for (size_t k = 0; mixed.size() < P.gitems.size(); ++k)
+            for (int q = 0; q < ENM_NQ; ++q)
+                if (k < lists[q].size()) mixed.push_back(lists[q][k]);
+        P.gitems.swap(mixed);
+    }
     // psi(0..max n) for the count kernels.
     P.psi_tab = P.n_res64;
     P.n_res64 += P.psi_n + 1;
