@@ -308,10 +308,11 @@ __device__ __forceinline__ int count_range64(const double* const (&c64)[D], cons
 }
 
 // Grid count: the tile's own strip and then strips outward, each within the
-// y window of the tile's column-1 range widened by the wave's largest radius
-// R.  A side stops at the first strip whose column-0 bound is more than every
-// live lane's radius away (strictly: a count includes distance == r), and
-// all strips beyond it are further still.  Returns candidates visited.
+// union of the y intervals [y_i - r_i, y_i + r_i] of the lanes that can
+// reach it (strip_window).  A side stops at the first strip whose column-0
+// bound is more than every live lane's radius away (strictly: a count
+// includes distance == r), and all strips beyond it are further still.
+// Returns candidates visited.
 template <int D>
 __device__ int count_grid(const CountDev& cd, const double* __restrict__ col64,
                           const double* const (&c64)[D], const double (&q64)[D], double r, bool live,
@@ -322,52 +323,44 @@ __device__ int count_grid(const CountDev& cd, const double* __restrict__ col64,
     const int s = (t0 - lo) / S;
     const int nstrip = (hi - lo + S - 1) / S;
     const int ss = lo + s * S, se = min(hi, ss + S);
-    const double R = uni_f64(wave_max(live ? r : -INFINITY));
-    if (!(R >= 0.0)) return 0;  // no live lane
-    const double ylo = yc[t0], yhi = yc[t1 - 1];
+    if (!wave_any(live)) return 0;
     int visited = 0;
     int a, b;
-    {
-        const double slack = 0x1p-50 * (fmax(fabs(ylo), fabs(yhi)) + R);
-        const double lb = uni_f64((ylo - R * (1.0 + 0x1p-50)) - slack);
-        const double ub = uni_f64((yhi + R * (1.0 + 0x1p-50)) + slack);
-        a = uni(lower_bound_u(yc, ss, se, lb));
-        b = uni(upper_bound_u(yc, a, se, ub));
-        cnt += count_range64<D>(c64, q64, r, a, b);
-        visited += b - a;
-        bool ldone = s == 0, rdone = s == nstrip - 1;
-        for (int d = 1; !(ldone && rdone); ++d) {
-            if (!ldone) {
-                const int sl = s - d;
-                if (sl < 0) {
-                    ldone = true;
-                } else {
-                    const int le = lo + (sl + 1) * S;
-                    const double xb = xk[le - 1];
-                    ldone = wave_all(!live || (q64[0] - xb > r));
-                    if (!ldone) {
-                        a = uni(lower_bound_u(yc, le - S, le, lb));
-                        b = uni(upper_bound_u(yc, a, le, ub));
-                        cnt += count_range64<D>(c64, q64, r, a, b);
-                        visited += b - a;
-                    }
+    strip_window(yc, ss, se, q64[1], r, live, a, b);
+    cnt += count_range64<D>(c64, q64, r, a, b);
+    visited += b - a;
+    bool ldone = s == 0, rdone = s == nstrip - 1;
+    for (int d = 1; !(ldone && rdone); ++d) {
+        if (!ldone) {
+            const int sl = s - d;
+            if (sl < 0) {
+                ldone = true;
+            } else {
+                const int le = lo + (sl + 1) * S;
+                const double xb = xk[le - 1];
+                const bool need = live && !(q64[0] - xb > r);
+                ldone = !wave_any(need);
+                if (!ldone) {
+                    strip_window(yc, le - S, le, q64[1], r, need, a, b);
+                    cnt += count_range64<D>(c64, q64, r, a, b);
+                    visited += b - a;
                 }
             }
-            if (!rdone) {
-                const int sr = s + d;
-                if (sr >= nstrip) {
-                    rdone = true;
-                } else {
-                    const int rs = lo + sr * S;
-                    const int re = min(hi, rs + S);
-                    const double xb = xk[rs];
-                    rdone = wave_all(!live || (xb - q64[0] > r));
-                    if (!rdone) {
-                        a = uni(lower_bound_u(yc, rs, re, lb));
-                        b = uni(upper_bound_u(yc, a, re, ub));
-                        cnt += count_range64<D>(c64, q64, r, a, b);
-                        visited += b - a;
-                    }
+        }
+        if (!rdone) {
+            const int sr = s + d;
+            if (sr >= nstrip) {
+                rdone = true;
+            } else {
+                const int rs = lo + sr * S;
+                const int re = min(hi, rs + S);
+                const double xb = xk[rs];
+                const bool need = live && !(xb - q64[0] > r);
+                rdone = !wave_any(need);
+                if (!rdone) {
+                    strip_window(yc, rs, re, q64[1], r, need, a, b);
+                    cnt += count_range64<D>(c64, q64, r, a, b);
+                    visited += b - a;
                 }
             }
         }
@@ -424,8 +417,7 @@ __device__ void countnd_item(const CountArgs& A, const Item& it, const CountDev&
         ub = wave_max(ub);
         if (lb > ub) { w0 = w1 = it.t0; }
         else {
-            w0 = uni(lower_bound_u(c64[0], it.lo, it.hi, lb));
-            w1 = uni(upper_bound_u(c64[0], w0, it.hi, ub));
+            wave_bounds(c64[0], it.lo, it.hi, lb, ub, w0, w1);
         }
     }
 

@@ -171,6 +171,55 @@ __device__ __forceinline__ int upper_bound_u(const double* __restrict__ v, int l
     return lo;
 }
 
+// Both bounds of [lb, ub] in the ascending keys v[lo, hi) at once, by the
+// whole wave: a = first position with v >= lb, b = first with v > ub (the
+// predicates of lower_bound_u / upper_bound_u).  Lanes 0-31 search a, lanes
+// 32-63 b; each round probes 32 evenly spaced positions of the interval
+// still open, and the ballot of "before the answer" (a prefix of the
+// probes, the keys being sorted) cuts it 32-fold: 2 rounds of independent
+// loads for a 512-key strip instead of 2 x 9 dependent ones.
+__device__ __forceinline__ void wave_bounds(const double* __restrict__ v, int lo, int hi, double lb, double ub,
+                                            int& a, int& b) {
+    const int lane = (int)(threadIdx.x & (ENM_WAVE - 1));
+    const bool upper = lane >= 32;
+    const int l = lane & 31;
+    int L = lo, H = hi;  // the answer lies in [L, H] (H: no key qualifies)
+    for (;;) {
+        const int len = H - L;
+        if (wave_all(len == 0)) break;
+        const int step = (len + 31) >> 5;
+        const int p = L + l * step;
+        const bool in = len > 0 && p < H;
+        const double val = in ? v[p] : 0.0;
+        const bool before = in && (upper ? (val <= ub) : (val < lb));
+        const unsigned long long m = __builtin_amdgcn_ballot_w64(before);
+        const int c = __popc(upper ? (unsigned)(m >> 32) : (unsigned)m);
+        if (len > 0) {
+            const int nl = c == 0 ? L : L + (c - 1) * step + 1;
+            H = min(H, L + c * step);
+            L = nl;
+        }
+    }
+    a = __builtin_amdgcn_readlane(L, 0);
+    b = __builtin_amdgcn_readlane(L, 32);
+}
+
+// Positions [a, b) of the strip [s0, s1) of a y-sorted column that can
+// hold a candidate j with fl(|y_j - y_i|) <= T_i for some live lane i: the
+// union of the lanes' intervals [y_i - T_i, y_i + T_i], each widened by a
+// relative 2^-50 so that the rounding of the bounds (far below the
+// widening) cannot exclude such a candidate.  Wave-uniform result.
+__device__ __forceinline__ void strip_window(const double* __restrict__ yc, int s0, int s1, double yq, double T,
+                                         bool live, int& a, int& b) {
+    const double slack = 0x1p-50 * (fabs(yq) + T);
+    const double lq = live ? (yq - T * (1.0 + 0x1p-50)) - slack : INFINITY;
+    const double uq = live ? (yq + T * (1.0 + 0x1p-50)) + slack : -INFINITY;
+    const double lb = uni_f64(wave_min(lq));
+    const double ub = uni_f64(wave_max(uq));
+    if (!(lb <= ub)) { a = b = s0; return; }
+    wave_bounds(yc, s0, s1, lb, ub, a, b);
+}
+
 // Persistent work loop of a wave64 workgroup over ENM_NQ queues.  The host
 // places each instance's items in one queue (large instances split into a
 // few contiguous blocks), balancing the queues by work; a workgroup starts on

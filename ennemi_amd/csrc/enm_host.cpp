@@ -107,6 +107,9 @@ struct Plan {
     unsigned deferred_seg = 0;        // capacity of one deferred 1-D query list segment
     int n_inst = 0;
     bool f32 = true, brute = false, per_instance = false;
+    // Kernels that read the FP32 column copies: the brute-force kNN and
+    // multi-D counts off the grid (band filter).  Only then are they made.
+    bool need32 = false;
     // device
     DevBuf tables, sk0, si0, sk1, si1, split, col64, col32, perm, res64, res32, maxabs, counters, stats,
         deferred;
@@ -517,6 +520,7 @@ int plan_batch(enm_ctx* ctx, Plan& P, const enm_instance* inst, int n_inst, cons
             cd.eps_perm = P.orders[sp.order].perm;
             cd.counts = P.n_res32;
             P.n_res32 += I.n;
+            if (cd.ncol >= 2 && cd.strip == 0) P.need32 = true;
             const int cdi = (int)P.cdescs.size();
             const int ccl = enm_count_dim_class(cd.ncol);
             cd.psi_pt = -1;
@@ -785,7 +789,8 @@ int enqueue(enm_ctx* ctx, Plan& P, const double* arena, double* out, double* d_e
     if ((rc = chk(enm_launch_gather(tab<GatherItem>(P, P.off_gitems), (int)P.gitems.size(),
                                     tab<SortSeg>(P, P.off_segs), tab<OrderDev>(P, P.off_orders), arena, skey,
                                     sidx, P.col64.as<double>(), P.col32.as<float>(), P.perm.as<int>(),
-                                    P.maxabs.as<unsigned long long>(), P.f32 ? 1 : 0, &nl, st), "gather")))
+                                    P.maxabs.as<unsigned long long>(), P.f32 && P.need32 ? 1 : 0, &nl, st),
+                      "gather")))
         return rc;
     if (!P.segs2.empty()) {
         // Second stage: sort each grid strip by the base order's column 1
@@ -800,7 +805,8 @@ int enqueue(enm_ctx* ctx, Plan& P, const double* arena, double* out, double* d_e
         if ((rc = chk(enm_launch_gather(tab<GatherItem>(P, P.off_gitems2), (int)P.gitems2.size(),
                                         tab<SortSeg>(P, P.off_segs2), tab<OrderDev>(P, P.off_orders), arena,
                                         skey2, sidx2, P.col64.as<double>(), P.col32.as<float>(),
-                                        P.perm.as<int>(), P.maxabs.as<unsigned long long>(), P.f32 ? 1 : 0, &nl, st),
+                                        P.perm.as<int>(), P.maxabs.as<unsigned long long>(),
+                                        P.f32 && P.need32 ? 1 : 0, &nl, st),
                       "gather")))
             return rc;
     }
@@ -817,7 +823,7 @@ int enqueue(enm_ctx* ctx, Plan& P, const double* arena, double* out, double* d_e
                                counters + g.counter,
                                tab<KnnDev>(P, P.off_kdescs), orders, maxabs, P.perm.as<int>(),
                                P.col64.as<double>(), P.col32.as<float>(), P.res64.as<double>(), stats,
-                               g.dclass, g.pclass, P.f32, P.brute, grid_for(ctx, g.n_items), st);
+                               g.dclass, g.pclass, P.f32 && P.brute, P.brute, grid_for(ctx, g.n_items), st);
             ++nl;
             return chk(e, "knn");
         }
@@ -1004,6 +1010,7 @@ int enm_run_batch(enm_ctx* ctx, const enm_instance* inst, int32_t n_inst, const 
     P.key = key;
     P.f32 = (flags & ENM_F_FP64) == 0;
     P.brute = (flags & ENM_F_BRUTE) != 0;
+    P.need32 = P.brute;
     P.per_instance = (flags & ENM_F_PER_INSTANCE) != 0;
     int rc = plan_batch(ctx, P, inst, n_inst, seg_len);
     if (rc) return rc;

@@ -61,6 +61,7 @@ template <int D, int P, bool F32>
 struct KnnTile {
     const double* c64[D];
     const float* c32[D];
+    double* stage;    // the wave's LDS block of 64 candidates, [D][64]
     double q64[D];
     float q32[D];
     double L[P];
@@ -111,6 +112,24 @@ struct KnnTile {
             for (int d = 1; d < D; ++d) m = fmax(m, fabs(cx[d][c] - q64[d]));
             dd[c] = m;
         }
+        return take(dd);
+    }
+
+    // FP64 chunk of the staged block: candidates c0 .. c0+7, read from LDS
+    // with one address per instruction (a broadcast).
+    __device__ __forceinline__ bool chunk_lds(int c0) {
+        double dd[CH];
+#pragma unroll
+        for (int c = 0; c < CH; ++c) dd[c] = fabs(stage[c0 + c] - q64[0]);
+#pragma unroll
+        for (int d = 1; d < D; ++d)
+#pragma unroll
+            for (int c = 0; c < CH; ++c) dd[c] = fmax(dd[c], fabs(stage[d * ENM_WAVE + c0 + c] - q64[d]));
+        return take(dd);
+    }
+
+    // List update with the exact distances of one chunk.
+    __device__ __forceinline__ bool take(double (&dd)[CH]) {
         double mn = dd[0];
 #pragma unroll
         for (int c = 1; c < CH; ++c) mn = fmin(mn, dd[c]);
@@ -198,50 +217,44 @@ struct KnnTile {
     // a chunk of arithmetic instead of stalling the wave); both buffers fit
     // in SGPRs (2 x 2 x 8 doubles).
     __device__ __forceinline__ void range(int a, int b) {
-        int j = a;
-        if constexpr (!F32 && D <= 2) {
-            if (j + CH <= b) {
-                double cur[D][CH];
-                load64(cur, j);
-                for (; j + CH <= b; j += CH) {
-                    double nxt[D][CH];
-                    // The last chunk reloads itself (in bounds, unused).
-                    load64(nxt, j + 2 * CH <= b ? j + CH : j);
-                    dense = chunk64v(cur);
+        if constexpr (!F32) {
+            // FP64: blocks of 64 candidates staged in LDS.  Lane l loads
+            // candidate base + l of every column (coalesced; +inf past b,
+            // which no list takes), the block goes to LDS, the next block's
+            // loads are issued, and the wave evaluates the staged block.
+            if (a >= b) return;
+            const int lane = (int)threadIdx.x;
+            double nv[D];
 #pragma unroll
-                    for (int d = 0; d < D; ++d)
+            for (int d = 0; d < D; ++d) nv[d] = a + lane < b ? c64[d][a + lane] : INFINITY;
+            for (int base = a; base < b; base += ENM_WAVE) {
+                __syncthreads();  // the previous block has been read
 #pragma unroll
-                        for (int c = 0; c < CH; ++c) cur[d][c] = nxt[d][c];
+                for (int d = 0; d < D; ++d) stage[d * ENM_WAVE + lane] = nv[d];
+                __syncthreads();
+                const int nb = base + ENM_WAVE;
+                if (nb < b) {
+#pragma unroll
+                    for (int d = 0; d < D; ++d) nv[d] = nb + lane < b ? c64[d][nb + lane] : INFINITY;
                 }
+                const int cnt = min(ENM_WAVE, b - base);
+                for (int c0 = 0; c0 < cnt; c0 += CH) dense = chunk_lds(c0);
             }
-        } else {
-            for (; j + CH <= b; j += CH) chunk(j);
+            return;
         }
+        int j = a;
+        for (; j + CH <= b; j += CH) chunk(j);
         for (; j < b; ++j) single(j);
     }
 };
 
-// Positions [a, b) of the strip [s0, s1) of a y-sorted column whose values
-// lie within R of [ylo, yhi], widened by a relative 2^-50 so the range is a
-// superset of every candidate with fl(|y_j - y_i|) < R for y_i in [ylo, yhi]
-// (each rounding of the bounds is far below the widening).
-__device__ __forceinline__ void y_window(const double* __restrict__ yc, int s0, int s1, double ylo,
-                                         double yhi, double R, int& a, int& b) {
-    const double slack = 0x1p-50 * (fmax(fabs(ylo), fabs(yhi)) + R);
-    const double lb = uni_f64((ylo - R * (1.0 + 0x1p-50)) - slack);
-    const double ub = uni_f64((yhi + R * (1.0 + 0x1p-50)) + slack);
-    a = uni(lower_bound_u(yc, s0, s1, lb));
-    b = uni(upper_bound_u(yc, a, s1, ub));
-}
-
 // Grid sweep of one tile (already seeded with its own 64 points): the rest
 // of the tile's strip near the tile in y order, then whole strips outward on
-// both sides.  A strip is visited only within the y window of the tile's
-// y range widened by R = the largest current radius of the wave's lanes; a
-// side stops at the first strip whose x bound is at least every lane's
-// radius away (all strips beyond it are further in x).  Each candidate is
-// visited at most once, and every candidate that could still enter a lane's
-// list is visited.  Returns the number of candidates visited.
+// both sides.  A strip is visited only within the y window of the lanes'
+// current radii; a side stops at the first strip whose x bound is at least
+// every lane's radius away (all strips beyond it are further in x).  Each
+// candidate is visited at most once, and every candidate that could still
+// enter a lane's list is visited.  Returns the number of candidates visited.
 template <int D, int P, bool F32>
 __device__ int grid_sweep(KnnTile<D, P, F32>& w, const KnnDev& kd, const double* __restrict__ col64,
                           int lo, int hi, int t0, int t1, bool active) {
@@ -252,17 +265,16 @@ __device__ int grid_sweep(KnnTile<D, P, F32>& w, const KnnDev& kd, const double*
     const int s = (t0 - lo) / S;
     const int nstrip = (hi - lo + S - 1) / S;
     const int ss = lo + s * S, se = min(hi, ss + S);
+    const double yq = w.q64[1];
     int visited = 0;
     // Seed: up to SEED neighbours in y on each side of the tile.
     const int a0 = max(ss, t0 - SEED), b0 = min(se, t1 + SEED);
     w.range(a0, t0);
     w.range(t1, b0);
     visited += (t0 - a0) + (b0 - t1);
-    const double ylo = yc[t0], yhi = yc[t1 - 1];
     {
-        const double R = wave_max(active ? w.T : -INFINITY);
         int a, b;
-        y_window(yc, ss, se, ylo, yhi, R, a, b);
+        strip_window(yc, ss, se, yq, w.T, active, a, b);
         if (a < a0) { w.range(a, a0); visited += a0 - a; }
         if (b > b0) { w.range(b0, b); visited += b - b0; }
     }
@@ -275,11 +287,12 @@ __device__ int grid_sweep(KnnTile<D, P, F32>& w, const KnnDev& kd, const double*
             } else {
                 const int le = lo + (sl + 1) * S;  // strip sl is [le - S, le)
                 const double xb = xk[le - 1];      // largest x of strip sl
-                ldone = wave_all(!(w.q64[0] - xb < w.T));
+                // Lanes whose radius reaches this strip on the key axis.
+                const bool need = w.q64[0] - xb < w.T;
+                ldone = !wave_any(need);
                 if (!ldone) {
-                    const double R = wave_max(active ? w.T : -INFINITY);
                     int a, b;
-                    y_window(yc, le - S, le, ylo, yhi, R, a, b);
+                    strip_window(yc, le - S, le, yq, w.T, need, a, b);
                     w.range(a, b);
                     visited += b - a;
                 }
@@ -292,11 +305,11 @@ __device__ int grid_sweep(KnnTile<D, P, F32>& w, const KnnDev& kd, const double*
             } else {
                 const int rs = lo + sr * S;
                 const double xb = xk[rs];          // smallest x of strip sr
-                rdone = wave_all(!(xb - w.q64[0] < w.T));
+                const bool need = xb - w.q64[0] < w.T;
+                rdone = !wave_any(need);
                 if (!rdone) {
-                    const double R = wave_max(active ? w.T : -INFINITY);
                     int a, b;
-                    y_window(yc, rs, min(hi, rs + S), ylo, yhi, R, a, b);
+                    strip_window(yc, rs, min(hi, rs + S), yq, w.T, need, a, b);
                     w.range(a, b);
                     visited += b - a;
                 }
@@ -308,7 +321,7 @@ __device__ int grid_sweep(KnnTile<D, P, F32>& w, const KnnDev& kd, const double*
 
 template <int D, int P, bool F32>
 __device__ void knn_item(const KnnArgs& A, const Item& it, const double* __restrict__ col64,
-                         const float* __restrict__ col32, double* __restrict__ res64,
+                         const float* __restrict__ col32, double* __restrict__ res64, double* stage,
                          unsigned long long& pairs, unsigned long long& rechecks) {
     const KnnDev kd = A.descs[it.desc];
     const OrderDev od = A.orders[kd.order];
@@ -319,6 +332,7 @@ __device__ void knn_item(const KnnArgs& A, const Item& it, const double* __restr
     const int ncol = od.ncol;
 
     KnnTile<D, P, F32> w;
+    w.stage = stage;
     bool f32ok = true;
     w.guard = enm_guard(A.maxabs[od.inst], &f32ok);
 #pragma unroll
@@ -393,6 +407,7 @@ void knn_kernel(KnnArgs A, const double* __restrict__ col64, const float* __rest
     // out once at the end: one atomic per item on one address would queue
     // a million atomics on a single L2 channel.
     unsigned long long pairs = 0, rechecks = 0;
+    __shared__ __align__(16) double stage[D * ENM_WAVE];
     for_each_wave_item(A.qoff, A.counter, [&](int idx) {
         Item it = A.items[idx];
         it.desc = uni(it.desc); it.tile = uni(it.tile);
@@ -401,10 +416,10 @@ void knn_kernel(KnnArgs A, const double* __restrict__ col64, const float* __rest
             const KnnDev& kd = A.descs[it.desc];
             bool ok = true;
             (void)enm_guard(A.maxabs[A.orders[kd.order].inst], &ok);
-            if (ok) knn_item<D, P, true>(A, it, col64, col32, res64, pairs, rechecks);
-            else    knn_item<D, P, false>(A, it, col64, col32, res64, pairs, rechecks);
+            if (ok) knn_item<D, P, true>(A, it, col64, col32, res64, stage, pairs, rechecks);
+            else    knn_item<D, P, false>(A, it, col64, col32, res64, stage, pairs, rechecks);
         } else {
-            knn_item<D, P, false>(A, it, col64, col32, res64, pairs, rechecks);
+            knn_item<D, P, false>(A, it, col64, col32, res64, stage, pairs, rechecks);
         }
     });
     if (threadIdx.x == 0) {
