@@ -141,7 +141,7 @@ void count1d_kernel(CountArgs A, const double* __restrict__ col64, const double*
     __shared__ unsigned s_ticket;
     const int tid = threadIdx.x;
     const int seg = (int)(blockIdx.x % ENM_NDEF);
-    unsigned* dcount = n_deferred + seg;
+    unsigned* dcount = n_deferred + seg * ENM_CSTRIDE;
     unsigned* dlist = deferred + (size_t)seg * seg_cap;
     unsigned long long queries = 0;
     for_each_block_item(A.qoff, A.counter, &s_ticket, [&](int idx) {
@@ -220,7 +220,7 @@ void count1d_deferred_kernel(CountArgs A, const double* __restrict__ col64, cons
         unsigned acc = 0;
         for (int s = 0; s < ENM_NDEF; ++s) {
             pre[s] = acc;
-            acc += min(n_deferred[s], seg_cap);
+            acc += min(n_deferred[s * ENM_CSTRIDE], seg_cap);
         }
         pre[ENM_NDEF] = acc;
     }
@@ -490,7 +490,7 @@ void countnd_kernel(CountArgs A, const double* __restrict__ col64, const float* 
                     unsigned long long* __restrict__ stats) {
     // Counters accumulate per wave and go out once (see knn_kernel).
     unsigned long long pairs = 0, rechecks = 0;
-    for_each_wave_item(A.qoff, A.counter, [&](int idx) {
+    for_each_wave_item(A.qoff, A.counter, ENM_KCHUNK, [&](int idx) {
         Item it = A.items[idx];
         it.desc = uni(it.desc); it.tile = uni(it.tile);
         it.lo = uni(it.lo); it.hi = uni(it.hi); it.t0 = uni(it.t0); it.t1 = uni(it.t1);

@@ -226,26 +226,34 @@ __device__ __forceinline__ void strip_window(const double* __restrict__ yc, int 
 // queue blockIdx.x % ENM_NQ, which the round-robin dispatch of workgroups
 // over the XCDs maps to one XCD, so an instance's columns are read into one
 // XCD's L2 only.  A drained queue sends the workgroup on to the next one, so
-// every queue empties whatever the placement.  Each ticket counter only
-// grows, so a workgroup that found a queue empty never finds work there
-// again: every wave leaves after at most ENM_NQ empty pops.
+// every queue empties whatever the placement.  A ticket is `chunk`
+// consecutive items: the atomics on a queue's counter (each counter on its
+// own 128-byte line, ENM_CSTRIDE slots apart) serialize where the line
+// lives, so one per item would bound the whole launch at about ten
+// nanoseconds per item.  Each counter only grows, so a workgroup that found
+// a queue empty never finds work there again: every wave leaves after at
+// most ENM_NQ empty pops.
 template <class F>
-__device__ __forceinline__ void for_each_wave_item(const int* __restrict__ qoff, unsigned* counter, F&& f) {
+__device__ __forceinline__ void for_each_wave_item(const int* __restrict__ qoff, unsigned* counter, int chunk,
+                                                   F&& f) {
     const int home = (int)(blockIdx.x % ENM_NQ);
     for (int r = 0; r < ENM_NQ; ++r) {
         const int q = uni((home + r) % ENM_NQ);
         const int qb = uni(qoff[q]);
         const int qn = uni(qoff[q + 1]) - qb;
         for (;;) {
-            const unsigned t = pop_ticket(counter + q);
-            if (t >= (unsigned)qn) break;
-            f(qb + (int)t);
+            const unsigned t = pop_ticket(counter + q * ENM_CSTRIDE);
+            const long long first = (long long)t * chunk;
+            if (first >= qn) break;
+            const int last = (int)min(first + chunk, (long long)qn);
+            for (int k = (int)first; k < last; ++k) f(qb + k);
         }
     }
 }
 
-// The same for a multi-wave workgroup: thread 0 pops, the ticket is passed
-// through LDS between two barriers and made wave-uniform.
+// The same for a multi-wave workgroup, one item per ticket: thread 0 pops,
+// the ticket is passed through LDS between two barriers and made
+// wave-uniform.
 template <class F>
 __device__ __forceinline__ void for_each_block_item(const int* __restrict__ qoff, unsigned* counter,
                                                     unsigned* s_ticket, F&& f) {
@@ -255,7 +263,7 @@ __device__ __forceinline__ void for_each_block_item(const int* __restrict__ qoff
         const int qb = uni(qoff[q]);
         const int qn = uni(qoff[q + 1]) - qb;
         for (;;) {
-            if (threadIdx.x == 0) *s_ticket = atomicAdd(counter + q, 1u);
+            if (threadIdx.x == 0) *s_ticket = atomicAdd(counter + q * ENM_CSTRIDE, 1u);
             __syncthreads();
             const unsigned t = (unsigned)__builtin_amdgcn_readfirstlane((int)*s_ticket);
             __syncthreads();

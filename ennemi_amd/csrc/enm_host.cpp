@@ -58,10 +58,12 @@ struct Group {
     int pclass;
     int first;      // first item in the kernel's item list
     int n_items;
-    int counter;    // first of its ENM_NQ + ENM_NDEF slots in the counter array
-                    // (queue tickets, then the deferred 1-D query counts)
+    int counter;    // first of its ENM_NQ + ENM_NDEF counters in the counter array,
+                    // ENM_CSTRIDE slots apart (queue tickets, then the
+                    // deferred 1-D query counts)
     int qoff;       // first of its ENM_NQ + 1 queue offsets in the offset table
     int inst;       // instance (per-instance mode) or -1
+    int walk;       // kNN items on grid orders (per-lane walk kernel)
 };
 
 struct ExportSpace {
@@ -331,6 +333,7 @@ struct PendingItem {
     int64_t cost;
     int inst;
     int dclass, pclass;
+    int walk = 0;   // kNN on a grid order: the per-lane walk kernel
 };
 
 int plan_batch(enm_ctx* ctx, Plan& P, const enm_instance* inst, int n_inst, const int64_t* seg_len) {
@@ -484,7 +487,7 @@ int plan_batch(enm_ctx* ctx, Plan& P, const enm_instance* inst, int n_inst, cons
                 // Work per tile: the scope range in brute force, about
                 // constant with window pruning.
                 pi.cost = P.brute ? (int64_t)(rg.second - rg.first) : 1;
-                pi.inst = ii; pi.dclass = dcl; pi.pclass = pcl;
+                pi.inst = ii; pi.dclass = dcl; pi.pclass = pcl; pi.walk = grid ? 1 : 0;
                 mine.push_back(pi);
             }
         }
@@ -591,6 +594,7 @@ int plan_batch(enm_ctx* ctx, Plan& P, const enm_instance* inst, int n_inst, cons
             if (P.per_instance && a.inst != b.inst) return a.inst < b.inst;
             if (a.dclass != b.dclass) return a.dclass < b.dclass;
             if (a.pclass != b.pclass) return a.pclass < b.pclass;
+            if (a.walk != b.walk) return a.walk < b.walk;
             return a.cost > b.cost;
         });
         // A deferred 1-D query code holds the item index in its upper bits.
@@ -599,7 +603,8 @@ int plan_batch(enm_ctx* ctx, Plan& P, const enm_instance* inst, int n_inst, cons
         while (i < pend.size()) {
             size_t j = i;
             while (j < pend.size() && pend[j].dclass == pend[i].dclass && pend[j].pclass == pend[i].pclass &&
-                   (!P.per_instance || pend[j].inst == pend[i].inst) && j - i < max_group)
+                   pend[j].walk == pend[i].walk && (!P.per_instance || pend[j].inst == pend[i].inst) &&
+                   j - i < max_group)
                 ++j;
             Group g;
             g.is_count = is_count;
@@ -607,8 +612,9 @@ int plan_batch(enm_ctx* ctx, Plan& P, const enm_instance* inst, int n_inst, cons
             g.pclass = pend[i].pclass;
             g.first = (int)out.size();
             g.n_items = (int)(j - i);
-            g.counter = (int)P.groups.size() * (ENM_NQ + ENM_NDEF);
+            g.counter = (int)P.groups.size() * (ENM_NQ + ENM_NDEF) * ENM_CSTRIDE;
             g.inst = P.per_instance ? pend[i].inst : -1;
+            g.walk = pend[i].walk;
             const size_t blk = std::max<size_t>(64, (j - i + 4 * ENM_NQ - 1) / (4 * ENM_NQ));
             struct Block { int64_t cost; size_t a, b; };
             std::vector<Block> blocks;
@@ -727,7 +733,7 @@ int allocate_and_upload(enm_ctx* ctx, Plan& P, hipStream_t st) {
     if ((rc = need(P.res64, P.n_res64 * 8, "radii"))) return rc;
     if ((rc = need(P.res32, P.n_res32 * 4, "counts"))) return rc;
     if ((rc = need(P.maxabs, (size_t)P.n_inst * 8, "maxima"))) return rc;
-    if ((rc = need(P.counters, P.groups.size() * (ENM_NQ + ENM_NDEF) * 4 + 4, "counters"))) return rc;
+    if ((rc = need(P.counters, P.groups.size() * (ENM_NQ + ENM_NDEF) * ENM_CSTRIDE * 4 + 4, "counters"))) return rc;
     if ((rc = need(P.deferred, (size_t)P.deferred_seg * ENM_NDEF * 4, "deferred queries"))) return rc;
     if ((rc = need(P.stats, ST_N * 8, "stats"))) return rc;
     e = hipMemcpyAsync(P.tables.p, P.pinned, P.table_bytes, hipMemcpyHostToDevice, st);
@@ -771,7 +777,8 @@ int enqueue(enm_ctx* ctx, Plan& P, const double* arena, double* out, double* d_e
     unsigned long long* stats = P.stats.as<unsigned long long>();
     if ((rc = chk(enm_launch_fill_u64(stats, ST_N, 0ull, st), "fill"))) return rc;
     if ((rc = chk(enm_launch_fill_u64(P.maxabs.as<unsigned long long>(), P.n_inst, 0ull, st), "fill"))) return rc;
-    if ((rc = chk(enm_launch_fill_i32(P.counters.as<int>(), (int64_t)P.groups.size() * (ENM_NQ + ENM_NDEF), 0, st),
+    if ((rc = chk(enm_launch_fill_i32(P.counters.as<int>(),
+                                      (int64_t)P.groups.size() * (ENM_NQ + ENM_NDEF) * ENM_CSTRIDE, 0, st),
                   "fill")))
         return rc;
     if ((rc = chk(enm_launch_psi_table(P.res64.as<double>() + P.psi_tab, P.psi_n, st), "fill"))) return rc;
@@ -823,7 +830,7 @@ int enqueue(enm_ctx* ctx, Plan& P, const double* arena, double* out, double* d_e
                                counters + g.counter,
                                tab<KnnDev>(P, P.off_kdescs), orders, maxabs, P.perm.as<int>(),
                                P.col64.as<double>(), P.col32.as<float>(), P.res64.as<double>(), stats,
-                               g.dclass, g.pclass, P.f32 && P.brute, P.brute, grid_for(ctx, g.n_items), st);
+                               g.dclass, g.pclass, P.f32 && P.brute, P.brute, g.walk, grid_for(ctx, g.n_items), st);
             ++nl;
             return chk(e, "knn");
         }
@@ -832,7 +839,8 @@ int enqueue(enm_ctx* ctx, Plan& P, const double* arena, double* out, double* d_e
                              P.perm.as<int>(), psi_tab, P.col64.as<double>(), P.col32.as<float>(),
                              P.res64.as<double>(), P.res32.as<int>(), stats, g.dclass, P.f32, P.brute,
                              grid_for(ctx, g.n_items * (g.dclass == 1 ? ENM_C1_THREADS / 64 : 1)),
-                             P.deferred.as<unsigned>(), counters + g.counter + ENM_NQ, P.deferred_seg, st);
+                             P.deferred.as<unsigned>(), counters + g.counter + ENM_NQ * ENM_CSTRIDE,
+                             P.deferred_seg, st);
         nl += g.dclass == 1 ? 2 : 1;
         return chk(e, "count");
     };

@@ -29,6 +29,8 @@
 #define ENM_C1_CAP (ENM_C1_ITEM + 2 * ENM_C1_HALO)  // LDS window (doubles, 32 KB)
 #define ENM_C1_OFF_BITS 11     // position bits of a deferred 1-D query code
 #define ENM_NDEF 64            // segments (and counters) of the deferred 1-D query list
+#define ENM_CSTRIDE 32         // unsigned slots between two atomic counters (one per 128 B line)
+#define ENM_KCHUNK 8           // kNN / multi-D count items taken per ticket
 #define ENM_PSI_TILE 512       // positions per psi partial sum of a 1-D space
 #define ENM_SORT_TILE 2048     // elements per block-sort tile and merge chunk
 #define ENM_SORT_THREADS 256

@@ -6,34 +6,33 @@
 // all j of the query's scope range, i itself included (distance 0); fewer
 // than k+1 points leave it +inf, as cKDTree does.
 //
-// Shape.  A work item is one wave and a tile of 64 consecutive positions of
-// the key-sorted order (one query per lane, coordinates in VGPRs).  The wave
-// walks candidates in wave-uniform order, so candidate coordinates are
-// wave-uniform loads served by the scalar cache and used as SGPR operands;
-// no LDS is needed.  Each lane keeps an ascending FP64 list of its K1 = k+1
-// smallest distances in registers (P slots, the lowest P-K1 pinned at -inf).
+// Grid orders (two or more columns): the key-sorted points cut into strips,
+// each strip sorted by the second column.  A work item is one wave and 64
+// consecutive positions of one strip (one query per lane); each lane walks
+// the strips around its own point by itself (knn_walk): its own strip from
+// its own position, then the strips outward that its current radius still
+// reaches on the key axis, each from the lane's y position in both
+// directions until the y distance reaches the radius.  Candidates arrive in
+// about increasing distance, so each lane evaluates a few dozen candidates.
+// Each lane keeps an ascending FP64 list of its K1 = k+1 smallest distances
+// in registers (P slots, the lowest P-K1 pinned at -inf).
 //
-// Pruning.  With two or more columns the order is a grid: the key-sorted
-// points cut into strips, each strip sorted by the second column, tiles of 64
-// y-consecutive points of one strip (compact in both axes).  A tile visits
-// its own strip and then neighbouring strips outward, each only within the
-// y window of its queries widened by the wave's largest current radius, and
-// stops on a side at the first strip that every lane's radius excludes on the
-// key axis (grid_sweep).  With one column, candidates are visited from the
-// tile outward in key order, alternating left and right in steps of 64; a
-// side stops once every lane's key distance to the last visited candidate is
-// >= its current K1-th distance T.  Either way every candidate that could
-// still enter a list is visited once.  With ENM_F_BRUTE the whole scope range
-// is scanned instead.
+// Other orders (one column, or ENM_F_BRUTE): a wave takes a tile of 64
+// consecutive positions and walks candidates in wave-uniform order, 64 at a
+// time staged in LDS (FP64) or 8 at a time through the scalar cache (FP32
+// filter); with one column, from the tile outward in key order, alternating
+// left and right in steps of 64, a side stopping once every lane's key
+// distance to the last visited candidate is >= its current K1-th distance T.
+// With ENM_F_BRUTE the whole scope range is scanned.
 //
-// Precision.  In FP64 mode every distance is exact FP64.  In FP32-filter mode
-// the bulk distance is computed from FP32 copies, and a chunk of 8 candidates
-// is settled in FP64 only when some lane's FP32 distance is <= fl32_up(T + g),
-// with the per-instance guard g bounding |d32 - d64|.  T only decreases, so
-// one pass is exact: radii are bit-identical to the FP64 mode.  While most
-// chunks improve some lane's list (early in a sweep, and in the compact grid
-// windows) the wave skips the filter and evaluates chunks in FP64 directly.
-// List updates for up to 8 slots are branch-free: the 8 exact distances are
+// Precision.  Every distance that enters a list is exact FP64.  The FP32
+// filter (brute force in FP32-filter mode) computes the bulk distance from
+// FP32 copies, and a chunk of 8 candidates is settled in FP64 only when some
+// lane's FP32 distance is <= fl32_up(T + g), with the per-instance guard g
+// bounding |d32 - d64|.  T only decreases, so one pass is exact: radii are
+// bit-identical to the FP64 mode.  While most chunks improve some lane's list
+// the wave skips the filter and evaluates chunks in FP64 directly.  List
+// updates for up to 8 slots are branch-free: the 8 exact distances are
 // sorted by a 19-comparator network and merged into the list.
 //
 // Work distribution.  A persistent grid pops items from ENM_NQ queues, one
@@ -248,35 +247,105 @@ struct KnnTile {
     }
 };
 
-// Grid sweep of one tile (already seeded with its own 64 points): the rest
-// of the tile's strip near the tile in y order, then whole strips outward on
-// both sides.  A strip is visited only within the y window of the lanes'
-// current radii; a side stops at the first strip whose x bound is at least
-// every lane's radius away (all strips beyond it are further in x).  Each
-// candidate is visited at most once, and every candidate that could still
-// enter a lane's list is visited.  Returns the number of candidates visited.
+// First position in [ks, ke) whose key is >= y, one search per lane (the
+// trip count depends only on ke - ks, up to one step).
+__device__ __forceinline__ int lane_lower_bound(const double* __restrict__ v, int ks, int ke, double y) {
+    int first = ks, len = ke - ks;
+    while (len > 0) {
+        const int half = len >> 1;
+        const bool go = v[first + half] < y;
+        first = go ? first + half + 1 : first;
+        len = go ? len - half - 1 : half;
+    }
+    return first;
+}
+
+// Walk of one strip [ks, ke) (sorted by column 1) for every live lane, from
+// its own start: upward from u and downward from dn, one candidate per
+// direction and iteration, so candidates come in about increasing y
+// distance.  A direction stops at the first candidate whose y distance is
+// >= the lane's current radius T (the strip is sorted, so every later one
+// is further still, and T only decreases); the others are evaluated and
+// offered to the list (an insertion at or above T leaves it unchanged).
+// The next candidate's coordinates are loaded before the current one is
+// evaluated.  Adds the lane's evaluated candidates to `visited`.
 template <int D, int P, bool F32>
-__device__ int grid_sweep(KnnTile<D, P, F32>& w, const KnnDev& kd, const double* __restrict__ col64,
-                          int lo, int hi, int t0, int t1, bool active) {
-    constexpr int SEED = 64;
+__device__ __forceinline__ void walk_strip(KnnTile<D, P, F32>& w, int ks, int ke, int u, int dn, bool live,
+                                           unsigned& visited) {
+    const double* __restrict__ yc = w.c64[1];
+    const double qy = w.q64[1];
+    bool up = live && u < ke;
+    bool down = live && dn >= ks;
+    double cu[D], cd[D];
+    {
+        const int ju = up ? u : ks, jd = down ? dn : ks;
+#pragma unroll
+        for (int d = 0; d < D; ++d) { cu[d] = w.c64[d][ju]; cd[d] = w.c64[d][jd]; }
+    }
+    while (wave_any(up || down)) {
+        double nu[D], nd[D];
+        {
+            const int ju = up && u + 1 < ke ? u + 1 : ks;
+            const int jd = down && dn - 1 >= ks ? dn - 1 : ks;
+#pragma unroll
+            for (int d = 0; d < D; ++d) { nu[d] = w.c64[d][ju]; nd[d] = w.c64[d][jd]; }
+        }
+        {
+            const double dy = cu[1] - qy;
+            up = up && dy < w.T;
+            double m = dy;
+#pragma unroll
+            for (int d = 0; d < D; ++d)
+                if (d != 1) m = fmax(m, fabs(cu[d] - w.q64[d]));
+            list_insert<P>(w.L, up ? m : INFINITY);
+            w.T = w.L[P - 1];
+            visited += up;
+        }
+        {
+            const double dy = qy - cd[1];
+            down = down && dy < w.T;
+            double m = dy;
+#pragma unroll
+            for (int d = 0; d < D; ++d)
+                if (d != 1) m = fmax(m, fabs(cd[d] - w.q64[d]));
+            list_insert<P>(w.L, down ? m : INFINITY);
+            w.T = w.L[P - 1];
+            visited += down;
+        }
+        ++u;
+        --dn;
+        up = up && u < ke;
+        down = down && dn >= ks;
+#pragma unroll
+        for (int d = 0; d < D; ++d) { cu[d] = nu[d]; cd[d] = nd[d]; }
+    }
+}
+
+// kNN on a grid order by per-lane walks: each lane walks its own strip
+// from its own position, then the strips outward on each side that its
+// radius still reaches on the key axis (x gap to the strip < T), entering
+// each at its own y position (per-lane binary search).  A side stops for
+// the wave once no lane reaches the next strip (strips further out are
+// further in x).  Every candidate that could enter a lane's list is
+// evaluated, in about increasing distance, so lists settle early; a lane
+// evaluates a few dozen candidates instead of the whole tile's window.
+// Returns the lane's number of evaluated candidates (the query itself
+// included).
+template <int D, int P, bool F32>
+__device__ unsigned knn_walk(KnnTile<D, P, F32>& w, const KnnDev& kd, const double* __restrict__ col64,
+                             int lo, int hi, int t0, int i, bool active) {
     const int S = kd.strip;
     const double* xk = col64 + kd.base_key;  // base order: column 0 sorted over [lo, hi)
-    const double* yc = w.c64[1];             // column 1, sorted inside each strip
     const int s = (t0 - lo) / S;
     const int nstrip = (hi - lo + S - 1) / S;
-    const int ss = lo + s * S, se = min(hi, ss + S);
-    const double yq = w.q64[1];
-    int visited = 0;
-    // Seed: up to SEED neighbours in y on each side of the tile.
-    const int a0 = max(ss, t0 - SEED), b0 = min(se, t1 + SEED);
-    w.range(a0, t0);
-    w.range(t1, b0);
-    visited += (t0 - a0) + (b0 - t1);
+    unsigned visited = active ? 1u : 0u;
+    if (active) {
+        list_insert<P>(w.L, 0.0);  // the query itself
+        w.T = w.L[P - 1];
+    }
     {
-        int a, b;
-        strip_window(yc, ss, se, yq, w.T, active, a, b);
-        if (a < a0) { w.range(a, a0); visited += a0 - a; }
-        if (b > b0) { w.range(b0, b); visited += b - b0; }
+        const int ss = lo + s * S, se = min(hi, ss + S);
+        walk_strip(w, ss, se, i + 1, i - 1, active, visited);
     }
     bool ldone = s == 0, rdone = s == nstrip - 1;
     for (int d = 1; !(ldone && rdone); ++d) {
@@ -285,16 +354,12 @@ __device__ int grid_sweep(KnnTile<D, P, F32>& w, const KnnDev& kd, const double*
             if (sl < 0) {
                 ldone = true;
             } else {
-                const int le = lo + (sl + 1) * S;  // strip sl is [le - S, le)
-                const double xb = xk[le - 1];      // largest x of strip sl
-                // Lanes whose radius reaches this strip on the key axis.
-                const bool need = w.q64[0] - xb < w.T;
+                const int ks = lo + sl * S, ke = ks + S;  // a full strip (left of s)
+                const bool need = active && w.q64[0] - xk[ke - 1] < w.T;
                 ldone = !wave_any(need);
                 if (!ldone) {
-                    int a, b;
-                    strip_window(yc, le - S, le, yq, w.T, need, a, b);
-                    w.range(a, b);
-                    visited += b - a;
+                    const int p = lane_lower_bound(w.c64[1], ks, ke, w.q64[1]);
+                    walk_strip(w, ks, ke, p, p - 1, need, visited);
                 }
             }
         }
@@ -303,15 +368,12 @@ __device__ int grid_sweep(KnnTile<D, P, F32>& w, const KnnDev& kd, const double*
             if (sr >= nstrip) {
                 rdone = true;
             } else {
-                const int rs = lo + sr * S;
-                const double xb = xk[rs];          // smallest x of strip sr
-                const bool need = xb - w.q64[0] < w.T;
+                const int ks = lo + sr * S, ke = min(hi, ks + S);
+                const bool need = active && xk[ks] - w.q64[0] < w.T;
                 rdone = !wave_any(need);
                 if (!rdone) {
-                    int a, b;
-                    strip_window(yc, rs, min(hi, rs + S), yq, w.T, need, a, b);
-                    w.range(a, b);
-                    visited += b - a;
+                    const int p = lane_lower_bound(w.c64[1], ks, ke, w.q64[1]);
+                    walk_strip(w, ks, ke, p, p - 1, need, visited);
                 }
             }
         }
@@ -356,15 +418,7 @@ __device__ void knn_item(const KnnArgs& A, const Item& it, const double* __restr
     const int lo = it.lo, hi = it.hi, t0 = it.t0, t1 = it.t1;
     int visited = t1 - t0;
     w.range(t0, t1);
-    bool swept = false;
-    if constexpr (D >= 2) {
-        if (kd.strip > 0 && !A.brute) {
-            visited += grid_sweep(w, kd, col64, lo, hi, t0, t1, active);
-            swept = true;
-        }
-    }
-    if (swept) {
-    } else if (A.brute) {
+    if (A.brute) {
         w.range(lo, t0);
         w.range(t1, hi);
         visited = hi - lo;
@@ -399,6 +453,50 @@ __device__ void knn_item(const KnnArgs& A, const Item& it, const double* __restr
     if (F32) rechecks += wave_sum_u64(w.rechecks);
 }
 
+// One grid-order item by per-lane walks (knn_walk).
+template <int D, int P>
+__device__ void walk_item(const KnnArgs& A, const Item& it, const double* __restrict__ col64,
+                          double* __restrict__ res64, unsigned long long& pairs) {
+    const KnnDev kd = A.descs[it.desc];
+    const OrderDev od = A.orders[kd.order];
+    const int64_t n = od.n;
+    const int i = it.t0 + (int)threadIdx.x;
+    const bool active = i < it.t1;
+    const int ncol = od.ncol;
+    KnnTile<D, P, false> w;
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+        const int c = d < ncol ? d : 0;
+        w.c64[d] = col64 + od.c64 + c * n;
+        w.q64[d] = active ? w.c64[d][i] : 0.0;
+    }
+    const int K1 = kd.k + 1;
+#pragma unroll
+    for (int t = 0; t < P; ++t) w.L[t] = (t < P - K1) ? -INFINITY : INFINITY;
+    w.T = active ? INFINITY : -INFINITY;
+    const unsigned v = knn_walk(w, kd, col64, it.lo, it.hi, it.t0, i, active);
+    const double eps = w.L[P - 1];
+    if (active) {
+        res64[kd.eps + i] = eps;
+        if (kd.eps_orig >= 0) res64[kd.eps_orig + A.perm[od.perm + i]] = eps;
+    }
+    pairs += wave_sum_u64(v);
+}
+
+template <int D, int P>
+__global__ __launch_bounds__(ENM_WAVE)
+void knn_walk_kernel(KnnArgs A, const double* __restrict__ col64, double* __restrict__ res64,
+                     unsigned long long* __restrict__ stats) {
+    unsigned long long pairs = 0;
+    for_each_wave_item(A.qoff, A.counter, ENM_KCHUNK, [&](int idx) {
+        Item it = A.items[idx];
+        it.desc = uni(it.desc); it.tile = uni(it.tile);
+        it.lo = uni(it.lo); it.hi = uni(it.hi); it.t0 = uni(it.t0); it.t1 = uni(it.t1);
+        walk_item<D, P>(A, it, col64, res64, pairs);
+    });
+    if (threadIdx.x == 0) atomicAdd(stats + ST_KNN_PAIRS, pairs);
+}
+
 template <int D, int P, bool F32>
 __global__ __launch_bounds__(ENM_WAVE)
 void knn_kernel(KnnArgs A, const double* __restrict__ col64, const float* __restrict__ col32,
@@ -408,7 +506,7 @@ void knn_kernel(KnnArgs A, const double* __restrict__ col64, const float* __rest
     // a million atomics on a single L2 channel.
     unsigned long long pairs = 0, rechecks = 0;
     __shared__ __align__(16) double stage[D * ENM_WAVE];
-    for_each_wave_item(A.qoff, A.counter, [&](int idx) {
+    for_each_wave_item(A.qoff, A.counter, ENM_KCHUNK, [&](int idx) {
         Item it = A.items[idx];
         it.desc = uni(it.desc); it.tile = uni(it.tile);
         it.lo = uni(it.lo); it.hi = uni(it.hi); it.t0 = uni(it.t0); it.t1 = uni(it.t1);
@@ -429,8 +527,14 @@ void knn_kernel(KnnArgs A, const double* __restrict__ col64, const float* __rest
 }
 
 template <int D, int P>
-hipError_t launch_dp(const KnnArgs& a, bool f32, int grid, const double* col64, const float* col32,
+hipError_t launch_dp(const KnnArgs& a, bool f32, bool walk, int grid, const double* col64, const float* col32,
                      double* res64, unsigned long long* stats, hipStream_t st) {
+    if constexpr (D >= 2) {
+        if (walk) {
+            hipLaunchKernelGGL((knn_walk_kernel<D, P>), dim3(grid), dim3(ENM_WAVE), 0, st, a, col64, res64, stats);
+            return hipGetLastError();
+        }
+    }
     if (f32)
         hipLaunchKernelGGL((knn_kernel<D, P, true>), dim3(grid), dim3(ENM_WAVE), 0, st,
                            a, col64, col32, res64, stats);
@@ -441,13 +545,13 @@ hipError_t launch_dp(const KnnArgs& a, bool f32, int grid, const double* col64, 
 }
 
 template <int D>
-hipError_t launch_d(const KnnArgs& a, int P, bool f32, int grid, const double* col64,
+hipError_t launch_d(const KnnArgs& a, int P, bool f32, bool walk, int grid, const double* col64,
                     const float* col32, double* res64, unsigned long long* stats, hipStream_t st) {
     switch (P) {
-        case 4: return launch_dp<D, 4>(a, f32, grid, col64, col32, res64, stats, st);
-        case 8: return launch_dp<D, 8>(a, f32, grid, col64, col32, res64, stats, st);
-        case 16: return launch_dp<D, 16>(a, f32, grid, col64, col32, res64, stats, st);
-        case 32: return launch_dp<D, 32>(a, f32, grid, col64, col32, res64, stats, st);
+        case 4: return launch_dp<D, 4>(a, f32, walk, grid, col64, col32, res64, stats, st);
+        case 8: return launch_dp<D, 8>(a, f32, walk, grid, col64, col32, res64, stats, st);
+        case 16: return launch_dp<D, 16>(a, f32, walk, grid, col64, col32, res64, stats, st);
+        case 32: return launch_dp<D, 32>(a, f32, walk, grid, col64, col32, res64, stats, st);
         default: return hipErrorInvalidValue;
     }
 }
@@ -475,19 +579,19 @@ hipError_t enm_launch_knn(const Item* items, const int* qoff, int n_items, unsig
                           const unsigned long long* maxabs, const int* perm,
                           const double* col64, const float* col32, double* res64,
                           unsigned long long* stats, int dclass, int pclass, bool f32, bool brute,
-                          int grid, hipStream_t st) {
+                          bool walk, int grid, hipStream_t st) {
     if (n_items <= 0) return hipSuccess;
     KnnArgs a{items, qoff, counter, descs, orders, maxabs, perm, brute ? 1 : 0};
     switch (dclass) {
-        case 1: return launch_d<1>(a, pclass, f32, grid, col64, col32, res64, stats, st);
-        case 2: return launch_d<2>(a, pclass, f32, grid, col64, col32, res64, stats, st);
-        case 3: return launch_d<3>(a, pclass, f32, grid, col64, col32, res64, stats, st);
-        case 4: return launch_d<4>(a, pclass, f32, grid, col64, col32, res64, stats, st);
-        case 5: return launch_d<5>(a, pclass, f32, grid, col64, col32, res64, stats, st);
-        case 6: return launch_d<6>(a, pclass, f32, grid, col64, col32, res64, stats, st);
-        case 8: return launch_d<8>(a, pclass, f32, grid, col64, col32, res64, stats, st);
-        case 12: return launch_d<12>(a, pclass, f32, grid, col64, col32, res64, stats, st);
-        case 16: return launch_d<16>(a, pclass, f32, grid, col64, col32, res64, stats, st);
+        case 1: return launch_d<1>(a, pclass, f32, walk, grid, col64, col32, res64, stats, st);
+        case 2: return launch_d<2>(a, pclass, f32, walk, grid, col64, col32, res64, stats, st);
+        case 3: return launch_d<3>(a, pclass, f32, walk, grid, col64, col32, res64, stats, st);
+        case 4: return launch_d<4>(a, pclass, f32, walk, grid, col64, col32, res64, stats, st);
+        case 5: return launch_d<5>(a, pclass, f32, walk, grid, col64, col32, res64, stats, st);
+        case 6: return launch_d<6>(a, pclass, f32, walk, grid, col64, col32, res64, stats, st);
+        case 8: return launch_d<8>(a, pclass, f32, walk, grid, col64, col32, res64, stats, st);
+        case 12: return launch_d<12>(a, pclass, f32, walk, grid, col64, col32, res64, stats, st);
+        case 16: return launch_d<16>(a, pclass, f32, walk, grid, col64, col32, res64, stats, st);
         default: return hipErrorInvalidValue;
     }
 }

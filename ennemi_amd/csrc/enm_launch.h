@@ -25,7 +25,7 @@ hipError_t enm_launch_knn(const Item* items, const int* qoff, int n_items, unsig
                           const unsigned long long* maxabs, const int* perm,
                           const double* col64, const float* col32, double* res64,
                           unsigned long long* stats, int dclass, int pclass, bool f32, bool brute,
-                          int grid, hipStream_t st);
+                          bool walk, int grid, hipStream_t st);
 
 int enm_count_dim_class(int d);
 hipError_t enm_launch_count(const Item* items, const int* qoff, int n_items, unsigned* counter,
