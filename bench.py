@@ -194,31 +194,44 @@ def main() -> int:
     log(f"headline: {wall:.2f} ms/step wall, {event_ms:.2f} ms events; stages {stage_mean}")
     log(f"stats {st}")
 
-    def roof(stage_ms, st):
-        """Roofline of the dominant stage of one step."""
-        dom = max(("knn", "count", "sort", "gather"), key=lambda s: stage_ms[s])
-        if dom == "knn":
-            ops = st["knn_pairs"] * 2 * 2  # 2D ops per pair, D = 2 (x, y)
-            ach = ops / (stage_ms["knn"] * 1e-3) / 1e12
-            return {"bound": "valu", "kernel": "knn_kernel<D=2,P=4,FP32 filter>",
-                    "achieved": ach, "peak": PEAK_FP32_TOPS, "unit": "T ops/s",
-                    "frac": ach / PEAK_FP32_TOPS, "traffic": None,
-                    "algorithmic": f"{st['knn_pairs']} pair evaluations x 4 ops (2 sub, 1 max, 1 compare)"}
-        if dom == "count":
-            byts = st["count_queries_1d"] * 20
-            ach = byts / (stage_ms["count"] * 1e-3) / 1e9
-            return {"bound": "hbm", "kernel": "count1d_kernel", "achieved": ach,
-                    "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": ach / PEAK_HBM_GBS, "traffic": None,
-                    "algorithmic": f"{st['count_queries_1d']} queries x 20 B"}
-        byts = st["sort_keys"] * 24 * max(1, int(np.ceil(np.log2(max(ns.max(), 2048) / 2048))) + 1)
-        ach = byts / (stage_ms["sort"] * 1e-3) / 1e9
-        return {"bound": "hbm", "kernel": "sort", "achieved": ach, "peak": PEAK_HBM_GBS,
-                "unit": "GB/s", "frac": ach / PEAK_HBM_GBS, "traffic": None,
-                "algorithmic": f"{st['sort_keys']} keys x 24 B per pass"}
-
-    roofline = roof(stage_mean, st)
+    # Roofline of the dominant kernel: the kNN walk (one launch per step for
+    # this workload, timed by the events around the kNN stage).  It is
+    # HBM-roofed by its algorithmic traffic: each query point's two
+    # coordinates read once (16 B), its radius written in two orders (16 B)
+    # and the permutation read (4 B) -- 36 B per point; candidate reads are
+    # cache hits by design.  `traffic` is the measured HBM bytes per launch
+    # (FETCH_SIZE + WRITE_SIZE, copy-calibrated) from the committed profile.
+    n_points = int(ns.sum())
+    knn_bytes = 36 * n_points
+    knn_ach = knn_bytes / (stage_mean["knn"] * 1e-3) / 1e9
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "lag_scan_776_ksg_hbm_traffic.json")
+    if os.path.exists(tpath):
+        with open(tpath) as f:
+            traffic = json.load(f).get("knn_walk_kernel<2, 4>", {}).get("hbm_bytes_per_launch")
+    roofline = {"bound": "hbm", "kernel": "knn_walk_kernel<D=2,P=4>", "achieved": knn_ach,
+                "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": knn_ach / PEAK_HBM_GBS, "traffic": traffic,
+                "algorithmic": f"{n_points} query points x 36 B (2 coordinates read, radius written "
+                               f"in kNN and point order, permutation read)",
+                "launch_ms": round(stage_mean["knn"], 4),
+                "pair_evaluations": st["knn_pairs"],
+                "pair_rate_T_per_s": st["knn_pairs"] / (stage_mean["knn"] * 1e-3) / 1e12}
+    # Every stage against its own roof (algorithmic bytes / stage time).
+    # sort: two orders per instance (x, y), 24 B per key and pass (key and
+    # index read and written), a block pass plus the merge passes; gather:
+    # the two orders' columns laid out (64 B/point), the strip sort (24 B)
+    # and the grid order's copy (44 B); count: 20 B per 1-D query.
+    sort_passes = 1 + max(0, int(np.ceil(np.log2(max(int(ns.max()), 2048) / 2048))))
+    stage_bytes = {
+        "sort": 2.0 * n_points * 24 * sort_passes,
+        "gather": n_points * (64.0 + 24 + 44),
+        "knn": float(knn_bytes),
+        "count": st["count_queries_1d"] * 20.0,
+    }
+    roofline["stages"] = {k: {"ms": round(stage_mean[k], 4),
+                              "GB_per_s": stage_bytes[k] / (stage_mean[k] * 1e-3) / 1e9}
+                          for k in stage_bytes}
     roofline["stage_ms"] = {k: round(v, 4) for k, v in stage_mean.items()}
-    roofline["knn_recheck_rate"] = st["knn_rechecks"] / max(1, st["knn_pairs"])
 
     comparators = {}
     if not args.no_compare:
@@ -280,13 +293,15 @@ def main() -> int:
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32+f64",
+        "dtype": "f64",
         "data": "synthetic",
         "config": {"workload": "masked lag scan: 8 coupled AR(1) series x 100,096, "
                                "lags -48..48, k=3, drop_nan (776 KSG estimates, N~1e5)",
                    "n_estimates": n_est, "n_min": int(ns.min()), "n_max": int(ns.max()),
-                   "mode": "batched persistent work queue, FP32 filter + FP64 recheck, "
-                           "grid-window pruning",
+                   "mode": "one batched launch sequence: segmented sorts, strip-grid orders, "
+                           "per-lane FP64 strip walks for the kNN, LDS binary-search 1-D counts "
+                           "(FP32 filter + FP64 recheck only where it pays: brute force, off-grid "
+                           "multi-D counts)",
                    "tuning": {"strip_points": tuning[0], "waves_per_cu": tuning[1]}},
         "roofline": roofline,
         "cpu_baseline": cpu,
