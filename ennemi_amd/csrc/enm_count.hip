@@ -45,6 +45,7 @@ struct CountArgs {
     const Item* items;
     const int* qoff;    // ENM_NQ + 1 queue offsets into items
     unsigned* counter;  // ENM_NQ ticket counters
+    int chunk;          // items per ticket (multi-D counts)
     const CountDev* descs;
     const OrderDev* orders;
     const unsigned long long* maxabs;
@@ -490,7 +491,7 @@ void countnd_kernel(CountArgs A, const double* __restrict__ col64, const float* 
                     unsigned long long* __restrict__ stats) {
     // Counters accumulate per wave and go out once (see knn_kernel).
     unsigned long long pairs = 0, rechecks = 0;
-    for_each_wave_item(A.qoff, A.counter, ENM_KCHUNK, [&](int idx) {
+    for_each_wave_item(A.qoff, A.counter, A.chunk, [&](int idx) {
         Item it = A.items[idx];
         it.desc = uni(it.desc); it.tile = uni(it.tile);
         it.lo = uni(it.lo); it.hi = uni(it.hi); it.t0 = uni(it.t0); it.t1 = uni(it.t1);
@@ -540,7 +541,9 @@ hipError_t enm_launch_count(const Item* items, const int* qoff, int n_items, uns
                             int grid, unsigned* deferred, unsigned* n_deferred, unsigned seg_cap,
                             hipStream_t st) {
     if (n_items <= 0) return hipSuccess;
-    CountArgs a{items, qoff, counter, descs, orders, maxabs, perm, psi_tab, brute ? 1 : 0};
+    // Tickets of up to ENM_KCHUNK items, at least four per wave of the grid.
+    const int chunk = std::max(1, std::min(ENM_KCHUNK, n_items / (4 * std::max(1, grid))));
+    CountArgs a{items, qoff, counter, chunk, descs, orders, maxabs, perm, psi_tab, brute ? 1 : 0};
     if (dclass == 1) {
         // `grid` counts waves; a 1-D workgroup has four.
         const int blocks = std::max(1, std::min(n_items, grid / (ENM_C1_THREADS / ENM_WAVE)));

@@ -18,9 +18,9 @@
 // in registers (P slots, the lowest P-K1 pinned at -inf).
 //
 // Other orders (one column, or ENM_F_BRUTE): a wave takes a tile of 64
-// consecutive positions and walks candidates in wave-uniform order, 64 at a
-// time staged in LDS (FP64) or 8 at a time through the scalar cache (FP32
-// filter); with one column, from the tile outward in key order, alternating
+// consecutive positions and walks candidates in wave-uniform order, 8 at a
+// time through the scalar cache (candidate coordinates are SGPR operands);
+// with one column, from the tile outward in key order, alternating
 // left and right in steps of 64, a side stopping once every lane's key
 // distance to the last visited candidate is >= its current K1-th distance T.
 // With ENM_F_BRUTE the whole scope range is scanned.
@@ -40,6 +40,8 @@
 #include "enm_device.h"
 #include "enm_launch.h"
 
+#include <algorithm>
+
 namespace {
 
 constexpr int CH = 8;     // candidates per filter chunk
@@ -49,6 +51,7 @@ struct KnnArgs {
     const Item* items;
     const int* qoff;    // ENM_NQ + 1 queue offsets into items
     unsigned* counter;  // ENM_NQ ticket counters
+    int chunk;          // items per ticket
     const KnnDev* descs;
     const OrderDev* orders;
     const unsigned long long* maxabs;
@@ -60,7 +63,6 @@ template <int D, int P, bool F32>
 struct KnnTile {
     const double* c64[D];
     const float* c32[D];
-    double* stage;    // the wave's LDS block of 64 candidates, [D][64]
     double q64[D];
     float q32[D];
     double L[P];
@@ -111,19 +113,6 @@ struct KnnTile {
             for (int d = 1; d < D; ++d) m = fmax(m, fabs(cx[d][c] - q64[d]));
             dd[c] = m;
         }
-        return take(dd);
-    }
-
-    // FP64 chunk of the staged block: candidates c0 .. c0+7, read from LDS
-    // with one address per instruction (a broadcast).
-    __device__ __forceinline__ bool chunk_lds(int c0) {
-        double dd[CH];
-#pragma unroll
-        for (int c = 0; c < CH; ++c) dd[c] = fabs(stage[c0 + c] - q64[0]);
-#pragma unroll
-        for (int d = 1; d < D; ++d)
-#pragma unroll
-            for (int c = 0; c < CH; ++c) dd[c] = fmax(dd[c], fabs(stage[d * ENM_WAVE + c0 + c] - q64[d]));
         return take(dd);
     }
 
@@ -216,33 +205,28 @@ struct KnnTile {
     // a chunk of arithmetic instead of stalling the wave); both buffers fit
     // in SGPRs (2 x 2 x 8 doubles).
     __device__ __forceinline__ void range(int a, int b) {
-        if constexpr (!F32) {
-            // FP64: blocks of 64 candidates staged in LDS.  Lane l loads
-            // candidate base + l of every column (coalesced; +inf past b,
-            // which no list takes), the block goes to LDS, the next block's
-            // loads are issued, and the wave evaluates the staged block.
-            if (a >= b) return;
-            const int lane = (int)threadIdx.x;
-            double nv[D];
-#pragma unroll
-            for (int d = 0; d < D; ++d) nv[d] = a + lane < b ? c64[d][a + lane] : INFINITY;
-            for (int base = a; base < b; base += ENM_WAVE) {
-                __syncthreads();  // the previous block has been read
-#pragma unroll
-                for (int d = 0; d < D; ++d) stage[d * ENM_WAVE + lane] = nv[d];
-                __syncthreads();
-                const int nb = base + ENM_WAVE;
-                if (nb < b) {
-#pragma unroll
-                    for (int d = 0; d < D; ++d) nv[d] = nb + lane < b ? c64[d][nb + lane] : INFINITY;
-                }
-                const int cnt = min(ENM_WAVE, b - base);
-                for (int c0 = 0; c0 < cnt; c0 += CH) dense = chunk_lds(c0);
-            }
-            return;
-        }
         int j = a;
-        for (; j + CH <= b; j += CH) chunk(j);
+        if constexpr (!F32 && D <= 2) {
+            // FP64 with up to two columns: the next chunk's coordinates are
+            // loaded before the current chunk is evaluated (software
+            // pipelining of the scalar loads; both buffers fit in SGPRs).
+            if (j + CH <= b) {
+                double cur[D][CH];
+                load64(cur, j);
+                for (; j + CH <= b; j += CH) {
+                    double nxt[D][CH];
+                    // The last chunk reloads itself (in bounds, unused).
+                    load64(nxt, j + 2 * CH <= b ? j + CH : j);
+                    dense = chunk64v(cur);
+#pragma unroll
+                    for (int d = 0; d < D; ++d)
+#pragma unroll
+                        for (int c = 0; c < CH; ++c) cur[d][c] = nxt[d][c];
+                }
+            }
+        } else {
+            for (; j + CH <= b; j += CH) chunk(j);
+        }
         for (; j < b; ++j) single(j);
     }
 };
@@ -383,7 +367,7 @@ __device__ unsigned knn_walk(KnnTile<D, P, F32>& w, const KnnDev& kd, const doub
 
 template <int D, int P, bool F32>
 __device__ void knn_item(const KnnArgs& A, const Item& it, const double* __restrict__ col64,
-                         const float* __restrict__ col32, double* __restrict__ res64, double* stage,
+                         const float* __restrict__ col32, double* __restrict__ res64,
                          unsigned long long& pairs, unsigned long long& rechecks) {
     const KnnDev kd = A.descs[it.desc];
     const OrderDev od = A.orders[kd.order];
@@ -394,7 +378,6 @@ __device__ void knn_item(const KnnArgs& A, const Item& it, const double* __restr
     const int ncol = od.ncol;
 
     KnnTile<D, P, F32> w;
-    w.stage = stage;
     bool f32ok = true;
     w.guard = enm_guard(A.maxabs[od.inst], &f32ok);
 #pragma unroll
@@ -488,7 +471,7 @@ __global__ __launch_bounds__(ENM_WAVE)
 void knn_walk_kernel(KnnArgs A, const double* __restrict__ col64, double* __restrict__ res64,
                      unsigned long long* __restrict__ stats) {
     unsigned long long pairs = 0;
-    for_each_wave_item(A.qoff, A.counter, ENM_KCHUNK, [&](int idx) {
+    for_each_wave_item(A.qoff, A.counter, A.chunk, [&](int idx) {
         Item it = A.items[idx];
         it.desc = uni(it.desc); it.tile = uni(it.tile);
         it.lo = uni(it.lo); it.hi = uni(it.hi); it.t0 = uni(it.t0); it.t1 = uni(it.t1);
@@ -505,8 +488,7 @@ void knn_kernel(KnnArgs A, const double* __restrict__ col64, const float* __rest
     // out once at the end: one atomic per item on one address would queue
     // a million atomics on a single L2 channel.
     unsigned long long pairs = 0, rechecks = 0;
-    __shared__ __align__(16) double stage[D * ENM_WAVE];
-    for_each_wave_item(A.qoff, A.counter, ENM_KCHUNK, [&](int idx) {
+    for_each_wave_item(A.qoff, A.counter, A.chunk, [&](int idx) {
         Item it = A.items[idx];
         it.desc = uni(it.desc); it.tile = uni(it.tile);
         it.lo = uni(it.lo); it.hi = uni(it.hi); it.t0 = uni(it.t0); it.t1 = uni(it.t1);
@@ -514,10 +496,10 @@ void knn_kernel(KnnArgs A, const double* __restrict__ col64, const float* __rest
             const KnnDev& kd = A.descs[it.desc];
             bool ok = true;
             (void)enm_guard(A.maxabs[A.orders[kd.order].inst], &ok);
-            if (ok) knn_item<D, P, true>(A, it, col64, col32, res64, stage, pairs, rechecks);
-            else    knn_item<D, P, false>(A, it, col64, col32, res64, stage, pairs, rechecks);
+            if (ok) knn_item<D, P, true>(A, it, col64, col32, res64, pairs, rechecks);
+            else    knn_item<D, P, false>(A, it, col64, col32, res64, pairs, rechecks);
         } else {
-            knn_item<D, P, false>(A, it, col64, col32, res64, stage, pairs, rechecks);
+            knn_item<D, P, false>(A, it, col64, col32, res64, pairs, rechecks);
         }
     });
     if (threadIdx.x == 0) {
@@ -581,7 +563,10 @@ hipError_t enm_launch_knn(const Item* items, const int* qoff, int n_items, unsig
                           unsigned long long* stats, int dclass, int pclass, bool f32, bool brute,
                           bool walk, int grid, hipStream_t st) {
     if (n_items <= 0) return hipSuccess;
-    KnnArgs a{items, qoff, counter, descs, orders, maxabs, perm, brute ? 1 : 0};
+    // Tickets of up to ENM_KCHUNK items, but at least four per wave of the
+    // grid, so small launches (one instance) still fill the machine.
+    const int chunk = std::max(1, std::min(ENM_KCHUNK, n_items / (4 * std::max(1, grid))));
+    KnnArgs a{items, qoff, counter, chunk, descs, orders, maxabs, perm, brute ? 1 : 0};
     switch (dclass) {
         case 1: return launch_d<1>(a, pclass, f32, walk, grid, col64, col32, res64, stats, st);
         case 2: return launch_d<2>(a, pclass, f32, walk, grid, col64, col32, res64, stats, st);
