@@ -801,21 +801,31 @@ int enqueue(enm_ctx* ctx, Plan& P, const double* arena, double* out, double* d_e
         return rc;
     if (!P.segs2.empty()) {
         // Second stage: sort each grid strip by the base order's column 1
-        // (keys read from col64), then gather the grid orders.
-        if ((rc = chk(enm_launch_sort(tab<SortSeg>(P, P.off_segs2), (int)P.segs2.size(), P.ntiles2, P.maxlen2,
+        // (keys read from col64) and lay out the grid orders; strips that
+        // fit one block do both in one kernel.
+        if (P.maxlen2 <= ENM_SORT_TILE) {
+            if ((rc = chk(enm_launch_strip_sort(tab<GatherItem>(P, P.off_gitems2), (int)P.gitems2.size(),
+                                                tab<SortSeg>(P, P.off_segs2), tab<OrderDev>(P, P.off_orders),
+                                                P.tile2, P.col64.as<double>(), P.col32.as<float>(),
+                                                P.perm.as<int>(), P.f32 && P.need32 ? 1 : 0, &nl, st),
+                          "sort")))
+                return rc;
+        } else if ((rc = chk(enm_launch_sort(tab<SortSeg>(P, P.off_segs2), (int)P.segs2.size(), P.ntiles2, P.maxlen2,
                                       P.tile2, P.col64.as<double>(), P.sk0.as<double>(), P.si0.as<int>(),
                                       P.sk1.as<double>(), P.si1.as<int>(), P.split.as<int>(), &fb, &nl, st),
-                      "sort")))
+                      "sort"))) {
             return rc;
-        const double* skey2 = fb ? P.sk1.as<double>() : P.sk0.as<double>();
-        const int* sidx2 = fb ? P.si1.as<int>() : P.si0.as<int>();
-        if ((rc = chk(enm_launch_gather(tab<GatherItem>(P, P.off_gitems2), (int)P.gitems2.size(),
-                                        tab<SortSeg>(P, P.off_segs2), tab<OrderDev>(P, P.off_orders), arena,
-                                        skey2, sidx2, P.col64.as<double>(), P.col32.as<float>(),
-                                        P.perm.as<int>(), P.maxabs.as<unsigned long long>(),
-                                        P.f32 && P.need32 ? 1 : 0, &nl, st),
-                      "gather")))
-            return rc;
+        } else {
+            const double* skey2 = fb ? P.sk1.as<double>() : P.sk0.as<double>();
+            const int* sidx2 = fb ? P.si1.as<int>() : P.si0.as<int>();
+            if ((rc = chk(enm_launch_gather(tab<GatherItem>(P, P.off_gitems2), (int)P.gitems2.size(),
+                                            tab<SortSeg>(P, P.off_segs2), tab<OrderDev>(P, P.off_orders), arena,
+                                            skey2, sidx2, P.col64.as<double>(), P.col32.as<float>(),
+                                            P.perm.as<int>(), P.maxabs.as<unsigned long long>(),
+                                            P.f32 && P.need32 ? 1 : 0, &nl, st),
+                          "gather")))
+                return rc;
+        }
     }
     if ((rc = mark(3))) return rc;
     const OrderDev* orders = tab<OrderDev>(P, P.off_orders);

@@ -12,6 +12,9 @@ hipError_t enm_launch_fill_i32(int* p, int64_t n, int v, hipStream_t st);
 hipError_t enm_launch_sort(const SortSeg* d_segs, int nsegs, int ntiles, int64_t maxlen, int tile,
                            const double* arena, double* k0, int* i0, double* k1, int* i1, int* split,
                            int* final_buf, int* n_launch, hipStream_t st);
+hipError_t enm_launch_strip_sort(const GatherItem* d_items, int nitems, const SortSeg* d_segs,
+                                 const OrderDev* d_orders, int tile, double* col64, float* col32, int* perm,
+                                 int with32, int* n_launch, hipStream_t st);
 hipError_t enm_launch_gather(const GatherItem* d_items, int nitems, const SortSeg* d_segs,
                              const OrderDev* d_orders, const double* arena,
                              const double* skey, const int* sidx,

@@ -74,33 +74,15 @@ __device__ int co_rank(int d, const double* ka, const int* ia, int na,
     return lo;
 }
 
-// Sort one TILE chunk in a 256-thread block: each thread sorts TILE/256
-// consecutive keys in registers, then merge levels of doubling run width in
-// LDS, each thread producing TILE/256 consecutive outputs of its pair of runs
-// from a merge-path split (outputs kept in registers across a barrier, so the
-// merge works in place).  Padding keys (+inf, INT_MAX) sort last.
+// Sort the TILE (key, index) pairs of sk/si in LDS with a 256-thread block:
+// each thread sorts TILE/256 consecutive pairs in registers, then merge
+// levels of doubling run width, each thread producing TILE/256 consecutive
+// outputs of its pair of runs from a merge-path split (outputs kept in
+// registers across a barrier, so the merge works in place).  Padding keys
+// (+inf, INT_MAX) sort last.  Starts and ends with a barrier.
 template <int TILE>
-__global__ __launch_bounds__(ENM_SORT_THREADS)
-void block_sort_kernel(const SortSeg* __restrict__ segs, int nsegs,
-                       const double* __restrict__ arena,
-                       double* __restrict__ kout, int* __restrict__ iout) {
+__device__ __forceinline__ void block_sort_lds(double* sk, int* si) {
     constexpr int PER = TILE / ENM_SORT_THREADS;
-    __shared__ double sk[TILE];
-    __shared__ int si[TILE];
-    const int tile = blockIdx.x;
-    const int s = seg_of_tile(segs, nsegs, tile);
-    const SortSeg sg = segs[s];
-    const int off = (tile - sg.first_tile) * TILE;
-    const int len = min(TILE, sg.len - off);
-    for (int t = threadIdx.x; t < TILE; t += ENM_SORT_THREADS) {
-        if (t < len) {
-            sk[t] = arena[sg.src + off + t];
-            si[t] = sg.p0 + off + t;
-        } else {
-            sk[t] = INFINITY;
-            si[t] = 0x7fffffff;
-        }
-    }
     __syncthreads();
     const int e0 = threadIdx.x * PER;
     double k[PER];
@@ -136,9 +118,76 @@ void block_sort_kernel(const SortSeg* __restrict__ segs, int nsegs,
         for (int e = 0; e < PER; ++e) { sk[e0 + e] = k[e]; si[e0 + e] = x[e]; }
         __syncthreads();
     }
+}
+
+// Sort one TILE chunk of a segment: keys arena[src + off + t] with their
+// point indices p0 + off + t, into the sort buffers.
+template <int TILE>
+__global__ __launch_bounds__(ENM_SORT_THREADS)
+void block_sort_kernel(const SortSeg* __restrict__ segs, int nsegs,
+                       const double* __restrict__ arena,
+                       double* __restrict__ kout, int* __restrict__ iout) {
+    __shared__ double sk[TILE];
+    __shared__ int si[TILE];
+    const int tile = blockIdx.x;
+    const int s = seg_of_tile(segs, nsegs, tile);
+    const SortSeg sg = segs[s];
+    const int off = (tile - sg.first_tile) * TILE;
+    const int len = min(TILE, sg.len - off);
+    for (int t = threadIdx.x; t < TILE; t += ENM_SORT_THREADS) {
+        if (t < len) {
+            sk[t] = arena[sg.src + off + t];
+            si[t] = sg.p0 + off + t;
+        } else {
+            sk[t] = INFINITY;
+            si[t] = 0x7fffffff;
+        }
+    }
+    block_sort_lds<TILE>(sk, si);
     for (int t = threadIdx.x; t < len; t += ENM_SORT_THREADS) {
         kout[sg.sbase + off + t] = sk[t];
         iout[sg.sbase + off + t] = si[t];
+    }
+}
+
+// Build a grid order strip by strip: one block per strip (at most TILE
+// points) sorts the strip's points by the base order's column 1 in LDS and
+// copies each point's columns and instance index from the base order into
+// the grid order directly (strip-local reads, coalesced writes), with no
+// pass through the sort buffers.  Item: the strip's sort segment (src =
+// column 1 of the base order at the strip, p0 = the strip's first position,
+// which is the same in both orders).
+template <int TILE>
+__global__ __launch_bounds__(ENM_SORT_THREADS)
+void strip_sort_kernel(const GatherItem* __restrict__ items, const SortSeg* __restrict__ segs,
+                       const OrderDev* __restrict__ orders, double* col64, float* col32, int* perm,
+                       int with32) {
+    __shared__ double sk[TILE];
+    __shared__ int si[TILE];
+    const GatherItem it = items[blockIdx.x];
+    const SortSeg sg = segs[it.sseg];
+    const OrderDev& od = orders[it.order];
+    const int len = sg.len;
+    for (int t = threadIdx.x; t < TILE; t += ENM_SORT_THREADS) {
+        if (t < len) {
+            sk[t] = col64[sg.src + t];
+            si[t] = sg.p0 + t;
+        } else {
+            sk[t] = INFINITY;
+            si[t] = 0x7fffffff;
+        }
+    }
+    block_sort_lds<TILE>(sk, si);
+    const int64_t n = od.n;
+    const int ncol = od.ncol;
+    for (int t = threadIdx.x; t < len; t += ENM_SORT_THREADS) {
+        const int idx = si[t];
+        const int64_t p = it.pos0 + t;
+        perm[od.perm + p] = perm[od.base_perm + idx];
+        for (int c = 0; c < ncol; ++c) {
+            col64[od.c64 + c * n + p] = col64[od.base_c64 + c * n + idx];
+            if (with32) col32[od.c32 + c * n + p] = col32[od.base_c32 + c * n + idx];
+        }
     }
 }
 
@@ -363,6 +412,25 @@ hipError_t enm_launch_sort(const SortSeg* d_segs, int nsegs, int ntiles, int64_t
     }
     *final_buf = cur;
     return hipSuccess;
+}
+
+hipError_t enm_launch_strip_sort(const GatherItem* d_items, int nitems, const SortSeg* d_segs,
+                                 const OrderDev* d_orders, int tile, double* col64, float* col32, int* perm,
+                                 int with32, int* n_launch, hipStream_t st) {
+    if (nitems <= 0) return hipSuccess;
+    if (tile == 512)
+        hipLaunchKernelGGL(strip_sort_kernel<512>, dim3(nitems), dim3(ENM_SORT_THREADS), 0, st,
+                           d_items, d_segs, d_orders, col64, col32, perm, with32);
+    else if (tile == 1024)
+        hipLaunchKernelGGL(strip_sort_kernel<1024>, dim3(nitems), dim3(ENM_SORT_THREADS), 0, st,
+                           d_items, d_segs, d_orders, col64, col32, perm, with32);
+    else if (tile == ENM_SORT_TILE)
+        hipLaunchKernelGGL(strip_sort_kernel<ENM_SORT_TILE>, dim3(nitems), dim3(ENM_SORT_THREADS), 0, st,
+                           d_items, d_segs, d_orders, col64, col32, perm, with32);
+    else
+        return hipErrorInvalidValue;
+    ++*n_launch;
+    return hipGetLastError();
 }
 
 hipError_t enm_launch_gather(const GatherItem* d_items, int nitems, const SortSeg* d_segs,
