@@ -487,7 +487,9 @@ int plan_batch(enm_ctx* ctx, Plan& P, const enm_instance* inst, int n_inst, cons
                 // Work per tile: the scope range in brute force, about
                 // constant with window pruning.
                 pi.cost = P.brute ? (int64_t)(rg.second - rg.first) : 1;
-                pi.inst = ii; pi.dclass = dcl; pi.pclass = pcl; pi.walk = grid ? 1 : 0;
+                // Per-lane walks on two-column grids; with more columns the
+                // gridded two prune less and tiles share candidates.
+                pi.inst = ii; pi.dclass = dcl; pi.pclass = pcl; pi.walk = grid && D == 2 ? 1 : 0;
                 mine.push_back(pi);
             }
         }

@@ -6,21 +6,25 @@
 // all j of the query's scope range, i itself included (distance 0); fewer
 // than k+1 points leave it +inf, as cKDTree does.
 //
-// Grid orders (two or more columns): the key-sorted points cut into strips,
-// each strip sorted by the second column.  A work item is one wave and 64
-// consecutive positions of one strip (one query per lane); each lane walks
-// the strips around its own point by itself (knn_walk): its own strip from
-// its own position, then the strips outward that its current radius still
+// Grid orders: the key-sorted points cut into strips, each strip sorted by
+// the second column.  A work item is one wave and 64 consecutive positions
+// of one strip (one query per lane).  With two columns each lane walks the
+// strips around its own point by itself (knn_walk): its own strip from its
+// own position, then the strips outward that its current radius still
 // reaches on the key axis, each from the lane's y position in both
 // directions until the y distance reaches the radius.  Candidates arrive in
-// about increasing distance, so each lane evaluates a few dozen candidates.
+// about increasing distance, so each lane evaluates about ten candidates.
 // Each lane keeps an ascending FP64 list of its K1 = k+1 smallest distances
 // in registers (P slots, the lowest P-K1 pinned at -inf).
 //
-// Other orders (one column, or ENM_F_BRUTE): a wave takes a tile of 64
-// consecutive positions and walks candidates in wave-uniform order, 8 at a
-// time through the scalar cache (candidate coordinates are SGPR operands);
-// with one column, from the tile outward in key order, alternating
+// Grid orders of three or more columns, where the two gridded columns prune
+// less and a lane's walk would be long, share candidates instead: a wave
+// takes a tile of 64 consecutive positions of a strip and walks candidates
+// in wave-uniform order (grid_sweep: its own strip, then the strips outward,
+// each within the union of the lanes' y intervals).  Other orders (one
+// column, or ENM_F_BRUTE) do the same over the key-sorted order, 8
+// candidates at a time through the scalar cache (candidate coordinates are
+// SGPR operands); with one column, from the tile outward in key order, alternating
 // left and right in steps of 64, a side stopping once every lane's key
 // distance to the last visited candidate is >= its current K1-th distance T.
 // With ENM_F_BRUTE the whole scope range is scanned.
@@ -365,6 +369,77 @@ __device__ unsigned knn_walk(KnnTile<D, P, F32>& w, const KnnDev& kd, const doub
     return visited;
 }
 
+// Grid sweep of one tile (already seeded with its own 64 points): the rest
+// of the tile's strip near the tile in y order, then whole strips outward on
+// both sides.  A strip is visited only within the y window of the lanes'
+// current radii; a side stops at the first strip whose x bound is at least
+// every lane's radius away (all strips beyond it are further in x).  Each
+// candidate is visited at most once, and every candidate that could still
+// enter a lane's list is visited.  Returns the number of candidates visited.
+template <int D, int P, bool F32>
+__device__ int grid_sweep(KnnTile<D, P, F32>& w, const KnnDev& kd, const double* __restrict__ col64,
+                          int lo, int hi, int t0, int t1, bool active) {
+    constexpr int SEED = 64;
+    const int S = kd.strip;
+    const double* xk = col64 + kd.base_key;  // base order: column 0 sorted over [lo, hi)
+    const double* yc = w.c64[1];             // column 1, sorted inside each strip
+    const int s = (t0 - lo) / S;
+    const int nstrip = (hi - lo + S - 1) / S;
+    const int ss = lo + s * S, se = min(hi, ss + S);
+    const double yq = w.q64[1];
+    int visited = 0;
+    // Seed: up to SEED neighbours in y on each side of the tile.
+    const int a0 = max(ss, t0 - SEED), b0 = min(se, t1 + SEED);
+    w.range(a0, t0);
+    w.range(t1, b0);
+    visited += (t0 - a0) + (b0 - t1);
+    {
+        int a, b;
+        strip_window(yc, ss, se, yq, w.T, active, a, b);
+        if (a < a0) { w.range(a, a0); visited += a0 - a; }
+        if (b > b0) { w.range(b0, b); visited += b - b0; }
+    }
+    bool ldone = s == 0, rdone = s == nstrip - 1;
+    for (int d = 1; !(ldone && rdone); ++d) {
+        if (!ldone) {
+            const int sl = s - d;
+            if (sl < 0) {
+                ldone = true;
+            } else {
+                const int le = lo + (sl + 1) * S;  // strip sl is [le - S, le)
+                const double xb = xk[le - 1];      // largest x of strip sl
+                // Lanes whose radius reaches this strip on the key axis.
+                const bool need = w.q64[0] - xb < w.T;
+                ldone = !wave_any(need);
+                if (!ldone) {
+                    int a, b;
+                    strip_window(yc, le - S, le, yq, w.T, need, a, b);
+                    w.range(a, b);
+                    visited += b - a;
+                }
+            }
+        }
+        if (!rdone) {
+            const int sr = s + d;
+            if (sr >= nstrip) {
+                rdone = true;
+            } else {
+                const int rs = lo + sr * S;
+                const double xb = xk[rs];          // smallest x of strip sr
+                const bool need = xb - w.q64[0] < w.T;
+                rdone = !wave_any(need);
+                if (!rdone) {
+                    int a, b;
+                    strip_window(yc, rs, min(hi, rs + S), yq, w.T, need, a, b);
+                    w.range(a, b);
+                    visited += b - a;
+                }
+            }
+        }
+    }
+    return visited;
+}
+
 template <int D, int P, bool F32>
 __device__ void knn_item(const KnnArgs& A, const Item& it, const double* __restrict__ col64,
                          const float* __restrict__ col32, double* __restrict__ res64,
@@ -401,7 +476,15 @@ __device__ void knn_item(const KnnArgs& A, const Item& it, const double* __restr
     const int lo = it.lo, hi = it.hi, t0 = it.t0, t1 = it.t1;
     int visited = t1 - t0;
     w.range(t0, t1);
-    if (A.brute) {
+    bool swept = false;
+    if constexpr (D >= 2) {
+        if (kd.strip > 0 && !A.brute) {
+            visited += grid_sweep(w, kd, col64, lo, hi, t0, t1, active);
+            swept = true;
+        }
+    }
+    if (swept) {
+    } else if (A.brute) {
         w.range(lo, t0);
         w.range(t1, hi);
         visited = hi - lo;
