@@ -129,7 +129,7 @@ def main() -> int:
     ap.add_argument("--length", type=int, default=100_096)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-compare", action="store_true")
-    ap.add_argument("--cpu-lags", type=int, default=12, help="lags in the CPU baseline sample")
+    ap.add_argument("--cpu-lags", type=int, default=24, help="lags in the CPU baseline sample")
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--fp64", action="store_true", help="headline batch in FP64 mode (no FP32 filter)")
     ap.add_argument("--tuning", default=None,
