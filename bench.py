@@ -217,13 +217,20 @@ def main() -> int:
                 "pair_evaluations": st["knn_pairs"],
                 "pair_rate_T_per_s": st["knn_pairs"] / (stage_mean["knn"] * 1e-3) / 1e12}
     # Every stage against its own roof (algorithmic bytes / stage time).
-    # sort: two orders per instance (x, y), 24 B per key and pass (key and
-    # index read and written), a block pass plus the merge passes; gather:
-    # the two orders' columns laid out (64 B/point), the strip sort (24 B)
-    # and the grid order's copy (44 B); count: 20 B per 1-D query.
-    sort_passes = 1 + max(0, int(np.ceil(np.log2(max(int(ns.max()), 2048) / 2048))))
+    # sort: two orders per instance (x, y); segments of 2049 .. 2^20 keys are
+    # sample-sorted at 56 B per key (bucket count: key read, bucket id
+    # written; scatter: key and bucket id read, key and index written; run
+    # sort: key and index read and written), others take a block pass plus
+    # merge passes at 24 B per key each; gather: the two orders' columns
+    # laid out (64 B/point), the strip sort (24 B) and the grid order's copy
+    # (44 B); count: 20 B per 1-D query.
+    longest = int(ns.max())
+    if 2048 < longest <= 1 << 20:
+        sort_bytes_per_key = 56.0
+    else:
+        sort_bytes_per_key = 24.0 * (1 + max(0, int(np.ceil(np.log2(max(longest, 2048) / 2048)))))
     stage_bytes = {
-        "sort": 2.0 * n_points * 24 * sort_passes,
+        "sort": 2.0 * n_points * sort_bytes_per_key,
         "gather": n_points * (64.0 + 24 + 44),
         "knn": float(knn_bytes),
         "count": st["count_queries_1d"] * 20.0,

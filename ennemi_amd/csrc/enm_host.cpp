@@ -101,6 +101,13 @@ struct Plan {
     std::vector<ExportInst> exports;
     int ntiles = 0, ntiles2 = 0;
     int tile2 = ENM_SORT_TILE;   // block-sort tile of the second stage
+    // Sample sort of the first stage (segments longer than one block-sort
+    // tile, none longer than ENM_SS_MAXLEN); otherwise merge passes.
+    bool sample_sort = false;
+    std::vector<SampleSeg> ssegs;    // per first-stage segment
+    std::vector<SortSeg> samp_segs;  // sample ranges to sort
+    int samp_tiles = 0;
+    int64_t samp_maxlen = 0, n_samp = 0, n_counts = 0, n_buckets = 0;
     int64_t maxlen = 0, maxlen2 = 0;
     int64_t sort_elems = 0, sort_elems2 = 0;
     int64_t n_col64 = 0, n_col32 = 0, n_perm = 0, n_res64 = 0, n_res32 = 0;
@@ -114,10 +121,10 @@ struct Plan {
     bool need32 = false;
     // device
     DevBuf tables, sk0, si0, sk1, si1, split, col64, col32, perm, res64, res32, maxabs, counters, stats,
-        deferred;
+        deferred, samp_k0, samp_k1, samp_i0, samp_i1, samp_split, bid, ss_counts, ss_tiles, ss_ctl;
     size_t off_segs = 0, off_gitems = 0, off_orders = 0, off_kdescs = 0, off_cdescs = 0, off_insts = 0,
            off_cls = 0, off_kitems = 0, off_citems = 0, off_segs2 = 0, off_gitems2 = 0, off_rdescs = 0,
-           off_ritems = 0, off_sitems = 0, off_qoffs = 0, table_bytes = 0;
+           off_ritems = 0, off_sitems = 0, off_qoffs = 0, off_ssegs = 0, off_samp_segs = 0, table_bytes = 0;
     void* pinned = nullptr;
     hipEvent_t upload_done = nullptr;
     hipGraphExec_t graph = nullptr;
@@ -667,6 +674,35 @@ int plan_batch(enm_ctx* ctx, Plan& P, const enm_instance* inst, int n_inst, cons
                 if (k < lists[q].size()) mixed.push_back(lists[q][k]);
         P.gitems.swap(mixed);
     }
+    // First stage: sample sort when segments need more than one block-sort
+    // tile and none is longer than ENM_SS_MAXLEN.
+    P.sample_sort = P.maxlen > ENM_SORT_TILE && P.maxlen <= ENM_SS_MAXLEN;
+    if (P.sample_sort) {
+        for (const SortSeg& sg : P.segs) {
+            SampleSeg q;
+            std::memset(&q, 0, sizeof(q));
+            q.nb = (int32_t)std::min<int64_t>(ENM_SS_MAXB, (sg.len + ENM_SS_BUCKET - 1) / ENM_SS_BUCKET);
+            q.ns = q.nb > 1 ? ENM_SS_OVER * q.nb : 0;   // < len: nb > 1 means len > ENM_SS_BUCKET
+            q.samp = (int32_t)P.n_samp;
+            q.cbase = P.n_counts;
+            if (q.ns > 0) {
+                SortSeg s2;
+                s2.src = q.samp;
+                s2.sbase = q.samp;
+                s2.len = q.ns;
+                s2.first_tile = P.samp_tiles;
+                s2.p0 = 0;
+                s2.pad = 0;
+                P.samp_tiles += (q.ns + ENM_SORT_TILE - 1) / ENM_SORT_TILE;
+                P.samp_maxlen = std::max<int64_t>(P.samp_maxlen, q.ns);
+                P.samp_segs.push_back(s2);
+            }
+            P.n_samp += q.ns;
+            P.n_counts += (int64_t)((sg.len + ENM_SORT_TILE - 1) / ENM_SORT_TILE) * q.nb;
+            P.n_buckets += q.nb;
+            P.ssegs.push_back(q);
+        }
+    }
     // psi(0..max n) for the count kernels.
     P.psi_tab = P.n_res64;
     P.n_res64 += P.psi_n + 1;
@@ -702,6 +738,8 @@ int allocate_and_upload(enm_ctx* ctx, Plan& P, hipStream_t st) {
     P.off_ritems = append(blob, P.ritems);
     P.off_sitems = append(blob, P.sitems);
     P.off_qoffs = append(blob, P.qoffs);
+    P.off_ssegs = append(blob, P.ssegs);
+    P.off_samp_segs = append(blob, P.samp_segs);
     P.table_bytes = blob.size();
     hipError_t e;
     if (P.upload_done) {
@@ -738,6 +776,17 @@ int allocate_and_upload(enm_ctx* ctx, Plan& P, hipStream_t st) {
     if ((rc = need(P.counters, P.groups.size() * (ENM_NQ + ENM_NDEF) * ENM_CSTRIDE * 4 + 4, "counters"))) return rc;
     if ((rc = need(P.deferred, (size_t)P.deferred_seg * ENM_NDEF * 4, "deferred queries"))) return rc;
     if ((rc = need(P.stats, ST_N * 8, "stats"))) return rc;
+    if (P.sample_sort) {
+        if ((rc = need(P.samp_k0, P.n_samp * 8 + 8, "samples"))) return rc;
+        if ((rc = need(P.samp_k1, P.n_samp * 8 + 8, "samples"))) return rc;
+        if ((rc = need(P.samp_i0, P.n_samp * 4 + 4, "samples"))) return rc;
+        if ((rc = need(P.samp_i1, P.n_samp * 4 + 4, "samples"))) return rc;
+        if ((rc = need(P.samp_split, (size_t)P.samp_tiles * 4 + 4, "sample splits"))) return rc;
+        if ((rc = need(P.bid, P.sort_elems * 2 + 2, "buckets"))) return rc;
+        if ((rc = need(P.ss_counts, P.n_counts * 4 + 4, "bucket counts"))) return rc;
+        if ((rc = need(P.ss_tiles, P.n_buckets * sizeof(SsTile), "bucket runs"))) return rc;
+        if ((rc = need(P.ss_ctl, 256, "run counter"))) return rc;
+    }
     e = hipMemcpyAsync(P.tables.p, P.pinned, P.table_bytes, hipMemcpyHostToDevice, st);
     if (e != hipSuccess) return hip_fail(ctx, e, "hipMemcpyAsync(tables)");
     e = hipEventRecord(P.upload_done, st);
@@ -787,11 +836,36 @@ int enqueue(enm_ctx* ctx, Plan& P, const double* arena, double* out, double* d_e
     nl += 4;
     if ((rc = mark(1))) return rc;
     int fb = 0;
-    if ((rc = chk(enm_launch_sort(tab<SortSeg>(P, P.off_segs), (int)P.segs.size(), P.ntiles, P.maxlen,
-                                  ENM_SORT_TILE, arena,
-                                  P.sk0.as<double>(), P.si0.as<int>(), P.sk1.as<double>(), P.si1.as<int>(),
-                                  P.split.as<int>(), &fb, &nl, st), "sort")))
+    if (P.sample_sort) {
+        SampleSortArgs a;
+        a.segs = tab<SortSeg>(P, P.off_segs);
+        a.nsegs = (int)P.segs.size();
+        a.ntiles = P.ntiles;
+        a.ss = tab<SampleSeg>(P, P.off_ssegs);
+        a.samp_segs = tab<SortSeg>(P, P.off_samp_segs);
+        a.n_samp_segs = (int)P.samp_segs.size();
+        a.samp_tiles = P.samp_tiles;
+        a.samp_maxlen = P.samp_maxlen;
+        a.samp_k0 = P.samp_k0.as<double>();
+        a.samp_k1 = P.samp_k1.as<double>();
+        a.samp_i0 = P.samp_i0.as<int>();
+        a.samp_i1 = P.samp_i1.as<int>();
+        a.samp_split = P.samp_split.as<int>();
+        a.bid = P.bid.as<unsigned short>();
+        a.counts = P.ss_counts.as<int>();
+        a.tiles = P.ss_tiles.as<SsTile>();
+        a.tile_count = P.ss_ctl.as<unsigned>();
+        // ss_tile_kernel holds 48 KB of LDS: three blocks per CU.
+        a.tile_grid = std::max(1, ctx->num_cus * 3);
+        if ((rc = chk(enm_launch_sample_sort(a, arena, P.sk0.as<double>(), P.si0.as<int>(), P.sk1.as<double>(),
+                                             P.si1.as<int>(), &nl, st), "sort")))
+            return rc;
+    } else if ((rc = chk(enm_launch_sort(tab<SortSeg>(P, P.off_segs), (int)P.segs.size(), P.ntiles, P.maxlen,
+                                         ENM_SORT_TILE, arena, nullptr,
+                                         P.sk0.as<double>(), P.si0.as<int>(), P.sk1.as<double>(), P.si1.as<int>(),
+                                         P.split.as<int>(), &fb, &nl, st), "sort"))) {
         return rc;
+    }
     if ((rc = mark(2))) return rc;
     const double* skey = fb ? P.sk1.as<double>() : P.sk0.as<double>();
     const int* sidx = fb ? P.si1.as<int>() : P.si0.as<int>();
@@ -813,7 +887,7 @@ int enqueue(enm_ctx* ctx, Plan& P, const double* arena, double* out, double* d_e
                           "sort")))
                 return rc;
         } else if ((rc = chk(enm_launch_sort(tab<SortSeg>(P, P.off_segs2), (int)P.segs2.size(), P.ntiles2, P.maxlen2,
-                                      P.tile2, P.col64.as<double>(), P.sk0.as<double>(), P.si0.as<int>(),
+                                      P.tile2, P.col64.as<double>(), nullptr, P.sk0.as<double>(), P.si0.as<int>(),
                                       P.sk1.as<double>(), P.si1.as<int>(), P.split.as<int>(), &fb, &nl, st),
                       "sort"))) {
             return rc;

@@ -47,6 +47,33 @@ struct SortSeg {
     int32_t pad;
 };
 
+// Sample sort of the first-stage segments (segments of up to ENM_SS_MAXLEN
+// keys): each segment is split into nb buckets by nb - 1 splitters drawn from
+// ENM_SS_OVER * nb sorted samples, its keys are scattered bucket by bucket,
+// and runs of whole buckets of at most ENM_SS_TILE keys are sorted in LDS.
+#define ENM_SS_BUCKET 512      // target keys per bucket
+#define ENM_SS_OVER 16         // samples per bucket
+#define ENM_SS_MAXB 1024       // most buckets of one segment
+#define ENM_SS_MAXLEN (ENM_SS_MAXB * 1024)
+#define ENM_SS_TILE 4096       // keys per LDS-sorted run of buckets
+#define ENM_SS_THREADS 512
+
+// Per first-stage segment (same index as its SortSeg).
+struct SampleSeg {
+    int64_t cbase;  // its (chunk, bucket) count table: chunk-major, nb per chunk
+    int32_t samp;   // offset of its samples in the sample buffers
+    int32_t ns;     // number of samples: ENM_SS_OVER * nb, or 0 when nb == 1
+    int32_t nb;     // number of buckets
+    int32_t pad;
+};
+
+// One run of whole buckets to sort: sort-buffer positions [pos, pos + len).
+struct SsTile {
+    int64_t pos;
+    int32_t len;
+    int32_t pad;
+};
+
 // Gather work: one ENM_SORT_TILE chunk of one sort segment into one order.
 struct GatherItem {
     int32_t order;

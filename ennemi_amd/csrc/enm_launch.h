@@ -10,8 +10,28 @@ hipError_t enm_launch_fill_u64(unsigned long long* p, int64_t n, unsigned long l
 hipError_t enm_launch_fill_i32(int* p, int64_t n, int v, hipStream_t st);
 
 hipError_t enm_launch_sort(const SortSeg* d_segs, int nsegs, int ntiles, int64_t maxlen, int tile,
-                           const double* arena, double* k0, int* i0, double* k1, int* i1, int* split,
-                           int* final_buf, int* n_launch, hipStream_t st);
+                           const double* arena, const int* iarena, double* k0, int* i0, double* k1, int* i1,
+                           int* split, int* final_buf, int* n_launch, hipStream_t st);
+
+// Sample sort of the first-stage segments (all at most ENM_SS_MAXLEN keys)
+// into (k0, i0); (k1, i1) hold the scattered buckets on the way.
+struct SampleSortArgs {
+    const SortSeg* segs;
+    int nsegs, ntiles;
+    const SampleSeg* ss;
+    const SortSeg* samp_segs;   // the segments' sample ranges (those with samples)
+    int n_samp_segs, samp_tiles;
+    int64_t samp_maxlen;
+    double *samp_k0, *samp_k1;
+    int *samp_i0, *samp_i1, *samp_split;
+    unsigned short* bid;        // bucket of every key, by sort-buffer position
+    int* counts;                // (chunk, bucket) counts, then scatter offsets
+    SsTile* tiles;              // runs of buckets (at most the total bucket count)
+    unsigned* tile_count;
+    int tile_grid;
+};
+hipError_t enm_launch_sample_sort(const SampleSortArgs& a, const double* arena, double* k0, int* i0,
+                                  double* k1, int* i1, int* n_launch, hipStream_t st);
 hipError_t enm_launch_strip_sort(const GatherItem* d_items, int nitems, const SortSeg* d_segs,
                                  const OrderDev* d_orders, int tile, double* col64, float* col32, int* perm,
                                  int with32, int* n_launch, hipStream_t st);

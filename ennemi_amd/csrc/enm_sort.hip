@@ -14,6 +14,16 @@
 // its two input slices in LDS and each thread merges 8 outputs.  Keys compare as
 // (value, index) pairs, so all keys are distinct and the order is total.
 // Traffic per pass is 12 B read + 12 B written per key (HBM-bound).
+//
+// First-stage segments of up to ENM_SS_MAXLEN keys are sample-sorted
+// instead: nb - 1 splitters from ENM_SS_OVER * nb sorted samples cut a
+// segment into nb buckets (ss_count_kernel counts every 2048-key chunk's keys
+// per bucket, ss_scan_kernel turns the counts into scatter offsets and packs
+// consecutive buckets into runs of at most ENM_SS_TILE keys,
+// ss_scatter_kernel moves the keys bucket by bucket), and ss_tile_kernel
+// sorts each run in LDS.  56 B of traffic per key instead of 24 B per merge
+// pass.  A run of one bucket larger than ENM_SS_TILE (sampling bad luck) is
+// merge-sorted by one block through global memory, so every input sorts.
 #include "enm_device.h"
 #include "enm_launch.h"
 
@@ -41,8 +51,8 @@ __device__ __forceinline__ void cex_pair(double& ka, int& ia, double& kb, int& i
     ka = k0; ia = i0; kb = k1; ib = i1;
 }
 
-// Sort PER (key, index) pairs held in registers (PER = 2, 4 or 8; Batcher's
-// odd-even merge networks).
+// Sort PER (key, index) pairs held in registers (PER = 2, 4, 8 or 16;
+// Batcher's odd-even merge sorting networks: 1, 5, 19 and 63 comparators).
 template <int PER>
 __device__ __forceinline__ void sort_regs(double (&k)[PER], int (&x)[PER]) {
 #define ENM_CX(a, b) cex_pair(k[a], x[a], k[b], x[b])
@@ -50,85 +60,174 @@ __device__ __forceinline__ void sort_regs(double (&k)[PER], int (&x)[PER]) {
         ENM_CX(0, 1);
     } else if constexpr (PER == 4) {
         ENM_CX(0, 1); ENM_CX(2, 3); ENM_CX(0, 2); ENM_CX(1, 3); ENM_CX(1, 2);
-    } else {
-        static_assert(PER == 8, "2, 4 or 8 per thread");
+    } else if constexpr (PER == 8) {
         ENM_CX(0, 1); ENM_CX(2, 3); ENM_CX(4, 5); ENM_CX(6, 7);
         ENM_CX(0, 2); ENM_CX(1, 3); ENM_CX(4, 6); ENM_CX(5, 7);
         ENM_CX(1, 2); ENM_CX(5, 6);
         ENM_CX(0, 4); ENM_CX(1, 5); ENM_CX(2, 6); ENM_CX(3, 7);
         ENM_CX(2, 4); ENM_CX(3, 5);
         ENM_CX(1, 2); ENM_CX(3, 4); ENM_CX(5, 6);
+    } else {
+        static_assert(PER == 16, "2, 4, 8 or 16 per thread");
+        // Two sorted halves of 8 (the 19-comparator network on each) ...
+        ENM_CX(0, 1); ENM_CX(2, 3); ENM_CX(0, 2); ENM_CX(1, 3); ENM_CX(1, 2);
+        ENM_CX(4, 5); ENM_CX(6, 7); ENM_CX(4, 6); ENM_CX(5, 7); ENM_CX(5, 6);
+        ENM_CX(0, 4); ENM_CX(2, 6); ENM_CX(2, 4); ENM_CX(1, 5); ENM_CX(3, 7); ENM_CX(3, 5);
+        ENM_CX(1, 2); ENM_CX(3, 4); ENM_CX(5, 6);
+        ENM_CX(8, 9); ENM_CX(10, 11); ENM_CX(8, 10); ENM_CX(9, 11); ENM_CX(9, 10);
+        ENM_CX(12, 13); ENM_CX(14, 15); ENM_CX(12, 14); ENM_CX(13, 15); ENM_CX(13, 14);
+        ENM_CX(8, 12); ENM_CX(10, 14); ENM_CX(10, 12); ENM_CX(9, 13); ENM_CX(11, 15); ENM_CX(11, 13);
+        ENM_CX(9, 10); ENM_CX(11, 12); ENM_CX(13, 14);
+        // ... and their odd-even merge (25 comparators).
+        ENM_CX(0, 8); ENM_CX(4, 12); ENM_CX(4, 8); ENM_CX(2, 10); ENM_CX(6, 14); ENM_CX(6, 10);
+        ENM_CX(2, 4); ENM_CX(6, 8); ENM_CX(10, 12); ENM_CX(1, 9); ENM_CX(5, 13); ENM_CX(5, 9);
+        ENM_CX(3, 11); ENM_CX(7, 15); ENM_CX(7, 11); ENM_CX(3, 5); ENM_CX(7, 9); ENM_CX(11, 13);
+        ENM_CX(1, 2); ENM_CX(3, 4); ENM_CX(5, 6); ENM_CX(7, 8); ENM_CX(9, 10); ENM_CX(11, 12);
+        ENM_CX(13, 14);
     }
 #undef ENM_CX
 }
 
-// Number of A elements among the first d outputs of merge(A, B).
+// Number of A elements among the first d outputs of merge(A, B).  Keys
+// are compared first; the indices are read only where some lane's keys tie.
 __device__ int co_rank(int d, const double* ka, const int* ia, int na,
                        const double* kb, const int* ib, int nb) {
     int lo = max(0, d - nb), hi = min(d, na);
     while (lo < hi) {
         const int i = (lo + hi) >> 1;
         const int j = d - i - 1;
-        if (kless(ka[i], ia[i], kb[j], ib[j])) lo = i + 1; else hi = i;
+        const double x = ka[i], y = kb[j];
+        bool lt = x < y;
+        if (x == y) lt = ia[i] < ib[j];
+        if (lt) lo = i + 1; else hi = i;
     }
     return lo;
 }
 
-// Sort the TILE (key, index) pairs of sk/si in LDS with a 256-thread block:
-// each thread sorts TILE/256 consecutive pairs in registers, then merge
-// levels of doubling run width, each thread producing TILE/256 consecutive
-// outputs of its pair of runs from a merge-path split (outputs kept in
-// registers across a barrier, so the merge works in place).  Padding keys
-// (+inf, INT_MAX) sort last.  Starts and ends with a barrier.
-template <int TILE>
+// LDS tiles of (key, index) pairs keep one unused slot after every 32
+// elements (element i at lp(i)).  A thread's PER consecutive elements then
+// start at a different bank from its neighbour's: without the slot, the 64
+// lanes' 8-element runs (64 B apart) fall on the same few banks and every
+// register-phase read and write serializes 16 ways.
+#define ENM_LDS_PAD(n) ((n) + ((n) >> 5))
+__device__ __forceinline__ int lp(int i) { return i + (i >> 5); }
+
+// co_rank over an LDS tile: A at elements [a0, a0 + na), B at [b0, b0 + nb).
+__device__ __forceinline__ int co_rank_lds(int d, const double* sk, const int* si, int a0, int na,
+                                           int b0, int nb) {
+    int lo = max(0, d - nb), hi = min(d, na);
+    while (lo < hi) {
+        const int i = (lo + hi) >> 1;
+        const int pa = lp(a0 + i), pb = lp(b0 + d - i - 1);
+        const double x = sk[pa], y = sk[pb];
+        bool lt = x < y;
+        if (x == y) lt = si[pa] < si[pb];
+        if (lt) lo = i + 1; else hi = i;
+    }
+    return lo;
+}
+
+// The PER outputs of merge(A, B) of an LDS tile (A at [a0, a0 + na), B at
+// [b0, b0 + nb), na, nb >= 1) from merge-path split (a, b): the two heads
+// stay in registers and each step reloads only the side it consumed.
+// Clamped reads stay inside the slices; the flags decide.
+template <int PER>
+__device__ __forceinline__ void seq_merge(const double* sk, const int* si, int a0, int na, int b0, int nb,
+                                          int a, int b, int ne, double (&k)[PER], int (&x)[PER]) {
+    double ha = sk[lp(a0 + min(a, na - 1))], hb = sk[lp(b0 + min(b, nb - 1))];
+    int xa = si[lp(a0 + min(a, na - 1))], xb = si[lp(b0 + min(b, nb - 1))];
+#pragma unroll
+    for (int e = 0; e < PER; ++e) {
+        if (e < ne) {
+            const bool take_a = b >= nb || (a < na && kless(ha, xa, hb, xb));
+            k[e] = take_a ? ha : hb;
+            x[e] = take_a ? xa : xb;
+            if (e + 1 < PER) {
+                a += take_a ? 1 : 0;
+                b += take_a ? 0 : 1;
+                const int src = lp(take_a ? a0 + min(a, na - 1) : b0 + min(b, nb - 1));
+                const double v = sk[src];
+                const int vi = si[src];
+                ha = take_a ? v : ha;
+                xa = take_a ? vi : xa;
+                hb = take_a ? hb : v;
+                xb = take_a ? xb : vi;
+            }
+        }
+    }
+}
+
+// Sort the TILE (key, index) pairs of an LDS tile (ENM_LDS_PAD(TILE) slots)
+// with an NT-thread block: each thread sorts TILE/NT consecutive pairs in
+// registers, then merge levels of doubling run width, each thread producing
+// TILE/NT consecutive outputs of its pair of runs from a merge-path split
+// (outputs kept in registers across a barrier, so the merge works in place).
+// Padding keys (+inf, INT_MAX) sort last.  Starts and ends with a barrier.
+template <int TILE, int NT = ENM_SORT_THREADS>
 __device__ __forceinline__ void block_sort_lds(double* sk, int* si) {
-    constexpr int PER = TILE / ENM_SORT_THREADS;
+    constexpr int PER = TILE / NT;
     __syncthreads();
     const int e0 = threadIdx.x * PER;
     double k[PER];
     int x[PER];
 #pragma unroll
-    for (int e = 0; e < PER; ++e) { k[e] = sk[e0 + e]; x[e] = si[e0 + e]; }
+    for (int e = 0; e < PER; ++e) { k[e] = sk[lp(e0 + e)]; x[e] = si[lp(e0 + e)]; }
     sort_regs<PER>(k, x);
     __syncthreads();
 #pragma unroll
-    for (int e = 0; e < PER; ++e) { sk[e0 + e] = k[e]; si[e0 + e] = x[e]; }
+    for (int e = 0; e < PER; ++e) { sk[lp(e0 + e)] = k[e]; si[lp(e0 + e)] = x[e]; }
     __syncthreads();
     for (int run = PER; run < TILE; run <<= 1) {
         const int pb = (e0 / (2 * run)) * (2 * run);
-        const double* ka = sk + pb;
-        const int* ia = si + pb;
-        const double* kb = ka + run;
-        const int* ib = ia + run;
         const int d = e0 - pb;
-        int a = co_rank(d, ka, ia, run, kb, ib, run);
-        int b = d - a;
-#pragma unroll
-        for (int e = 0; e < PER; ++e) {
-            // Clamped reads stay inside the pair; the flags decide.
-            const int ac = min(a, run - 1), bc = min(b, run - 1);
-            const bool take_a = b >= run || (a < run && kless(ka[ac], ia[ac], kb[bc], ib[bc]));
-            const int src = take_a ? ac : run + bc;
-            k[e] = ka[src];
-            x[e] = ia[src];
-            if (take_a) ++a; else ++b;
-        }
+        const int a = co_rank_lds(d, sk, si, pb, run, pb + run, run);
+        seq_merge<PER>(sk, si, pb, run, pb + run, run, a, d - a, PER, k, x);
         __syncthreads();
 #pragma unroll
-        for (int e = 0; e < PER; ++e) { sk[e0 + e] = k[e]; si[e0 + e] = x[e]; }
+        for (int e = 0; e < PER; ++e) { sk[lp(e0 + e)] = k[e]; si[lp(e0 + e)] = x[e]; }
         __syncthreads();
     }
 }
 
+// Merge the staged sorted slices [0, la) and [la, la + lb) of an LDS tile
+// (la + lb <= NT * PER) in place: each thread takes PER consecutive outputs
+// into registers from its merge-path split, and they go back after a
+// barrier.  The caller synchronizes after staging; ends with a barrier.
+template <int NT, int PER>
+__device__ __forceinline__ void lds_merge(double* sk, int* si, int la, int lb) {
+    if (la == 0 || lb == 0) {   // one slice: already in order (block-uniform)
+        __syncthreads();
+        return;
+    }
+    const int e0 = threadIdx.x * PER;
+    const int ne = min(PER, la + lb - e0);
+    double ok[PER];
+    int oi[PER];
+    if (ne > 0) {
+        const int a = co_rank_lds(e0, sk, si, 0, la, la, lb);
+        seq_merge<PER>(sk, si, 0, la, la, lb, a, e0 - a, ne, ok, oi);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int e = 0; e < PER; ++e) {
+        if (e < ne) {
+            sk[lp(e0 + e)] = ok[e];
+            si[lp(e0 + e)] = oi[e];
+        }
+    }
+    __syncthreads();
+}
+
 // Sort one TILE chunk of a segment: keys arena[src + off + t] with their
-// point indices p0 + off + t, into the sort buffers.
+// point indices p0 + off + t (or iarena[src + off + t] when given), into the
+// sort buffers.
 template <int TILE>
 __global__ __launch_bounds__(ENM_SORT_THREADS)
 void block_sort_kernel(const SortSeg* __restrict__ segs, int nsegs,
-                       const double* __restrict__ arena,
+                       const double* __restrict__ arena, const int* __restrict__ iarena,
                        double* __restrict__ kout, int* __restrict__ iout) {
-    __shared__ double sk[TILE];
-    __shared__ int si[TILE];
+    __shared__ double sk[ENM_LDS_PAD(TILE)];
+    __shared__ int si[ENM_LDS_PAD(TILE)];
     const int tile = blockIdx.x;
     const int s = seg_of_tile(segs, nsegs, tile);
     const SortSeg sg = segs[s];
@@ -136,17 +235,17 @@ void block_sort_kernel(const SortSeg* __restrict__ segs, int nsegs,
     const int len = min(TILE, sg.len - off);
     for (int t = threadIdx.x; t < TILE; t += ENM_SORT_THREADS) {
         if (t < len) {
-            sk[t] = arena[sg.src + off + t];
-            si[t] = sg.p0 + off + t;
+            sk[lp(t)] = arena[sg.src + off + t];
+            si[lp(t)] = iarena ? iarena[sg.src + off + t] : sg.p0 + off + t;
         } else {
-            sk[t] = INFINITY;
-            si[t] = 0x7fffffff;
+            sk[lp(t)] = INFINITY;
+            si[lp(t)] = 0x7fffffff;
         }
     }
     block_sort_lds<TILE>(sk, si);
     for (int t = threadIdx.x; t < len; t += ENM_SORT_THREADS) {
-        kout[sg.sbase + off + t] = sk[t];
-        iout[sg.sbase + off + t] = si[t];
+        kout[sg.sbase + off + t] = sk[lp(t)];
+        iout[sg.sbase + off + t] = si[lp(t)];
     }
 }
 
@@ -162,26 +261,26 @@ __global__ __launch_bounds__(ENM_SORT_THREADS)
 void strip_sort_kernel(const GatherItem* __restrict__ items, const SortSeg* __restrict__ segs,
                        const OrderDev* __restrict__ orders, double* col64, float* col32, int* perm,
                        int with32) {
-    __shared__ double sk[TILE];
-    __shared__ int si[TILE];
+    __shared__ double sk[ENM_LDS_PAD(TILE)];
+    __shared__ int si[ENM_LDS_PAD(TILE)];
     const GatherItem it = items[blockIdx.x];
     const SortSeg sg = segs[it.sseg];
     const OrderDev& od = orders[it.order];
     const int len = sg.len;
     for (int t = threadIdx.x; t < TILE; t += ENM_SORT_THREADS) {
         if (t < len) {
-            sk[t] = col64[sg.src + t];
-            si[t] = sg.p0 + t;
+            sk[lp(t)] = col64[sg.src + t];
+            si[lp(t)] = sg.p0 + t;
         } else {
-            sk[t] = INFINITY;
-            si[t] = 0x7fffffff;
+            sk[lp(t)] = INFINITY;
+            si[lp(t)] = 0x7fffffff;
         }
     }
     block_sort_lds<TILE>(sk, si);
     const int64_t n = od.n;
     const int ncol = od.ncol;
     for (int t = threadIdx.x; t < len; t += ENM_SORT_THREADS) {
-        const int idx = si[t];
+        const int idx = si[lp(t)];
         const int64_t p = it.pos0 + t;
         perm[od.perm + p] = perm[od.base_perm + idx];
         for (int c = 0; c < ncol; ++c) {
@@ -231,8 +330,8 @@ void merge_pass_kernel(const SortSeg* __restrict__ segs, int nsegs, int64_t widt
                        const double* __restrict__ kin, const int* __restrict__ iin,
                        const int* __restrict__ split,
                        double* __restrict__ kout, int* __restrict__ iout) {
-    __shared__ double sk[ENM_SORT_TILE];
-    __shared__ int si[ENM_SORT_TILE];
+    __shared__ double sk[ENM_LDS_PAD(ENM_SORT_TILE)];
+    __shared__ int si[ENM_LDS_PAD(ENM_SORT_TILE)];
     const int tile = blockIdx.x;
     const SortSeg sg = segs[seg_of_tile(segs, nsegs, tile)];
     const MergeChunk m = merge_chunk(sg, tile, width);
@@ -255,45 +354,15 @@ void merge_pass_kernel(const SortSeg* __restrict__ segs, int nsegs, int64_t widt
     const int j0 = d0 - i0, j1 = d1 - i1;
     const int la = i1 - i0, lb = j1 - j0;
     for (int t = threadIdx.x; t < la + lb; t += ENM_SORT_THREADS) {
-        if (t < la) { sk[t] = ka[i0 + t]; si[t] = ia[i0 + t]; }
-        else        { sk[t] = kb[j0 + t - la]; si[t] = ib[j0 + t - la]; }
+        if (t < la) { sk[lp(t)] = ka[i0 + t]; si[lp(t)] = ia[i0 + t]; }
+        else        { sk[lp(t)] = kb[j0 + t - la]; si[lp(t)] = ib[j0 + t - la]; }
     }
     __syncthreads();
-    // Each thread merges 8 consecutive outputs into registers; they go back
-    // to LDS and out with coalesced stores.
-    constexpr int PER = ENM_SORT_TILE / ENM_SORT_THREADS;
-    const int e0 = threadIdx.x * PER;
-    const int ne = min(PER, la + lb - e0);
-    double ok[PER];
-    int oi[PER];
-    if (ne > 0) {
-        int a = co_rank(e0, sk, si, la, sk + la, si + la, lb);
-        int b = e0 - a;
-#pragma unroll
-        for (int e = 0; e < PER; ++e) {
-            if (e < ne) {
-                // Clamped reads stay inside the staged slices; the flags decide.
-                const int ac = max(0, min(a, la - 1)), bc = la + max(0, min(b, lb - 1));
-                const bool take_a = lb == 0 || b >= lb || (a < la && kless(sk[ac], si[ac], sk[bc], si[bc]));
-                const int src = take_a ? ac : bc;
-                ok[e] = sk[src];
-                oi[e] = si[src];
-                if (take_a) ++a; else ++b;
-            }
-        }
-    }
-    __syncthreads();
-#pragma unroll
-    for (int e = 0; e < PER; ++e) {
-        if (e < ne) {
-            sk[e0 + e] = ok[e];
-            si[e0 + e] = oi[e];
-        }
-    }
-    __syncthreads();
+    // Merged in LDS, then out with coalesced stores.
+    lds_merge<ENM_SORT_THREADS, ENM_SORT_TILE / ENM_SORT_THREADS>(sk, si, la, lb);
     for (int t = threadIdx.x; t < la + lb; t += ENM_SORT_THREADS) {
-        kout[base + d0 + t] = sk[t];
-        iout[base + d0 + t] = si[t];
+        kout[base + d0 + t] = sk[lp(t)];
+        iout[base + d0 + t] = si[lp(t)];
     }
 }
 
@@ -358,6 +427,334 @@ void gather_kernel(const GatherItem* __restrict__ items,
     }
 }
 
+// ---------------------------------------------------------------------------
+// Sample sort of the first-stage segments.
+
+// Exclusive prefix sums of v[0, n) in place (n <= 4 * NT), `part` being
+// NT / 64 ints of LDS scratch.  Call after a barrier; ends with one.
+template <int NT>
+__device__ __forceinline__ void block_exscan(int* v, int n, int* part) {
+    constexpr int EPT = 4;
+    const int t = threadIdx.x;
+    int loc[EPT];
+    int s = 0;
+#pragma unroll
+    for (int e = 0; e < EPT; ++e) {
+        const int i = t * EPT + e;
+        loc[e] = i < n ? v[i] : 0;
+        s += loc[e];
+    }
+    const int lane = t & (ENM_WAVE - 1);
+    int inc = s;
+#pragma unroll
+    for (int off = 1; off < ENM_WAVE; off <<= 1) {
+        const int y = __shfl_up(inc, off, ENM_WAVE);
+        if (lane >= off) inc += y;
+    }
+    if (lane == ENM_WAVE - 1) part[t / ENM_WAVE] = inc;
+    __syncthreads();
+    int run = inc - s;
+    for (int w = 0; w < t / ENM_WAVE; ++w) run += part[w];
+#pragma unroll
+    for (int e = 0; e < EPT; ++e) {
+        const int i = t * EPT + e;
+        if (i < n) v[i] = run;
+        run += loc[e];
+    }
+    __syncthreads();
+}
+
+// Position of sample i of ns in a segment of len keys: one position drawn
+// from each of ns equal strides by an integer hash of i, so periodic inputs
+// do not bias the splitters.  The positions are distinct.
+__device__ __forceinline__ int64_t ss_sample_pos(int i, int ns, int64_t len) {
+    uint32_t h = (uint32_t)i * 0x9E3779B1u;
+    h ^= h >> 15;
+    h *= 0x85EBCA77u;
+    h ^= h >> 13;
+    const int64_t step = len / ns;
+    return ((int64_t)i * len) / ns + (int64_t)(h % (uint32_t)step);
+}
+
+// One block per segment: copy its samples (key, point index) out.  Block 0
+// also clears the run counter that ss_scan_kernel fills.
+__global__ __launch_bounds__(ENM_SORT_THREADS)
+void ss_sample_kernel(const SortSeg* __restrict__ segs, const SampleSeg* __restrict__ ss,
+                      const double* __restrict__ arena, double* __restrict__ sk, int* __restrict__ si,
+                      unsigned* __restrict__ tile_count) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) *tile_count = 0u;
+    const SortSeg sg = segs[blockIdx.x];
+    const SampleSeg q = ss[blockIdx.x];
+    for (int i = threadIdx.x; i < q.ns; i += ENM_SORT_THREADS) {
+        const int64_t p = ss_sample_pos(i, q.ns, sg.len);
+        sk[q.samp + i] = arena[sg.src + p];
+        si[q.samp + i] = sg.p0 + (int)p;
+    }
+}
+
+// One block per 2048-key chunk: the bucket of each key (the number of
+// splitters not above it in (value, index) order), kept for the scatter, and
+// the chunk's keys per bucket.  Splitter j (j = 1 .. nb-1) is sorted sample
+// ENM_SS_OVER * j.
+__global__ __launch_bounds__(ENM_SORT_THREADS)
+void ss_count_kernel(const SortSeg* __restrict__ segs, int nsegs, const SampleSeg* __restrict__ ss,
+                     const double* __restrict__ arena, const double* __restrict__ samp_k,
+                     const int* __restrict__ samp_i, unsigned short* __restrict__ bid, int* __restrict__ counts) {
+    __shared__ double spk[ENM_SS_MAXB];
+    __shared__ int spi[ENM_SS_MAXB];
+    __shared__ int hist[ENM_SS_MAXB];
+    const int tile = blockIdx.x;
+    const int s = seg_of_tile(segs, nsegs, tile);
+    const SortSeg sg = segs[s];
+    const SampleSeg q = ss[s];
+    const int nb = q.nb, nsp = nb - 1;
+    for (int j = threadIdx.x; j < nsp; j += ENM_SORT_THREADS) {
+        spk[j] = samp_k[q.samp + ENM_SS_OVER * (j + 1)];
+        spi[j] = samp_i[q.samp + ENM_SS_OVER * (j + 1)];
+    }
+    for (int b = threadIdx.x; b < nb; b += ENM_SORT_THREADS) hist[b] = 0;
+    __syncthreads();
+    const int off = (tile - sg.first_tile) * ENM_SORT_TILE;
+    const int len = min(ENM_SORT_TILE, sg.len - off);
+    for (int t = threadIdx.x; t < len; t += ENM_SORT_THREADS) {
+        const double key = arena[sg.src + off + t];
+        const int idx = sg.p0 + off + t;
+        int lo = 0, hi = nsp;
+        while (lo < hi) {
+            const int mid = (lo + hi) >> 1;
+            if (kless(key, idx, spk[mid], spi[mid])) hi = mid; else lo = mid + 1;
+        }
+        bid[sg.sbase + off + t] = (unsigned short)lo;
+        atomicAdd(&hist[lo], 1);
+    }
+    __syncthreads();
+    int* row = counts + q.cbase + (int64_t)(tile - sg.first_tile) * nb;
+    for (int b = threadIdx.x; b < nb; b += ENM_SORT_THREADS) row[b] = hist[b];
+}
+
+// One block per segment: turn the (chunk, bucket) counts into segment
+// positions (bucket-major, chunk-minor: the first position of chunk c's keys
+// of bucket b), and pack consecutive buckets greedily into runs of at most
+// ENM_SS_TILE keys (a larger bucket is a run of its own), appended to the
+// run list.
+__global__ __launch_bounds__(ENM_SORT_THREADS)
+void ss_scan_kernel(const SortSeg* __restrict__ segs, const SampleSeg* __restrict__ ss,
+                    int* __restrict__ counts, SsTile* __restrict__ tiles, unsigned* __restrict__ tile_count) {
+    __shared__ int tot[ENM_SS_MAXB];
+    __shared__ int start[ENM_SS_MAXB];
+    __shared__ int part[ENM_SORT_THREADS / ENM_WAVE];
+    const SortSeg sg = segs[blockIdx.x];
+    const SampleSeg q = ss[blockIdx.x];
+    const int nb = q.nb;
+    const int nch = (sg.len + ENM_SORT_TILE - 1) / ENM_SORT_TILE;
+    int* cnt = counts + q.cbase;
+    for (int b = threadIdx.x; b < nb; b += ENM_SORT_THREADS) {
+        int sum = 0;
+        for (int c = 0; c < nch; ++c) sum += cnt[(int64_t)c * nb + b];
+        tot[b] = sum;
+        start[b] = sum;
+    }
+    __syncthreads();
+    block_exscan<ENM_SORT_THREADS>(start, nb, part);
+    for (int b = threadIdx.x; b < nb; b += ENM_SORT_THREADS) {
+        int run = start[b];
+        for (int c = 0; c < nch; ++c) {
+            const int v = cnt[(int64_t)c * nb + b];
+            cnt[(int64_t)c * nb + b] = run;
+            run += v;
+        }
+    }
+    if (threadIdx.x == 0) {
+        int nt = 0, cur = 0;
+        for (int b = 0; b < nb; ++b) {
+            const int z = tot[b];
+            if (cur > 0 && cur + z > ENM_SS_TILE) { ++nt; cur = 0; }
+            cur += z;
+        }
+        if (cur > 0) ++nt;
+        const unsigned base = atomicAdd(tile_count, (unsigned)nt);
+        int k = 0, t0 = 0;
+        cur = 0;
+        for (int b = 0; b < nb; ++b) {
+            const int z = tot[b];
+            if (cur > 0 && cur + z > ENM_SS_TILE) {
+                tiles[base + k++] = SsTile{sg.sbase + t0, cur, 0};
+                t0 += cur;
+                cur = 0;
+            }
+            cur += z;
+        }
+        if (cur > 0) tiles[base + k] = SsTile{sg.sbase + t0, cur, 0};
+    }
+}
+
+// One block per 2048-key chunk: order the chunk's keys by bucket in LDS (a
+// counting sort; the order inside a bucket does not matter, as the runs are
+// sorted whole later) and write each bucket's keys to their positions with
+// coalesced stores.
+__global__ __launch_bounds__(ENM_SORT_THREADS)
+void ss_scatter_kernel(const SortSeg* __restrict__ segs, int nsegs, const SampleSeg* __restrict__ ss,
+                       const double* __restrict__ arena, const unsigned short* __restrict__ bid,
+                       const int* __restrict__ offs, double* __restrict__ kout, int* __restrict__ iout) {
+    constexpr int PER = ENM_SORT_TILE / ENM_SORT_THREADS;
+    __shared__ double sk[ENM_SORT_TILE];
+    __shared__ int si[ENM_SORT_TILE];
+    __shared__ unsigned short sb[ENM_SORT_TILE];
+    __shared__ int hist[ENM_SS_MAXB];
+    __shared__ int goff[ENM_SS_MAXB];
+    __shared__ int part[ENM_SORT_THREADS / ENM_WAVE];
+    const int tile = blockIdx.x;
+    const int s = seg_of_tile(segs, nsegs, tile);
+    const SortSeg sg = segs[s];
+    const SampleSeg q = ss[s];
+    const int nb = q.nb;
+    const int off = (tile - sg.first_tile) * ENM_SORT_TILE;
+    const int len = min(ENM_SORT_TILE, sg.len - off);
+    const int* row = offs + q.cbase + (int64_t)(tile - sg.first_tile) * nb;
+    for (int b = threadIdx.x; b < nb; b += ENM_SORT_THREADS) {
+        hist[b] = 0;
+        goff[b] = row[b];
+    }
+    __syncthreads();
+    double key[PER];
+    int bk[PER], rk[PER];
+#pragma unroll
+    for (int e = 0; e < PER; ++e) {
+        const int t = threadIdx.x + e * ENM_SORT_THREADS;
+        if (t < len) {
+            key[e] = arena[sg.src + off + t];
+            bk[e] = bid[sg.sbase + off + t];
+            rk[e] = atomicAdd(&hist[bk[e]], 1);
+        }
+    }
+    __syncthreads();
+    block_exscan<ENM_SORT_THREADS>(hist, nb, part);
+#pragma unroll
+    for (int e = 0; e < PER; ++e) {
+        const int t = threadIdx.x + e * ENM_SORT_THREADS;
+        if (t < len) {
+            const int p = hist[bk[e]] + rk[e];
+            sk[p] = key[e];
+            si[p] = sg.p0 + off + t;
+            sb[p] = (unsigned short)bk[e];
+        }
+    }
+    __syncthreads();
+    for (int t = threadIdx.x; t < len; t += ENM_SORT_THREADS) {
+        const int b = sb[t];
+        const int64_t dst = sg.sbase + goff[b] + (t - hist[b]);
+        kout[dst] = sk[t];
+        iout[dst] = si[t];
+    }
+}
+
+// Merge sort of one run longer than ENM_SS_TILE by the whole block: sort
+// ENM_SS_TILE chunks in LDS from the scattered buffers (k1, i1) into the
+// output buffers (k0, i0), then merge passes of doubling width between the
+// two buffers' ranges of the run (which no other block touches), each
+// ENM_SS_TILE outputs merged in LDS from a merge-path split; the result ends
+// in (k0, i0).
+__device__ void ss_sort_long(double* sk, int* si, int* sp, int64_t pos, int len,
+                             double* k1, int* i1, double* k0, int* i0) {
+    constexpr int T = ENM_SS_TILE, NT = ENM_SS_THREADS;
+    for (int c0 = 0; c0 < len; c0 += T) {
+        const int cl = min(T, len - c0);
+        for (int t = threadIdx.x; t < T; t += NT) {
+            if (t < cl) {
+                sk[lp(t)] = k1[pos + c0 + t];
+                si[lp(t)] = i1[pos + c0 + t];
+            } else {
+                sk[lp(t)] = INFINITY;
+                si[lp(t)] = 0x7fffffff;
+            }
+        }
+        block_sort_lds<T, NT>(sk, si);
+        for (int t = threadIdx.x; t < cl; t += NT) {
+            k0[pos + c0 + t] = sk[lp(t)];
+            i0[pos + c0 + t] = si[lp(t)];
+        }
+        __syncthreads();
+    }
+    double* ks = k0;
+    int* is = i0;
+    double* kd = k1;
+    int* id = i1;
+    for (int w = T; w < len; w *= 2) {
+        for (int pb = 0; pb < len; pb += 2 * w) {
+            const int na = min(w, len - pb), nb = max(0, min(2 * w, len - pb) - w);
+            const double* ka = ks + pos + pb;
+            const int* ia = is + pos + pb;
+            const double* kb = ka + w;
+            const int* ib = ia + w;
+            for (int d0 = 0; d0 < na + nb; d0 += T) {
+                const int d1 = min(d0 + T, na + nb);
+                if (threadIdx.x < 2) {
+                    const int d = threadIdx.x ? d1 : d0;
+                    sp[threadIdx.x] = nb == 0 ? d : co_rank(d, ka, ia, na, kb, ib, nb);
+                }
+                __syncthreads();
+                const int a0 = sp[0], a1 = sp[1];
+                const int b0 = d0 - a0;
+                const int la = a1 - a0, lb = (d1 - a1) - b0;
+                for (int t = threadIdx.x; t < la + lb; t += NT) {
+                    if (t < la) { sk[lp(t)] = ka[a0 + t]; si[lp(t)] = ia[a0 + t]; }
+                    else        { sk[lp(t)] = kb[b0 + t - la]; si[lp(t)] = ib[b0 + t - la]; }
+                }
+                __syncthreads();
+                lds_merge<NT, T / NT>(sk, si, la, lb);
+                for (int t = threadIdx.x; t < la + lb; t += NT) {
+                    kd[pos + pb + d0 + t] = sk[lp(t)];
+                    id[pos + pb + d0 + t] = si[lp(t)];
+                }
+                __syncthreads();
+            }
+        }
+        double* tk = ks; ks = kd; kd = tk;
+        int* ti = is; is = id; id = ti;
+    }
+    if (ks != k0) {
+        for (int t = threadIdx.x; t < len; t += NT) {
+            k0[pos + t] = ks[pos + t];
+            i0[pos + t] = is[pos + t];
+        }
+        __syncthreads();
+    }
+}
+
+// Persistent blocks sort the runs of the run list in LDS, from the
+// scattered buffers (k1, i1) into the same positions of (k0, i0).  Three
+// blocks fit a CU's LDS; registers are held to six waves per SIMD to match.
+__global__ __launch_bounds__(ENM_SS_THREADS, 6)
+void ss_tile_kernel(const SsTile* __restrict__ tiles, const unsigned* __restrict__ tile_count,
+                    double* k1, int* i1, double* k0, int* i0) {
+    __shared__ double sk[ENM_LDS_PAD(ENM_SS_TILE)];
+    __shared__ int si[ENM_LDS_PAD(ENM_SS_TILE)];
+    __shared__ int sp[2];
+    const int total = (int)*tile_count;
+    for (int i = blockIdx.x; i < total; i += gridDim.x) {
+        const SsTile tl = tiles[i];
+        if (tl.len > ENM_SS_TILE) {
+            ss_sort_long(sk, si, sp, tl.pos, tl.len, k1, i1, k0, i0);
+            continue;
+        }
+        for (int t = threadIdx.x; t < ENM_SS_TILE; t += ENM_SS_THREADS) {
+            if (t < tl.len) {
+                sk[lp(t)] = k1[tl.pos + t];
+                si[lp(t)] = i1[tl.pos + t];
+            } else {
+                sk[lp(t)] = INFINITY;
+                si[lp(t)] = 0x7fffffff;
+            }
+        }
+        block_sort_lds<ENM_SS_TILE, ENM_SS_THREADS>(sk, si);
+        for (int t = threadIdx.x; t < tl.len; t += ENM_SS_THREADS) {
+            k0[tl.pos + t] = sk[lp(t)];
+            i0[tl.pos + t] = si[lp(t)];
+        }
+        __syncthreads();
+    }
+}
+
 __global__ void fill_u64_kernel(unsigned long long* __restrict__ p, int64_t n, unsigned long long v) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = v;
@@ -374,21 +771,21 @@ hipError_t enm_launch_fill_u64(unsigned long long* p, int64_t n, unsigned long l
 }
 
 hipError_t enm_launch_sort(const SortSeg* d_segs, int nsegs, int ntiles, int64_t maxlen, int tile,
-                           const double* arena, double* k0, int* i0, double* k1, int* i1, int* split,
-                           int* final_buf, int* n_launch, hipStream_t st) {
+                           const double* arena, const int* iarena, double* k0, int* i0, double* k1, int* i1,
+                           int* split, int* final_buf, int* n_launch, hipStream_t st) {
     *final_buf = 0;
     if (ntiles <= 0) return hipSuccess;
     // Segments no longer than the tile are sorted by the block sort alone;
     // longer ones (always with 2048-key tiles) continue with merge passes.
     if (tile == 512)
         hipLaunchKernelGGL(block_sort_kernel<512>, dim3(ntiles), dim3(ENM_SORT_THREADS), 0, st,
-                           d_segs, nsegs, arena, k0, i0);
+                           d_segs, nsegs, arena, iarena, k0, i0);
     else if (tile == 1024)
         hipLaunchKernelGGL(block_sort_kernel<1024>, dim3(ntiles), dim3(ENM_SORT_THREADS), 0, st,
-                           d_segs, nsegs, arena, k0, i0);
+                           d_segs, nsegs, arena, iarena, k0, i0);
     else if (tile == ENM_SORT_TILE)
         hipLaunchKernelGGL(block_sort_kernel<ENM_SORT_TILE>, dim3(ntiles), dim3(ENM_SORT_THREADS), 0, st,
-                           d_segs, nsegs, arena, k0, i0);
+                           d_segs, nsegs, arena, iarena, k0, i0);
     else
         return hipErrorInvalidValue;
     if (tile != ENM_SORT_TILE && maxlen > tile) return hipErrorInvalidValue;
@@ -412,6 +809,37 @@ hipError_t enm_launch_sort(const SortSeg* d_segs, int nsegs, int ntiles, int64_t
     }
     *final_buf = cur;
     return hipSuccess;
+}
+
+hipError_t enm_launch_sample_sort(const SampleSortArgs& a, const double* arena, double* k0, int* i0,
+                                  double* k1, int* i1, int* n_launch, hipStream_t st) {
+    if (a.nsegs <= 0) return hipSuccess;
+    // Samples go to the second sample buffers, sorted into the first (or
+    // back, as the merge passes end).
+    hipLaunchKernelGGL(ss_sample_kernel, dim3(a.nsegs), dim3(ENM_SORT_THREADS), 0, st,
+                       a.segs, a.ss, arena, a.samp_k1, a.samp_i1, a.tile_count);
+    ++*n_launch;
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    int fb = 0;
+    if (a.n_samp_segs > 0) {
+        e = enm_launch_sort(a.samp_segs, a.n_samp_segs, a.samp_tiles, a.samp_maxlen, ENM_SORT_TILE,
+                            a.samp_k1, a.samp_i1, a.samp_k0, a.samp_i0, a.samp_k1, a.samp_i1, a.samp_split,
+                            &fb, n_launch, st);
+        if (e != hipSuccess) return e;
+    }
+    const double* spk = fb ? a.samp_k1 : a.samp_k0;
+    const int* spi = fb ? a.samp_i1 : a.samp_i0;
+    hipLaunchKernelGGL(ss_count_kernel, dim3(a.ntiles), dim3(ENM_SORT_THREADS), 0, st,
+                       a.segs, a.nsegs, a.ss, arena, spk, spi, a.bid, a.counts);
+    hipLaunchKernelGGL(ss_scan_kernel, dim3(a.nsegs), dim3(ENM_SORT_THREADS), 0, st,
+                       a.segs, a.ss, a.counts, a.tiles, a.tile_count);
+    hipLaunchKernelGGL(ss_scatter_kernel, dim3(a.ntiles), dim3(ENM_SORT_THREADS), 0, st,
+                       a.segs, a.nsegs, a.ss, arena, a.bid, a.counts, k1, i1);
+    hipLaunchKernelGGL(ss_tile_kernel, dim3(a.tile_grid), dim3(ENM_SS_THREADS), 0, st,
+                       a.tiles, a.tile_count, k1, i1, k0, i0);
+    *n_launch += 4;
+    return hipGetLastError();
 }
 
 hipError_t enm_launch_strip_sort(const GatherItem* d_items, int nitems, const SortSeg* d_segs,

@@ -276,3 +276,79 @@ def test_large_ksg_against_kdtree():
     np.testing.assert_array_equal(c[0], ref["nx"])
     np.testing.assert_array_equal(c[1], ref["ny"])
     assert _close(v, w), (v, w)
+
+
+# ---------------------------------------------------------------------------
+# Sample sort of segments longer than one block-sort tile (2048 keys)
+# ---------------------------------------------------------------------------
+
+def _sample_positions(n):
+    """Positions the sample sort draws its splitter samples from in a
+    segment of n keys (the device's stride-and-hash rule)."""
+    nb = min(1024, -(-n // 512))
+    ns = 16 * nb if nb > 1 else 0
+    if ns == 0:
+        return np.zeros(0, dtype=np.int64)
+    step = n // ns
+    i = np.arange(ns, dtype=np.uint64)
+    h = (i * np.uint64(0x9E3779B1)) & np.uint64(0xFFFFFFFF)
+    h ^= h >> np.uint64(15)
+    h = (h * np.uint64(0x85EBCA77)) & np.uint64(0xFFFFFFFF)
+    h ^= h >> np.uint64(13)
+    return ((i * np.uint64(n)) // np.uint64(ns) + h % np.uint64(step)).astype(np.int64)
+
+
+def _ksg_vs_kdtree(x, y, k, flags=0):
+    nat = _nat()
+    v, eps, c = _batch_one(nat.KIND_KSG, k, [x], [y], flags=flags)
+    w, ref = orc.estimate_single_mi(x, y, k, backend="kdtree", detail=True)
+    np.testing.assert_array_equal(eps, ref["eps"])
+    np.testing.assert_array_equal(c[0], ref["nx"])
+    np.testing.assert_array_equal(c[1], ref["ny"])
+    assert _close(v, w), (v, w)
+
+
+@pytest.mark.parametrize("n", [2049, 4097, 9000, 30011])
+@pytest.mark.parametrize("rounding", [None, 1, 0])
+def test_sample_sorted_sizes_and_ties(n, rounding):
+    # Rounding to 0 decimals leaves a handful of distinct keys: the buckets
+    # are cut inside runs of equal values by the point index.
+    a = _data(7000 + n, n, 2, rounding)
+    _ksg_vs_kdtree(np.ascontiguousarray(a[:, 0]), np.ascontiguousarray(a[:, 1]), 3)
+
+
+def test_sample_sort_oversized_bucket():
+    # Every splitter sample holds a key above all others, so one bucket gets
+    # ~97 % of the segment (far above one LDS run): the block merge-sort
+    # fallback must give the same order.
+    n = 20000
+    rng = np.random.default_rng(99)
+    x = rng.normal(size=n)
+    pos = _sample_positions(n)
+    assert len(pos) == 16 * 40 and len(np.unique(pos)) == len(pos) and pos.max() < n
+    x[pos] = 1e3 + np.arange(len(pos))
+    y = rng.normal(size=n)
+    _ksg_vs_kdtree(x, y, 3)
+    # Both columns adversarial, FP64 mode.
+    y[pos] = -1e3 - np.arange(len(pos))
+    _ksg_vs_kdtree(x, y, 3, flags=_nat().F_FP64)
+
+
+def test_sample_sorted_class_segments():
+    # Ross MI with classes from 3 to 9000 points: class segments of every
+    # size sorted together (one bucket, several buckets, many runs).
+    from ennemi_amd import _estimators as est
+    nat = _nat()
+    sizes = [3, 700, 2049, 9000, 5000, 40]
+    rng = np.random.default_rng(5)
+    y = np.repeat(np.arange(len(sizes)), sizes)
+    rng.shuffle(y)
+    x = rng.normal(size=len(y)) + 0.3 * y
+    values, counts, order, grouped = est._grouped(y, [x])
+    v, eps, c = _batch_one(nat.KIND_ROSS, 3, grouped, classes=counts)
+    w, ref = orc.estimate_semidiscrete_mi(x, y, 3, detail=True)
+    inv = np.empty_like(order)
+    inv[order] = np.arange(len(y))
+    np.testing.assert_array_equal(eps[inv], ref["eps"])
+    np.testing.assert_array_equal(c[0][inv], ref["n_full"].astype(np.int64))
+    assert _close(v, w), (v, w)
