@@ -50,19 +50,26 @@ def log(*a):
 
 
 def time_batch(db, steps: int, warmup: int, stream):
-    """Run warmup + timed steps; return (ms/step wall, per-step stats list)."""
+    """Run warmup + timed steps; return (ms/step wall, per-step stats list).
+
+    The timed steps are enqueued back to back (the stream orders them, so
+    the host's launch work overlaps the device's), between two device
+    synchronisations.  The per-stage times come from a second pass of the
+    same number of steps, each read back after its own synchronisation."""
     import torch
     for _ in range(warmup):
         db.launch()
     stream.synchronize()
-    stats = []
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(steps):
         db.launch()
-        stats.append(db.stats())  # synchronises the stream, reads the stage events
     torch.cuda.synchronize()
     wall = (time.perf_counter() - t0) * 1e3 / steps
+    stats = []
+    for _ in range(steps):
+        db.launch()
+        stats.append(db.stats())  # synchronises the stream, reads the stage events
     return wall, stats
 
 
@@ -217,21 +224,25 @@ def main() -> int:
                 "pair_evaluations": st["knn_pairs"],
                 "pair_rate_T_per_s": st["knn_pairs"] / (stage_mean["knn"] * 1e-3) / 1e12}
     # Every stage against its own roof (algorithmic bytes / stage time).
-    # sort: two orders per instance (x, y); segments of 2049 .. 2^20 keys are
-    # sample-sorted at 56 B per key (bucket count: key read, bucket id
+    # sort: two orders per instance (x, y).  Segments of 2049 .. 2^20 keys
+    # are sample-sorted at 44 B per key (bucket count: key read, bucket id
     # written; scatter: key and bucket id read, key and index written; run
-    # sort: key and index read and written), others take a block pass plus
-    # merge passes at 24 B per key each; gather: the two orders' columns
-    # laid out (64 B/point), the strip sort (24 B) and the grid order's copy
-    # (44 B); count: 20 B per 1-D query.
+    # sort: key and index read) and laid out into the orders by the same
+    # kernel (64 B per point for the two orders' columns and permutations);
+    # otherwise a block pass and merge passes at 24 B per key each, then the
+    # gather (64 B per point).  gather: the grid order's strip sort (24 B)
+    # and copy (44 B); count: 20 B per 1-D query.
     longest = int(ns.max())
     if 2048 < longest <= 1 << 20:
-        sort_bytes_per_key = 56.0
+        sort_bytes = 2.0 * n_points * 44 + n_points * 64.0
+        gather_bytes = n_points * (24.0 + 44)
     else:
-        sort_bytes_per_key = 24.0 * (1 + max(0, int(np.ceil(np.log2(max(longest, 2048) / 2048)))))
+        passes = 1 + max(0, int(np.ceil(np.log2(max(longest, 2048) / 2048))))
+        sort_bytes = 2.0 * n_points * 24 * passes
+        gather_bytes = n_points * (64.0 + 24 + 44)
     stage_bytes = {
-        "sort": 2.0 * n_points * sort_bytes_per_key,
-        "gather": n_points * (64.0 + 24 + 44),
+        "sort": sort_bytes,
+        "gather": gather_bytes,
         "knn": float(knn_bytes),
         "count": st["count_queries_1d"] * 20.0,
     }

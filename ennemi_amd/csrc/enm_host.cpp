@@ -25,6 +25,7 @@
 #include <list>
 #include <map>
 #include <memory>
+#include <set>
 #include <string>
 #include <tuple>
 #include <vector>
@@ -106,6 +107,8 @@ struct Plan {
     bool sample_sort = false;
     std::vector<SampleSeg> ssegs;    // per first-stage segment
     std::vector<SortSeg> samp_segs;  // sample ranges to sort
+    std::vector<SsCons> cons;        // orders laid out from each segment
+    std::vector<int> qbase;          // run queue offsets (ENM_NQ + 1)
     int samp_tiles = 0;
     int64_t samp_maxlen = 0, n_samp = 0, n_counts = 0, n_buckets = 0;
     int64_t maxlen = 0, maxlen2 = 0;
@@ -124,7 +127,8 @@ struct Plan {
         deferred, samp_k0, samp_k1, samp_i0, samp_i1, samp_split, bid, ss_counts, ss_tiles, ss_ctl;
     size_t off_segs = 0, off_gitems = 0, off_orders = 0, off_kdescs = 0, off_cdescs = 0, off_insts = 0,
            off_cls = 0, off_kitems = 0, off_citems = 0, off_segs2 = 0, off_gitems2 = 0, off_rdescs = 0,
-           off_ritems = 0, off_sitems = 0, off_qoffs = 0, off_ssegs = 0, off_samp_segs = 0, table_bytes = 0;
+           off_ritems = 0, off_sitems = 0, off_qoffs = 0, off_ssegs = 0, off_samp_segs = 0, off_cons = 0,
+           off_qbase = 0, table_bytes = 0;
     void* pinned = nullptr;
     hipEvent_t upload_done = nullptr;
     hipGraphExec_t graph = nullptr;
@@ -678,9 +682,30 @@ int plan_batch(enm_ctx* ctx, Plan& P, const enm_instance* inst, int n_inst, cons
     // tile and none is longer than ENM_SS_MAXLEN.
     P.sample_sort = P.maxlen > ENM_SORT_TILE && P.maxlen <= ENM_SS_MAXLEN;
     if (P.sample_sort) {
-        for (const SortSeg& sg : P.segs) {
+        // The orders each segment is laid out into (the first-stage gather
+        // items, one per order and segment), and the run queue of the
+        // segment: by instance, as the gather items are interleaved.
+        std::vector<std::vector<SsCons>> seg_cons(P.segs.size());
+        std::set<std::pair<int, int>> seen;
+        for (const GatherItem& gi : P.gitems) {
+            if (!seen.insert({gi.sseg, gi.order}).second) continue;
+            SsCons c;
+            c.pos0 = gi.pos0;
+            c.order = gi.order;
+            c.pad = 0;
+            seg_cons[gi.sseg].push_back(c);
+        }
+        std::vector<int64_t> qload(ENM_NQ, 0);
+        for (size_t si = 0; si < P.segs.size(); ++si) {
+            const SortSeg& sg = P.segs[si];
             SampleSeg q;
             std::memset(&q, 0, sizeof(q));
+            std::sort(seg_cons[si].begin(), seg_cons[si].end(),
+                      [](const SsCons& a, const SsCons& b) { return a.order < b.order; });
+            q.cons0 = (int32_t)P.cons.size();
+            q.ncons = (int32_t)seg_cons[si].size();
+            P.cons.insert(P.cons.end(), seg_cons[si].begin(), seg_cons[si].end());
+            q.q = seg_cons[si].empty() ? 0 : P.orders[seg_cons[si][0].order].inst % ENM_NQ;
             q.nb = (int32_t)std::min<int64_t>(ENM_SS_MAXB, (sg.len + ENM_SS_BUCKET - 1) / ENM_SS_BUCKET);
             q.ns = q.nb > 1 ? ENM_SS_OVER * q.nb : 0;   // < len: nb > 1 means len > ENM_SS_BUCKET
             q.samp = (int32_t)P.n_samp;
@@ -700,8 +725,11 @@ int plan_batch(enm_ctx* ctx, Plan& P, const enm_instance* inst, int n_inst, cons
             P.n_samp += q.ns;
             P.n_counts += (int64_t)((sg.len + ENM_SORT_TILE - 1) / ENM_SORT_TILE) * q.nb;
             P.n_buckets += q.nb;
+            qload[q.q] += q.nb;   // a queue holds at most its segments' buckets
             P.ssegs.push_back(q);
         }
+        P.qbase.assign(ENM_NQ + 1, 0);
+        for (int q = 0; q < ENM_NQ; ++q) P.qbase[q + 1] = P.qbase[q] + (int)qload[q];
     }
     // psi(0..max n) for the count kernels.
     P.psi_tab = P.n_res64;
@@ -740,6 +768,8 @@ int allocate_and_upload(enm_ctx* ctx, Plan& P, hipStream_t st) {
     P.off_qoffs = append(blob, P.qoffs);
     P.off_ssegs = append(blob, P.ssegs);
     P.off_samp_segs = append(blob, P.samp_segs);
+    P.off_cons = append(blob, P.cons);
+    P.off_qbase = append(blob, P.qbase);
     P.table_bytes = blob.size();
     hipError_t e;
     if (P.upload_done) {
@@ -785,7 +815,7 @@ int allocate_and_upload(enm_ctx* ctx, Plan& P, hipStream_t st) {
         if ((rc = need(P.bid, P.sort_elems * 2 + 2, "buckets"))) return rc;
         if ((rc = need(P.ss_counts, P.n_counts * 4 + 4, "bucket counts"))) return rc;
         if ((rc = need(P.ss_tiles, P.n_buckets * sizeof(SsTile), "bucket runs"))) return rc;
-        if ((rc = need(P.ss_ctl, 256, "run counter"))) return rc;
+        if ((rc = need(P.ss_ctl, 2 * ENM_NQ * ENM_CSTRIDE * 4, "run queue counters"))) return rc;
     }
     e = hipMemcpyAsync(P.tables.p, P.pinned, P.table_bytes, hipMemcpyHostToDevice, st);
     if (e != hipSuccess) return hip_fail(ctx, e, "hipMemcpyAsync(tables)");
@@ -854,9 +884,17 @@ int enqueue(enm_ctx* ctx, Plan& P, const double* arena, double* out, double* d_e
         a.bid = P.bid.as<unsigned short>();
         a.counts = P.ss_counts.as<int>();
         a.tiles = P.ss_tiles.as<SsTile>();
-        a.tile_count = P.ss_ctl.as<unsigned>();
-        // ss_tile_kernel holds 48 KB of LDS: three blocks per CU.
+        a.qbase = tab<int>(P, P.off_qbase);
+        a.qctl = P.ss_ctl.as<unsigned>();
+        // ss_tile_kernel holds 50 KB of LDS: three blocks per CU.
         a.tile_grid = std::max(1, ctx->num_cus * 3);
+        a.cons = tab<SsCons>(P, P.off_cons);
+        a.orders = tab<OrderDev>(P, P.off_orders);
+        a.col64 = P.col64.as<double>();
+        a.col32 = P.col32.as<float>();
+        a.perm = P.perm.as<int>();
+        a.maxabs = P.maxabs.as<unsigned long long>();
+        a.with32 = P.f32 && P.need32 ? 1 : 0;
         if ((rc = chk(enm_launch_sample_sort(a, arena, P.sk0.as<double>(), P.si0.as<int>(), P.sk1.as<double>(),
                                              P.si1.as<int>(), &nl, st), "sort")))
             return rc;
@@ -869,7 +907,8 @@ int enqueue(enm_ctx* ctx, Plan& P, const double* arena, double* out, double* d_e
     if ((rc = mark(2))) return rc;
     const double* skey = fb ? P.sk1.as<double>() : P.sk0.as<double>();
     const int* sidx = fb ? P.si1.as<int>() : P.si0.as<int>();
-    if ((rc = chk(enm_launch_gather(tab<GatherItem>(P, P.off_gitems), (int)P.gitems.size(),
+    // (The sample sort lays its segments out into the orders itself.)
+    if (!P.sample_sort && (rc = chk(enm_launch_gather(tab<GatherItem>(P, P.off_gitems), (int)P.gitems.size(),
                                     tab<SortSeg>(P, P.off_segs), tab<OrderDev>(P, P.off_orders), arena, skey,
                                     sidx, P.col64.as<double>(), P.col32.as<float>(), P.perm.as<int>(),
                                     P.maxabs.as<unsigned long long>(), P.f32 && P.need32 ? 1 : 0, &nl, st),

@@ -64,14 +64,25 @@ struct SampleSeg {
     int32_t samp;   // offset of its samples in the sample buffers
     int32_t ns;     // number of samples: ENM_SS_OVER * nb, or 0 when nb == 1
     int32_t nb;     // number of buckets
+    int32_t q;      // run queue (one per XCD, by instance)
+    int32_t cons0;  // its consumers: SsCons entries [cons0, cons0 + ncons)
+    int32_t ncons;
+};
+
+// An order laid out from a sorted segment: the segment's sorted element t
+// goes to order position pos0 + t.
+struct SsCons {
+    int64_t pos0;
+    int32_t order;
     int32_t pad;
 };
 
-// One run of whole buckets to sort: sort-buffer positions [pos, pos + len).
+// One run of whole buckets to sort: sort-buffer positions [pos, pos + len)
+// of first-stage segment seg.
 struct SsTile {
     int64_t pos;
     int32_t len;
-    int32_t pad;
+    int32_t seg;
 };
 
 // Gather work: one ENM_SORT_TILE chunk of one sort segment into one order.

@@ -13,8 +13,10 @@ hipError_t enm_launch_sort(const SortSeg* d_segs, int nsegs, int ntiles, int64_t
                            const double* arena, const int* iarena, double* k0, int* i0, double* k1, int* i1,
                            int* split, int* final_buf, int* n_launch, hipStream_t st);
 
-// Sample sort of the first-stage segments (all at most ENM_SS_MAXLEN keys)
-// into (k0, i0); (k1, i1) hold the scattered buckets on the way.
+// Sample sort of the first-stage segments (all at most ENM_SS_MAXLEN keys),
+// laid out straight into the orders that consume them (the first-stage
+// gather); (k1, i1) hold the scattered buckets on the way, (k0, i0) the
+// result of a run too large for LDS.
 struct SampleSortArgs {
     const SortSeg* segs;
     int nsegs, ntiles;
@@ -26,9 +28,17 @@ struct SampleSortArgs {
     int *samp_i0, *samp_i1, *samp_split;
     unsigned short* bid;        // bucket of every key, by sort-buffer position
     int* counts;                // (chunk, bucket) counts, then scatter offsets
-    SsTile* tiles;              // runs of buckets (at most the total bucket count)
-    unsigned* tile_count;
+    SsTile* tiles;              // run queues (queue q from qbase[q]; at most its buckets)
+    const int* qbase;           // ENM_NQ + 1 queue offsets (device)
+    unsigned* qctl;             // 2 * ENM_NQ counters, ENM_CSTRIDE apart: appends, pops
     int tile_grid;
+    const SsCons* cons;         // consumers of each segment (SampleSeg::cons0, ncons)
+    const OrderDev* orders;
+    double* col64;
+    float* col32;
+    int* perm;
+    unsigned long long* maxabs;
+    int with32;
 };
 hipError_t enm_launch_sample_sort(const SampleSortArgs& a, const double* arena, double* k0, int* i0,
                                   double* k1, int* i1, int* n_launch, hipStream_t st);

@@ -477,12 +477,13 @@ __device__ __forceinline__ int64_t ss_sample_pos(int i, int ns, int64_t len) {
 }
 
 // One block per segment: copy its samples (key, point index) out.  Block 0
-// also clears the run counter that ss_scan_kernel fills.
+// also clears the run-queue counters (appends, then pops: 2 * ENM_NQ, each
+// on its own line) that ss_scan_kernel and ss_tile_kernel use.
 __global__ __launch_bounds__(ENM_SORT_THREADS)
 void ss_sample_kernel(const SortSeg* __restrict__ segs, const SampleSeg* __restrict__ ss,
                       const double* __restrict__ arena, double* __restrict__ sk, int* __restrict__ si,
-                      unsigned* __restrict__ tile_count) {
-    if (blockIdx.x == 0 && threadIdx.x == 0) *tile_count = 0u;
+                      unsigned* __restrict__ qctl) {
+    if (blockIdx.x == 0 && threadIdx.x < 2 * ENM_NQ) qctl[threadIdx.x * ENM_CSTRIDE] = 0u;
     const SortSeg sg = segs[blockIdx.x];
     const SampleSeg q = ss[blockIdx.x];
     for (int i = threadIdx.x; i < q.ns; i += ENM_SORT_THREADS) {
@@ -536,10 +537,11 @@ void ss_count_kernel(const SortSeg* __restrict__ segs, int nsegs, const SampleSe
 // positions (bucket-major, chunk-minor: the first position of chunk c's keys
 // of bucket b), and pack consecutive buckets greedily into runs of at most
 // ENM_SS_TILE keys (a larger bucket is a run of its own), appended to the
-// run list.
+// segment's run queue (queue q's runs start at tiles[qbase[q]]).
 __global__ __launch_bounds__(ENM_SORT_THREADS)
 void ss_scan_kernel(const SortSeg* __restrict__ segs, const SampleSeg* __restrict__ ss,
-                    int* __restrict__ counts, SsTile* __restrict__ tiles, unsigned* __restrict__ tile_count) {
+                    int* __restrict__ counts, SsTile* __restrict__ tiles, const int* __restrict__ qbase,
+                    unsigned* __restrict__ qctl) {
     __shared__ int tot[ENM_SS_MAXB];
     __shared__ int start[ENM_SS_MAXB];
     __shared__ int part[ENM_SORT_THREADS / ENM_WAVE];
@@ -572,19 +574,19 @@ void ss_scan_kernel(const SortSeg* __restrict__ segs, const SampleSeg* __restric
             cur += z;
         }
         if (cur > 0) ++nt;
-        const unsigned base = atomicAdd(tile_count, (unsigned)nt);
+        SsTile* out = tiles + qbase[q.q] + atomicAdd(qctl + q.q * ENM_CSTRIDE, (unsigned)nt);
         int k = 0, t0 = 0;
         cur = 0;
         for (int b = 0; b < nb; ++b) {
             const int z = tot[b];
             if (cur > 0 && cur + z > ENM_SS_TILE) {
-                tiles[base + k++] = SsTile{sg.sbase + t0, cur, 0};
+                out[k++] = SsTile{sg.sbase + t0, cur, (int)blockIdx.x};
                 t0 += cur;
                 cur = 0;
             }
             cur += z;
         }
-        if (cur > 0) tiles[base + k] = SsTile{sg.sbase + t0, cur, 0};
+        if (cur > 0) out[k] = SsTile{sg.sbase + t0, cur, (int)blockIdx.x};
     }
 }
 
@@ -721,37 +723,109 @@ __device__ void ss_sort_long(double* sk, int* si, int* sp, int64_t pos, int len,
     }
 }
 
-// Persistent blocks sort the runs of the run list in LDS, from the
-// scattered buffers (k1, i1) into the same positions of (k0, i0).  Three
-// blocks fit a CU's LDS; registers are held to six waves per SIMD to match.
+// Lay a sorted run out into every order that consumes its segment: the
+// point index, the key column and the order's other columns (read from the
+// arena at the point, FP32 copies when with32), and fold the magnitudes into
+// the instance's maximum (for the FP32 guard band).  Sorted element t of the
+// run is at LDS slot lp(t) (LDS) or at kg[t], ig[t] (global).
+struct SsLayout {
+    const SortSeg* segs;
+    const SampleSeg* ss;
+    const SsCons* cons;
+    const OrderDev* orders;
+    const double* arena;
+    double* col64;
+    float* col32;
+    int* perm;
+    unsigned long long* maxabs;
+    int with32;
+};
+
+template <bool LDS>
+__device__ void ss_layout_run(const SsLayout& L, const SsTile& tl, const double* kk, const int* ii,
+                              double* red) {
+    const SortSeg& sg = L.segs[tl.seg];
+    const SampleSeg& q = L.ss[tl.seg];
+    const int64_t r0 = tl.pos - sg.sbase;
+    for (int c = 0; c < q.ncons; ++c) {
+        const SsCons cn = L.cons[q.cons0 + c];
+        // Fields read one by one: a local copy of the descriptor, indexed by
+        // a loop counter, would live in scratch memory.
+        const OrderDev& od = L.orders[cn.order];
+        const int64_t n = od.n;
+        const int ncol = od.ncol;
+        double m = 0.0;
+        for (int t = threadIdx.x; t < tl.len; t += ENM_SS_THREADS) {
+            const double key = LDS ? kk[lp(t)] : kk[t];
+            const int idx = LDS ? ii[lp(t)] : ii[t];
+            const int64_t p = cn.pos0 + r0 + t;
+            L.perm[od.perm + p] = idx;
+            L.col64[od.c64 + p] = key;
+            if (L.with32) L.col32[od.c32 + p] = (float)key;
+            m = fmax(m, fabs(key));
+            for (int cc = 1; cc < ncol; ++cc) {
+                const double v = L.arena[od.src[cc] + idx];
+                L.col64[od.c64 + cc * n + p] = v;
+                if (L.with32) L.col32[od.c32 + cc * n + p] = (float)v;
+                m = fmax(m, fabs(v));
+            }
+        }
+        m = wave_max(m);
+        if ((threadIdx.x & (ENM_WAVE - 1)) == 0) red[threadIdx.x / ENM_WAVE] = m;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            double mm = red[0];
+            for (int w = 1; w < ENM_SS_THREADS / ENM_WAVE; ++w) mm = fmax(mm, red[w]);
+            // |v| >= 0, so the IEEE bit patterns order like the values.
+            atomicMax(L.maxabs + od.inst, (unsigned long long)__double_as_longlong(mm));
+        }
+        __syncthreads();
+    }
+}
+
+// Persistent blocks sort the runs in LDS, from the scattered buffers (k1,
+// i1), and lay them out into the orders.  Runs sit in ENM_NQ queues by
+// instance; block b starts on queue b mod ENM_NQ (its XCD, so one
+// instance's random column reads stay in one L2) and takes other queues'
+// runs when its own is empty.  Three blocks fit a CU's LDS; registers are
+// held to six waves per SIMD to match.
 __global__ __launch_bounds__(ENM_SS_THREADS, 6)
-void ss_tile_kernel(const SsTile* __restrict__ tiles, const unsigned* __restrict__ tile_count,
-                    double* k1, int* i1, double* k0, int* i0) {
+void ss_tile_kernel(const SsTile* __restrict__ tiles, const int* __restrict__ qbase, unsigned* qctl,
+                    double* k1, int* i1, double* k0, int* i0, SsLayout L) {
     __shared__ double sk[ENM_LDS_PAD(ENM_SS_TILE)];
     __shared__ int si[ENM_LDS_PAD(ENM_SS_TILE)];
     __shared__ int sp[2];
-    const int total = (int)*tile_count;
-    for (int i = blockIdx.x; i < total; i += gridDim.x) {
-        const SsTile tl = tiles[i];
-        if (tl.len > ENM_SS_TILE) {
-            ss_sort_long(sk, si, sp, tl.pos, tl.len, k1, i1, k0, i0);
-            continue;
-        }
-        for (int t = threadIdx.x; t < ENM_SS_TILE; t += ENM_SS_THREADS) {
-            if (t < tl.len) {
-                sk[lp(t)] = k1[tl.pos + t];
-                si[lp(t)] = i1[tl.pos + t];
-            } else {
-                sk[lp(t)] = INFINITY;
-                si[lp(t)] = 0x7fffffff;
+    __shared__ unsigned s_ticket;
+    __shared__ double red[ENM_SS_THREADS / ENM_WAVE];
+    const int home = (int)(blockIdx.x % ENM_NQ);
+    for (int r = 0; r < ENM_NQ; ++r) {
+        const int qq = (home + r) % ENM_NQ;
+        // Appends finished with the previous kernel: the count is final.
+        const unsigned qn = qctl[qq * ENM_CSTRIDE];
+        for (;;) {
+            if (threadIdx.x == 0) s_ticket = atomicAdd(qctl + (ENM_NQ + qq) * ENM_CSTRIDE, 1u);
+            __syncthreads();
+            const unsigned t = (unsigned)__builtin_amdgcn_readfirstlane((int)s_ticket);
+            __syncthreads();
+            if (t >= qn) break;
+            const SsTile tl = tiles[qbase[qq] + (int)t];
+            if (tl.len > ENM_SS_TILE) {
+                ss_sort_long(sk, si, sp, tl.pos, tl.len, k1, i1, k0, i0);
+                ss_layout_run<false>(L, tl, k0 + tl.pos, i0 + tl.pos, red);
+                continue;
             }
+            for (int u = threadIdx.x; u < ENM_SS_TILE; u += ENM_SS_THREADS) {
+                if (u < tl.len) {
+                    sk[lp(u)] = k1[tl.pos + u];
+                    si[lp(u)] = i1[tl.pos + u];
+                } else {
+                    sk[lp(u)] = INFINITY;
+                    si[lp(u)] = 0x7fffffff;
+                }
+            }
+            block_sort_lds<ENM_SS_TILE, ENM_SS_THREADS>(sk, si);
+            ss_layout_run<true>(L, tl, sk, si, red);
         }
-        block_sort_lds<ENM_SS_TILE, ENM_SS_THREADS>(sk, si);
-        for (int t = threadIdx.x; t < tl.len; t += ENM_SS_THREADS) {
-            k0[tl.pos + t] = sk[lp(t)];
-            i0[tl.pos + t] = si[lp(t)];
-        }
-        __syncthreads();
     }
 }
 
@@ -814,10 +888,11 @@ hipError_t enm_launch_sort(const SortSeg* d_segs, int nsegs, int ntiles, int64_t
 hipError_t enm_launch_sample_sort(const SampleSortArgs& a, const double* arena, double* k0, int* i0,
                                   double* k1, int* i1, int* n_launch, hipStream_t st) {
     if (a.nsegs <= 0) return hipSuccess;
+    if (a.qctl == nullptr || a.qbase == nullptr) return hipErrorInvalidValue;
     // Samples go to the second sample buffers, sorted into the first (or
     // back, as the merge passes end).
     hipLaunchKernelGGL(ss_sample_kernel, dim3(a.nsegs), dim3(ENM_SORT_THREADS), 0, st,
-                       a.segs, a.ss, arena, a.samp_k1, a.samp_i1, a.tile_count);
+                       a.segs, a.ss, arena, a.samp_k1, a.samp_i1, a.qctl);
     ++*n_launch;
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
@@ -833,11 +908,22 @@ hipError_t enm_launch_sample_sort(const SampleSortArgs& a, const double* arena, 
     hipLaunchKernelGGL(ss_count_kernel, dim3(a.ntiles), dim3(ENM_SORT_THREADS), 0, st,
                        a.segs, a.nsegs, a.ss, arena, spk, spi, a.bid, a.counts);
     hipLaunchKernelGGL(ss_scan_kernel, dim3(a.nsegs), dim3(ENM_SORT_THREADS), 0, st,
-                       a.segs, a.ss, a.counts, a.tiles, a.tile_count);
+                       a.segs, a.ss, a.counts, a.tiles, a.qbase, a.qctl);
     hipLaunchKernelGGL(ss_scatter_kernel, dim3(a.ntiles), dim3(ENM_SORT_THREADS), 0, st,
                        a.segs, a.nsegs, a.ss, arena, a.bid, a.counts, k1, i1);
+    SsLayout L;
+    L.segs = a.segs;
+    L.ss = a.ss;
+    L.cons = a.cons;
+    L.orders = a.orders;
+    L.arena = arena;
+    L.col64 = a.col64;
+    L.col32 = a.col32;
+    L.perm = a.perm;
+    L.maxabs = a.maxabs;
+    L.with32 = a.with32;
     hipLaunchKernelGGL(ss_tile_kernel, dim3(a.tile_grid), dim3(ENM_SS_THREADS), 0, st,
-                       a.tiles, a.tile_count, k1, i1, k0, i0);
+                       a.tiles, a.qbase, a.qctl, k1, i1, k0, i0, L);
     *n_launch += 4;
     return hipGetLastError();
 }
