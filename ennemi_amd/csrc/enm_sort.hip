@@ -523,7 +523,10 @@ void ss_count_kernel(const SortSeg* __restrict__ segs, int nsegs, const SampleSe
         int lo = 0, hi = nsp;
         while (lo < hi) {
             const int mid = (lo + hi) >> 1;
-            if (kless(key, idx, spk[mid], spi[mid])) hi = mid; else lo = mid + 1;
+            const double sv = spk[mid];
+            bool lt = key < sv;
+            if (key == sv) lt = idx < spi[mid];   // indices read only on ties
+            if (lt) hi = mid; else lo = mid + 1;
         }
         bid[sg.sbase + off + t] = (unsigned short)lo;
         atomicAdd(&hist[lo], 1);
