@@ -80,28 +80,11 @@ __device__ __forceinline__ void finish_item(const CountDev& cd, const Item& it, 
     }
 }
 
-// Boundary search inside an LDS copy w[0, nw) of the window: last position
-// in [i, nw) with fl(w[j] - x) <= r (right) or first in [0, i] with
-// fl(x - w[j]) <= r (left); position i holds x itself.  fl(v - x) is monotone
-// in v, so the predicate holds on a prefix (right) or suffix (left).
-__device__ __forceinline__ int lds_right(const double* w, int i, int nw, double x, double r) {
-    int good = i, bad = nw;
-    while (bad - good > 1) {
-        const int mid = (good + bad) >> 1;
-        if (w[mid] - x <= r) good = mid; else bad = mid;
-    }
-    return good;
-}
-__device__ __forceinline__ int lds_left(const double* w, int i, double x, double r) {
-    int good = i, bad = -1;
-    while (good - bad > 1) {
-        const int mid = (good + bad) >> 1;
-        if (x - w[mid] <= r) good = mid; else bad = mid;
-    }
-    return good;
-}
-
-// The same searches over the whole scope range [lo, hi) in global memory.
+// Boundary searches over the whole scope range [lo, hi) in global memory:
+// last position in [i, hi) with fl(v[j] - x) <= r (right) or first in
+// [lo, i] with fl(x - v[j]) <= r (left); position i holds x itself.
+// fl(v - x) is monotone in v, so the predicate holds on a prefix (right)
+// or suffix (left).
 __device__ __forceinline__ int global_right(const double* __restrict__ v, int i, int hi, double x, double r) {
     int good = i, bad = hi;
     while (bad - good > 1) {
@@ -166,21 +149,42 @@ void count1d_kernel(CountArgs A, const double* __restrict__ col64, const double*
             const int i = it.t0 + tid + ENM_C1_THREADS * q;
             rq[q] = i < it.t1 ? query_radius(cd, A, res64, i) : -1.0;
         }
+        // Both boundaries of all ENM_C1_Q queries are searched in lockstep
+        // (fixed halving steps from the query's own position, 2048 down to
+        // 1, which spans the whole window): 2 * ENM_C1_Q independent LDS
+        // reads per step instead of one dependent chain per search.
+        double xq[ENM_C1_Q];
+        int pr[ENM_C1_Q], pl[ENM_C1_Q];
+        bool live[ENM_C1_Q], dq[ENM_C1_Q];
+#pragma unroll
+        for (int q = 0; q < ENM_C1_Q; ++q) {
+            const int i = it.t0 + tid + ENM_C1_THREADS * q;
+            const double r = rq[q];
+            const int p = i < it.t1 ? i - w0 : 0;
+            const double x = win[p];
+            xq[q] = x;
+            pr[q] = p;
+            pl[q] = p;
+            live[q] = i < it.t1 && !(r < 0.0);
+            dq[q] = live[q] && ((w1 < it.hi && whi - x <= r) || (w0 > it.lo && x - wlo <= r));
+        }
+        for (int step = ENM_C1_CAP / 2; step >= 1; step >>= 1) {
+#pragma unroll
+            for (int q = 0; q < ENM_C1_Q; ++q) {
+                const int jr = pr[q] + step, jl = pl[q] - step;
+                const double vr = win[min(jr, nw - 1)], vl = win[max(jl, 0)];
+                if (jr < nw && vr - xq[q] <= rq[q]) pr[q] = jr;
+                if (jl >= 0 && xq[q] - vl <= rq[q]) pl[q] = jl;
+            }
+        }
 #pragma unroll
         for (int q = 0; q < ENM_C1_Q; ++q) {
             const int i = it.t0 + tid + ENM_C1_THREADS * q;
             const bool active = i < it.t1;
-            int cnt = 0;
-            bool defer = false;
             const double r = rq[q];
-            if (active) {
-                if (!(r < 0.0)) {
-                    const int p = i - w0;
-                    const double x = win[p];
-                    defer = (w1 < it.hi && whi - x <= r) || (w0 > it.lo && x - wlo <= r);
-                    if (!defer) cnt = lds_right(win, p, nw, x, r) - lds_left(win, p, x, r) + 1;
-                }
-            }
+            bool defer = dq[q];
+            const int cnt0 = live[q] && !defer ? pr[q] - pl[q] + 1 : 0;
+            int cnt = cnt0;
             const unsigned long long m = __builtin_amdgcn_ballot_w64(defer);
             if (m != 0) {
                 // One atomic per wave: every lane adds, lane 0 the wave's
