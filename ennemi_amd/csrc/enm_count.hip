@@ -245,7 +245,21 @@ void count1d_deferred_kernel(CountArgs A, const double* __restrict__ col64, cons
         const int i = it.t0 + (int)(code & ((1u << ENM_C1_OFF_BITS) - 1));
         const double r = query_radius(cd, A, res64, i);
         const double x = v[i];
-        res32[cd.counts + i] = global_right(v, i, it.hi, x, r) - global_left(v, it.lo, i, x, r) + 1;
+        // Both boundaries in lockstep by halving steps from i (the steps add
+        // up to at least the farther end of the scope range): two
+        // independent reads per step.  A deferred query has r >= 0, so the
+        // predicate holds at i itself.
+        const int span = max(it.hi - 1 - i, i - it.lo);
+        int step = 1;
+        while (2 * step <= span) step *= 2;
+        int pr = i, pl = i;
+        for (; step >= 1 && span > 0; step >>= 1) {
+            const int jr = pr + step, jl = pl - step;
+            const double vr = v[min(jr, it.hi - 1)], vl = v[max(jl, it.lo)];
+            if (jr < it.hi && vr - x <= r) pr = jr;
+            if (jl >= it.lo && x - vl <= r) pl = jl;
+        }
+        res32[cd.counts + i] = pr - pl + 1;
     }
 }
 

@@ -119,6 +119,7 @@ struct Plan {
     unsigned deferred_seg = 0;        // capacity of one deferred 1-D query list segment
     int n_inst = 0;
     bool f32 = true, brute = false, per_instance = false;
+    bool want_eps = false;   // the call exports radii (in point order)
     // Kernels that read the FP32 column copies: the brute-force kNN and
     // multi-D counts off the grid (band filter).  Only then are they made.
     bool need32 = false;
@@ -463,7 +464,13 @@ int plan_batch(enm_ctx* ctx, Plan& P, const enm_instance* inst, int n_inst, cons
                 space_strip[t] = S;
             }
         }
-        for (auto& s : spaces) if (s.order != K) need_orig = true;
+        // The radius in kNN order is kept for spaces on that order and for
+        // the export; other spaces read it through the point numbering.
+        bool need_k = P.want_eps;
+        for (auto& s : spaces) {
+            if (s.order != K) need_orig = true;
+            else need_k = true;
+        }
         // Entropy sums log radii in point order, whatever the kNN order.
         if (I.kind == ENM_KIND_KL) need_orig = true;
 
@@ -476,8 +483,8 @@ int plan_batch(enm_ctx* ctx, Plan& P, const enm_instance* inst, int n_inst, cons
         kd.base_key = P.orders[A].c64;
         kd.k = I.k;
         kd.kl = I.kind == ENM_KIND_KL;
-        kd.eps = P.n_res64;
-        P.n_res64 += I.n;
+        kd.eps = -1;
+        if (need_k) { kd.eps = P.n_res64; P.n_res64 += I.n; }
         kd.eps_orig = -1;
         if (need_orig) { kd.eps_orig = P.n_res64; P.n_res64 += I.n; }
         kd.part = -1;
@@ -1145,6 +1152,7 @@ int enm_run_batch(enm_ctx* ctx, const enm_instance* inst, int32_t n_inst, const 
     P.brute = (flags & ENM_F_BRUTE) != 0;
     P.need32 = P.brute;
     P.per_instance = (flags & ENM_F_PER_INSTANCE) != 0;
+    P.want_eps = d_eps != nullptr;
     int rc = plan_batch(ctx, P, inst, n_inst, seg_len);
     if (rc) return rc;
     // Evict before allocating so a cache full of large plans frees memory first.

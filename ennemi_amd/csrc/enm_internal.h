@@ -120,7 +120,7 @@ struct KnnDev {
     int32_t inst;
     int32_t k;
     int32_t kl;         // 1: an entropy instance (radii are summed as logs)
-    int64_t eps;        // res64 offset: radius per order position
+    int64_t eps;        // res64 offset: radius per order position (or -1)
     int64_t eps_orig;   // res64 offset: radius per instance point (or -1)
     int64_t part;       // unused (-1)
     int64_t base_key;   // col64 offset of the base order's sorted column 0

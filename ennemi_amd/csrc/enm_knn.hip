@@ -512,7 +512,7 @@ __device__ void knn_item(const KnnArgs& A, const Item& it, const double* __restr
     }
     const double eps = w.L[P - 1];
     if (active) {
-        res64[kd.eps + i] = eps;
+        if (kd.eps >= 0) res64[kd.eps + i] = eps;
         if (kd.eps_orig >= 0) res64[kd.eps_orig + A.perm[od.perm + i]] = eps;
     }
     pairs += (unsigned long long)(t1 - t0) * (unsigned long long)visited;
@@ -543,7 +543,7 @@ __device__ void walk_item(const KnnArgs& A, const Item& it, const double* __rest
     const unsigned v = knn_walk(w, kd, col64, it.lo, it.hi, it.t0, i, active);
     const double eps = w.L[P - 1];
     if (active) {
-        res64[kd.eps + i] = eps;
+        if (kd.eps >= 0) res64[kd.eps + i] = eps;
         if (kd.eps_orig >= 0) res64[kd.eps_orig + A.perm[od.perm + i]] = eps;
     }
     pairs += wave_sum_u64(v);
