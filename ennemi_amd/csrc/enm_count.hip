@@ -150,8 +150,8 @@ void count1d_kernel(CountArgs A, const double* __restrict__ col64, const double*
             rq[q] = i < it.t1 ? query_radius(cd, A, res64, i) : -1.0;
         }
         // Both boundaries of all ENM_C1_Q queries are searched in lockstep
-        // (fixed halving steps from the query's own position, 2048 down to
-        // 1, which spans the whole window): 2 * ENM_C1_Q independent LDS
+        // (halving power-of-two steps from the query's own position, which
+        // together span the whole window): 2 * ENM_C1_Q independent LDS
         // reads per step instead of one dependent chain per search.
         double xq[ENM_C1_Q];
         int pr[ENM_C1_Q], pl[ENM_C1_Q];
@@ -168,7 +168,9 @@ void count1d_kernel(CountArgs A, const double* __restrict__ col64, const double*
             live[q] = i < it.t1 && !(r < 0.0);
             dq[q] = live[q] && ((w1 < it.hi && whi - x <= r) || (w0 > it.lo && x - wlo <= r));
         }
-        for (int step = ENM_C1_CAP / 2; step >= 1; step >>= 1) {
+        int top = 1;   // largest power of two below the window: the steps reach any slot
+        while (2 * top < ENM_C1_CAP) top *= 2;
+        for (int step = top; step >= 1; step >>= 1) {
 #pragma unroll
             for (int q = 0; q < ENM_C1_Q; ++q) {
                 const int jr = pr[q] + step, jl = pl[q] - step;
