@@ -204,12 +204,13 @@ def main() -> int:
     # Roofline of the dominant kernel: the kNN walk (one launch per step for
     # this workload, timed by the events around the kNN stage).  It is
     # HBM-roofed by its algorithmic traffic: each query point's two
-    # coordinates read once (16 B), its radius written in two orders (16 B)
-    # and the permutation read (4 B) -- 36 B per point; candidate reads are
-    # cache hits by design.  `traffic` is the measured HBM bytes per launch
+    # coordinates read once (16 B), its radius written in point order (8 B;
+    # no count of this workload reads it in kNN order, and nothing is
+    # exported) and the permutation read (4 B) -- 28 B per point; candidate
+    # reads are cache hits by design.  `traffic` is the measured HBM bytes per launch
     # (FETCH_SIZE + WRITE_SIZE, copy-calibrated) from the committed profile.
     n_points = int(ns.sum())
-    knn_bytes = 36 * n_points
+    knn_bytes = 28 * n_points
     knn_ach = knn_bytes / (stage_mean["knn"] * 1e-3) / 1e9
     traffic = None
     tpath = os.path.join(ROOT, "profiles", "lag_scan_776_ksg_hbm_traffic.json")
@@ -218,8 +219,8 @@ def main() -> int:
             traffic = json.load(f).get("knn_walk_kernel<2, 4>", {}).get("hbm_bytes_per_launch")
     roofline = {"bound": "hbm", "kernel": "knn_walk_kernel<D=2,P=4>", "achieved": knn_ach,
                 "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": knn_ach / PEAK_HBM_GBS, "traffic": traffic,
-                "algorithmic": f"{n_points} query points x 36 B (2 coordinates read, radius written "
-                               f"in kNN and point order, permutation read)",
+                "algorithmic": f"{n_points} query points x 28 B (2 coordinates read, radius written "
+                               f"in point order, permutation read)",
                 "launch_ms": round(stage_mean["knn"], 4),
                 "pair_evaluations": st["knn_pairs"],
                 "pair_rate_T_per_s": st["knn_pairs"] / (stage_mean["knn"] * 1e-3) / 1e12}
