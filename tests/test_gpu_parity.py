@@ -352,3 +352,39 @@ def test_sample_sorted_class_segments():
     np.testing.assert_array_equal(eps[inv], ref["eps"])
     np.testing.assert_array_equal(c[0][inv], ref["n_full"].astype(np.int64))
     assert _close(v, w), (v, w)
+
+
+def test_sample_sorted_modes_graphs_and_exports_agree():
+    # Instances long enough for the sample sort (segments above 2048 keys):
+    # every mode bitwise equal to FP64 brute force per instance, graph
+    # replays equal to eager runs, and the values the same with and without
+    # exported radii (without, no radius is kept in kNN order).
+    nat = _nat()
+    from ennemi_amd._batch import Batch
+    import torch
+    rng = np.random.default_rng(21)
+    b = Batch()
+    for t in range(6):
+        n = 2500 + 1700 * t
+        a = rng.normal(size=(n, 4))
+        if t % 3 == 0:
+            b.add(nat.KIND_KSG, 3, [a[:, 0]], [a[:, 1]])
+        elif t % 3 == 1:
+            b.add(nat.KIND_CMI, 3, [a[:, 0]], [a[:, 1]], [a[:, 2]])
+        else:
+            b.add(nat.KIND_KL, 3, [a[:, 0], a[:, 1]])
+    ref = b.run(flags=nat.F_FP64 | nat.F_BRUTE | nat.F_PER_INSTANCE, export=True)
+    for flags in (0, nat.F_FP64, nat.F_PER_INSTANCE | nat.F_FP64):
+        got = b.run(flags=flags, export=True)
+        np.testing.assert_array_equal(got[0], ref[0])
+        for e1, e2 in zip(got[1], ref[1]):
+            np.testing.assert_array_equal(e1, e2)
+        for c1, c2 in zip(got[2], ref[2]):
+            np.testing.assert_array_equal(c1, c2)
+        np.testing.assert_array_equal(b.run(flags=flags), ref[0])
+    db = b.device(flags=nat.F_GRAPH)
+    for rep in range(3):
+        db.launch()
+        np.testing.assert_array_equal(db.results(), ref[0])
+        torch.ones(1000, device="cuda").sum().item()
+    assert db.stats()["graph_replayed"] == 1
