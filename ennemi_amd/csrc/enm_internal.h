@@ -51,7 +51,7 @@ struct SortSeg {
 // keys): each segment is split into nb buckets by nb - 1 splitters drawn from
 // ENM_SS_OVER * nb sorted samples, its keys are scattered bucket by bucket,
 // and runs of whole buckets of at most ENM_SS_TILE keys are sorted in LDS.
-#define ENM_SS_BUCKET 512      // target keys per bucket
+#define ENM_SS_BUCKET 768      // target keys per bucket
 #define ENM_SS_OVER 16         // samples per bucket
 #define ENM_SS_MAXB 1024       // most buckets of one segment
 #define ENM_SS_MAXLEN (ENM_SS_MAXB * 1024)

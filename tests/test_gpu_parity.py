@@ -285,7 +285,7 @@ def test_large_ksg_against_kdtree():
 def _sample_positions(n):
     """Positions the sample sort draws its splitter samples from in a
     segment of n keys (the device's stride-and-hash rule)."""
-    nb = min(1024, -(-n // 512))
+    nb = min(1024, -(-n // 768))
     ns = 16 * nb if nb > 1 else 0
     if ns == 0:
         return np.zeros(0, dtype=np.int64)
@@ -325,7 +325,7 @@ def test_sample_sort_oversized_bucket():
     rng = np.random.default_rng(99)
     x = rng.normal(size=n)
     pos = _sample_positions(n)
-    assert len(pos) == 16 * 40 and len(np.unique(pos)) == len(pos) and pos.max() < n
+    assert len(pos) == 16 * 27 and len(np.unique(pos)) == len(pos) and pos.max() < n
     x[pos] = 1e3 + np.arange(len(pos))
     y = rng.normal(size=n)
     _ksg_vs_kdtree(x, y, 3)
