@@ -105,7 +105,7 @@ __device__ __forceinline__ int global_left(const double* __restrict__ v, int lo,
 // 1-D counts.  An item is ENM_C1_ITEM consecutive positions of a key-sorted
 // order, ENM_C1_Q per thread of a 256-thread workgroup.  The workgroup
 // copies the keys of the item and ENM_C1_HALO positions on each side into
-// LDS with coalesced loads (32 KB, shared by its four waves), and each
+// LDS with coalesced loads (40 KB, shared by its four waves), and each
 // query binary-searches both boundaries there.  A query whose ball reaches
 // past the staged keys on either side (its predicate still holds at the
 // window's edge, about 1 % of queries on Gaussian-like data, but in almost

@@ -25,8 +25,8 @@
 #define ENM_C1_THREADS 256     // threads of a 1-D count workgroup
 #define ENM_C1_Q 8             // queries per thread of a 1-D count item
 #define ENM_C1_ITEM (ENM_C1_THREADS * ENM_C1_Q)   // positions per 1-D count item (2048)
-#define ENM_C1_HALO 1024       // keys staged on each side of a 1-D count item
-#define ENM_C1_CAP (ENM_C1_ITEM + 2 * ENM_C1_HALO)  // LDS window (doubles, 32 KB)
+#define ENM_C1_HALO 1528       // keys staged on each side of a 1-D count item
+#define ENM_C1_CAP (ENM_C1_ITEM + 2 * ENM_C1_HALO)  // LDS window (doubles, 40 KB: four workgroups per CU)
 #define ENM_C1_OFF_BITS 11     // position bits of a deferred 1-D query code
 #define ENM_NDEF 64            // segments (and counters) of the deferred 1-D query list
 #define ENM_CSTRIDE 32         // unsigned slots between two atomic counters (one per 128 B line)
