@@ -1,10 +1,14 @@
-"""Plug-in estimators for discrete variables (host, NumPy).
+"""Plug-in estimators for discrete variables: host fallback.
 
-These have no neighbour search and are not part of the accelerated path:
-they count unique values (np.unique), as the reference does in
-ennemi/_entropy_estimators.py:44-66 and 249-320.  They complete the
-``discrete_x=discrete_y=True`` and ``estimate_entropy(discrete=True)`` cases
-of the public API.
+The plug-in entropy and MI of discrete variables need no neighbour search,
+only the number of points per distinct value (or row).  This module computes
+them by factorising each variable into dense codes (the position of its value
+among the sorted distinct values, so code order is np.unique order) and
+counting code combinations with a histogram.  It serves the inputs that the
+device counting path does not take (strings and other non-numeric data); the
+estimates follow ennemi/_entropy_estimators.py:44-56 (entropy), :257-289
+(MI) and :291-320 (conditional MI), and the error and warning texts are the
+reference's.
 """
 
 from __future__ import annotations
@@ -13,62 +17,82 @@ from warnings import warn
 
 import numpy as np
 
+OBJECT_MESSAGE = ("Data type 'object' is not supported." +
+                  " Please pass only numeric, boolean, or string data." +
+                  " If your data is in a pandas DataFrame, convert string categories" +
+                  " to integers (pandas stores strings as objects).")
 
-def _assert_not_object(x) -> None:
-    if x.dtype.kind == "O":
-        raise TypeError("Data type 'object' is not supported." +
-                        " Please pass only numeric, boolean, or string data." +
-                        " If your data is in a pandas DataFrame, convert string categories" +
-                        " to integers (pandas stores strings as objects).")
+MANY_VALUES_MESSAGE = ("A discrete variable has relatively many unique values." +
+                       " Have you set marked the discrete variables in correct order?" +
+                       " If both X and Y are discrete, the conditioning variable cannot be continuous" +
+                       " (this limitation can be lifted in the future).")
+
+
+def reject_object(a) -> None:
+    """TypeError for object arrays (_entropy_estimators.py:58-66)."""
+    if a.dtype.kind == "O":
+        raise TypeError(OBJECT_MESSAGE)
 
 
 def verify_not_continuous(values, n: int) -> None:
-    """_entropy_estimators.py:249-254."""
+    """Warn when more than a quarter of the points are distinct values
+    (_entropy_estimators.py:249-254)."""
     if len(values) > n / 4:
-        warn("A discrete variable has relatively many unique values." +
-             " Have you set marked the discrete variables in correct order?" +
-             " If both X and Y are discrete, the conditioning variable cannot be continuous" +
-             " (this limitation can be lifted in the future).", UserWarning)
+        warn(MANY_VALUES_MESSAGE, UserWarning)
+
+
+def factorize(a) -> tuple[np.ndarray, np.ndarray]:
+    """(code of every row, points per code) of a 1-D or 2-D array, codes in
+    the sorted order of the distinct values or rows."""
+    _, inverse, counts = np.unique(a, axis=0, return_inverse=True, return_counts=True)
+    return np.ravel(inverse).astype(np.int64), counts.astype(np.int64)
+
+
+def mi_from_codes(cx, nx, cy, ny, n: int) -> float:
+    """sum over occupied (x, y) cells of c log(n c / (n_x n_y)), divided by n.
+
+    The cells are visited in (x code, y code) order, the order of the
+    reference's np.unique over the stacked columns, and summed by np.sum."""
+    nbin = len(nx) * len(ny)
+    joint = np.bincount(cx * len(ny) + cy, minlength=nbin)
+    cells = np.flatnonzero(joint)
+    c = joint[cells]
+    a = nx[cells // len(ny)]
+    b = ny[cells % len(ny)]
+    return float(np.sum(c * np.log(n * c / (a * b)))) / n
 
 
 def estimate_discrete_entropy(x) -> float:
-    """-sum p log p over the unique rows of x (_entropy_estimators.py:44-56)."""
-    n = x.shape[0]
-    _assert_not_object(x)
-    _, counts = np.unique(x, axis=0, return_counts=True)
-    probs = counts / n
-    return -np.sum(np.dot(probs, np.log(probs)))
+    """-sum p log p over the distinct rows of x."""
+    reject_object(x)
+    _, counts = factorize(x)
+    p = counts / x.shape[0]
+    return -float(np.dot(p, np.log(p)))
 
 
 def estimate_discrete_mi(x, y) -> float:
-    """sum p(x,y) log(p(x,y) / (p(x) p(y))) (_entropy_estimators.py:257-289)."""
+    """Plug-in MI of two discrete variables (both converted to their common
+    type first, as np.column_stack does in the reference)."""
     n = len(x)
-    data = np.column_stack((x, y))
-    _assert_not_object(data)
-    x_vals, x_counts = np.unique(data[:, 0], return_counts=True)
-    x_dict = dict(zip(x_vals, x_counts))
-    y_vals, y_counts = np.unique(data[:, 1], return_counts=True)
-    y_dict = dict(zip(y_vals, y_counts))
-    joint_vals, joint_counts = np.unique(data, axis=0, return_counts=True)
-    verify_not_continuous(x_vals, n)
-    verify_not_continuous(y_vals, n)
-
-    def sum_term(a):
-        joint_weight = int(a[2])
-        return joint_weight * np.log(n * joint_weight / (x_dict[a[0]] * y_dict[a[1]]))
-
-    return np.sum(np.apply_along_axis(sum_term, 1, np.column_stack((joint_vals, joint_counts)))) / n
+    pair = np.column_stack((x, y))
+    reject_object(pair)
+    cx, nx = factorize(pair[:, 0])
+    cy, ny = factorize(pair[:, 1])
+    verify_not_continuous(nx, n)
+    verify_not_continuous(ny, n)
+    return mi_from_codes(cx, nx, cy, ny, n)
 
 
 def estimate_conditional_discrete_mi(x, y, cond) -> float:
-    """sum_z p(z) MI(x; y | z) (_entropy_estimators.py:291-320)."""
+    """sum over condition values z of n_z MI(x; y | z), divided by n."""
     n = len(x)
-    _assert_not_object(cond)
-    cond_vals, cond_inverses, cond_counts = np.unique(cond, axis=0, return_inverse=True,
-                                                      return_counts=True)
-    cond_inverses = np.ravel(cond_inverses)
-    cond_probs = np.zeros(len(cond_vals))
-    for i in range(len(cond_vals)):
-        cond_probs[i] = cond_counts[i] * estimate_discrete_mi(x[cond_inverses == i],
-                                                              y[cond_inverses == i])
-    return np.sum(cond_probs).item() / n
+    reject_object(cond)
+    cz, nz = factorize(cond)
+    # Group the points by condition value, keeping their order in a group.
+    order = np.argsort(cz, kind="stable")
+    ends = np.cumsum(nz)
+    terms = np.zeros(len(nz))
+    for g in range(len(nz)):
+        rows = order[ends[g] - nz[g]:ends[g]]
+        terms[g] = nz[g] * estimate_discrete_mi(x[rows], y[rows])
+    return float(np.sum(terms)) / n

@@ -21,7 +21,11 @@ CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(ROOT, "_build", "obj")
 LIB = os.path.join(HERE, "libennemi_hip.so")
 
-SOURCES = ["enm_sort.hip", "enm_knn.hip", "enm_count.hip", "enm_reduce.hip", "enm_host.cpp"]
+# The kNN kernels compile one joint-dimension class per translation unit
+# (largest first, so the parallel build starts with the slowest).
+KNN_DIMS = (2, 3, 4, 5, 6, 8, 12, 16, 1)
+SOURCES = ([f"enm_knn_d{d}.hip" for d in KNN_DIMS] +
+           ["enm_count.hip", "enm_sort.hip", "enm_knn.hip", "enm_reduce.hip", "enm_host.cpp"])
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950",
@@ -55,7 +59,7 @@ def _compile(src: str) -> str:
 def build(verbose: bool = False) -> str:
     """Compile every source and link the shared library; return its path."""
     os.makedirs(OBJ, exist_ok=True)
-    with ThreadPoolExecutor(max_workers=min(8, len(SOURCES))) as ex:
+    with ThreadPoolExecutor(max_workers=min(os.cpu_count() or 8, len(SOURCES))) as ex:
         objs = list(ex.map(_compile, SOURCES))
     tmp = LIB + ".tmp"
     cmd = [hipcc(), "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp, *objs]

@@ -1,0 +1,648 @@
+// Chebyshev (max-norm) k-th-neighbour radius, replacing
+// cKDTree.query(pts, k=[k+1], p=np.inf) of the reference
+// (ennemi/_entropy_estimators.py:39,108,142,194,240).
+//
+// eps_i is the (k+1)-th smallest D(i,j) = max_d fl64(|a_d[i] - a_d[j]|) over
+// all j of the query's scope range, i itself included (distance 0); fewer
+// than k+1 points leave it +inf, as cKDTree does.
+//
+// Grid orders: the key-sorted points cut into strips, each strip sorted by
+// the second column.  A work item is one wave and 64 consecutive positions
+// of one strip (one query per lane).  With two columns each lane walks the
+// strips around its own point by itself (knn_walk): its own strip from its
+// own position, then the strips outward that its current radius still
+// reaches on the key axis, each from the lane's y position in both
+// directions until the y distance reaches the radius.  Candidates arrive in
+// about increasing distance, so each lane evaluates about ten candidates.
+// Each lane keeps an ascending FP64 list of its K1 = k+1 smallest distances
+// in registers (P slots, the lowest P-K1 pinned at -inf).
+//
+// Grid orders of three or more columns, where the two gridded columns prune
+// less and a lane's walk would be long, share candidates instead: a wave
+// takes a tile of 64 consecutive positions of a strip and walks candidates
+// in wave-uniform order (grid_sweep: its own strip, then the strips outward,
+// each within the union of the lanes' y intervals).  Other orders (one
+// column, or ENM_F_BRUTE) do the same over the key-sorted order, 8
+// candidates at a time through the scalar cache (candidate coordinates are
+// SGPR operands); with one column, from the tile outward in key order, alternating
+// left and right in steps of 64, a side stopping once every lane's key
+// distance to the last visited candidate is >= its current K1-th distance T.
+// With ENM_F_BRUTE the whole scope range is scanned.
+//
+// Precision.  Every distance that enters a list is exact FP64.  The FP32
+// filter (brute force in FP32-filter mode) computes the bulk distance from
+// FP32 copies, and a chunk of 8 candidates is settled in FP64 only when some
+// lane's FP32 distance is <= fl32_up(T + g), with the per-instance guard g
+// bounding |d32 - d64|.  T only decreases, so one pass is exact: radii are
+// bit-identical to the FP64 mode.  While most chunks improve some lane's list
+// the wave skips the filter and evaluates chunks in FP64 directly.  List
+// updates for up to 8 slots are branch-free: the 8 exact distances are
+// sorted by a 19-comparator network and merged into the list.
+//
+// Work distribution.  A persistent grid pops items from ENM_NQ queues, one
+// per XCD (for_each_wave_item), in the order the host laid them out.
+#pragma once
+#include "enm_device.h"
+#include "enm_launch.h"
+
+#include <algorithm>
+
+
+// Arguments of every kNN kernel (shared by the per-dimension translation
+// units, which instantiate the kernels of one joint dimension each).
+struct KnnArgs {
+    const Item* items;
+    const int* qoff;    // ENM_NQ + 1 queue offsets into items
+    unsigned* counter;  // ENM_NQ ticket counters
+    int chunk;          // items per ticket
+    const KnnDev* descs;
+    const OrderDev* orders;
+    const unsigned long long* maxabs;
+    const int* perm;
+    int brute;
+};
+
+// Launch of the kNN kernels of one joint dimension D (one translation unit
+// per dimension class, so the kernels compile in parallel).
+#define ENM_KNN_DIMS(X) X(1) X(2) X(3) X(4) X(5) X(6) X(8) X(12) X(16)
+#define ENM_KNN_DECLARE(D) \
+    hipError_t enm_knn_launch_d##D(const KnnArgs& a, int P, bool f32, bool walk, int grid, \
+                                   const double* col64, const float* col32, double* res64, \
+                                   unsigned long long* stats, hipStream_t st);
+ENM_KNN_DIMS(ENM_KNN_DECLARE)
+
+#ifdef ENM_KNN_KERNELS
+namespace {
+
+constexpr int CH = 8;     // candidates per filter chunk
+constexpr int SWEEP = 64; // candidates per side per sweep step
+
+
+
+template <int D, int P, bool F32>
+struct KnnTile {
+    const double* c64[D];
+    const float* c32[D];
+    double q64[D];
+    float q32[D];
+    double L[P];
+    double T;
+    float thr;
+    double guard;
+    unsigned rechecks;
+    bool dense;   // FP32 mode: evaluating chunks directly in FP64
+
+    __device__ __forceinline__ void set_thr() {
+        if (F32) thr = f32_up(T + guard);
+    }
+
+    __device__ __forceinline__ void recheck(int j) {
+        double e = fabs(c64[0][j] - q64[0]);
+#pragma unroll
+        for (int d = 1; d < D; ++d) e = fmax(e, fabs(c64[d][j] - q64[d]));
+        ++rechecks;
+        if (e < T) {
+            list_insert<P>(L, e);
+            T = L[P - 1];
+            set_thr();
+        }
+    }
+
+    // Candidate coordinates of chunk j (wave-uniform: scalar loads).
+    __device__ __forceinline__ void load64(double (&cx)[D][CH], int j) const {
+#pragma unroll
+        for (int d = 0; d < D; ++d)
+#pragma unroll
+            for (int c = 0; c < CH; ++c) cx[d][c] = c64[d][j + c];
+    }
+
+    __device__ __forceinline__ bool chunk64(int j) {
+        double cx[D][CH];
+        load64(cx, j);
+        return chunk64v(cx);
+    }
+
+    // FP64 chunk: exact distances; merge when some lane improves.  Returns
+    // whether any lane could improve its list.
+    __device__ __forceinline__ bool chunk64v(const double (&cx)[D][CH]) {
+        double dd[CH];
+#pragma unroll
+        for (int c = 0; c < CH; ++c) {
+            double m = fabs(cx[0][c] - q64[0]);
+#pragma unroll
+            for (int d = 1; d < D; ++d) m = fmax(m, fabs(cx[d][c] - q64[d]));
+            dd[c] = m;
+        }
+        return take(dd);
+    }
+
+    // List update with the exact distances of one chunk.
+    __device__ __forceinline__ bool take(double (&dd)[CH]) {
+        double mn = dd[0];
+#pragma unroll
+        for (int c = 1; c < CH; ++c) mn = fmin(mn, dd[c]);
+        if (!wave_any(mn < T)) return false;
+        bool merged = false;
+        if constexpr (P <= 8) {
+            // After a chunk that improved some list (dense regime) most of
+            // the 8 candidates matter to some lane: one network merge.
+            if (dense) {
+                sort8(dd);
+                list_merge8<P>(L, dd);
+                T = L[P - 1];
+                merged = true;
+            }
+        }
+        if (!merged) {
+            // Sparse regime: only the few candidates some lane needs.
+#pragma unroll
+            for (int c = 0; c < CH; ++c) {
+                if (wave_any(dd[c] < T) && dd[c] < T) {
+                    list_insert<P>(L, dd[c]);
+                    T = L[P - 1];
+                }
+            }
+        }
+        if (F32) set_thr();
+        return true;
+    }
+
+    // FP32-filter chunk: FP32 distances; if some lane's minimum passes the
+    // guard-widened threshold, the chunk is settled exactly in FP64.
+    // Returns whether that happened.
+    __device__ __forceinline__ bool chunk32(int j) {
+        float dd[CH];
+#pragma unroll
+        for (int c = 0; c < CH; ++c) {
+            float m = fabsf(c32[0][j + c] - q32[0]);
+#pragma unroll
+            for (int d = 1; d < D; ++d) m = fmaxf(m, fabsf(c32[d][j + c] - q32[d]));
+            dd[c] = m;
+        }
+        float mn = dd[0];
+#pragma unroll
+        for (int c = 1; c < CH; ++c) mn = fminf(mn, dd[c]);
+        if (!wave_any(mn <= thr)) return false;
+#pragma unroll
+        for (int c = 0; c < CH; ++c) rechecks += dd[c] <= thr;
+        // A candidate the filter rejected has an FP64 distance > T, so
+        // settling the whole chunk in FP64 gives the same list.
+        (void)chunk64(j);
+        return true;
+    }
+
+    // One chunk of 8 candidates.  The wave adapts to how often chunks
+    // improve some lane's list: after such a chunk ("dense") it evaluates the
+    // next in FP64 directly and updates lists with the merge network; after a
+    // chunk that improved none it filters in FP32 (FP32-filter mode) and
+    // inserts the few passing candidates one by one.
+    __device__ __forceinline__ void chunk(int j) {
+        if (F32 && !dense) dense = chunk32(j);
+        else dense = chunk64(j);
+    }
+
+    __device__ __forceinline__ void single(int j) {
+        if (F32) {
+            float m = fabsf(c32[0][j] - q32[0]);
+#pragma unroll
+            for (int d = 1; d < D; ++d) m = fmaxf(m, fabsf(c32[d][j] - q32[d]));
+            if (wave_any(m <= thr) && m <= thr) recheck(j);
+        } else {
+            double m = fabs(c64[0][j] - q64[0]);
+#pragma unroll
+            for (int d = 1; d < D; ++d) m = fmax(m, fabs(c64[d][j] - q64[d]));
+            if (m < T) {
+                list_insert<P>(L, m);
+                T = L[P - 1];
+            }
+        }
+    }
+
+    // Process the wave-uniform candidate range [a, b).  In FP64 mode with
+    // up to two columns the next chunk's coordinates are loaded before the
+    // current chunk is evaluated (software pipelining: the scalar loads of a
+    // chunk often miss the scalar cache, and their L2 latency then overlaps
+    // a chunk of arithmetic instead of stalling the wave); both buffers fit
+    // in SGPRs (2 x 2 x 8 doubles).
+    __device__ __forceinline__ void range(int a, int b) {
+        int j = a;
+        if constexpr (!F32 && D <= 2) {
+            // FP64 with up to two columns: the next chunk's coordinates are
+            // loaded before the current chunk is evaluated (software
+            // pipelining of the scalar loads; both buffers fit in SGPRs).
+            if (j + CH <= b) {
+                double cur[D][CH];
+                load64(cur, j);
+                for (; j + CH <= b; j += CH) {
+                    double nxt[D][CH];
+                    // The last chunk reloads itself (in bounds, unused).
+                    load64(nxt, j + 2 * CH <= b ? j + CH : j);
+                    dense = chunk64v(cur);
+#pragma unroll
+                    for (int d = 0; d < D; ++d)
+#pragma unroll
+                        for (int c = 0; c < CH; ++c) cur[d][c] = nxt[d][c];
+                }
+            }
+        } else {
+            for (; j + CH <= b; j += CH) chunk(j);
+        }
+        for (; j < b; ++j) single(j);
+    }
+};
+
+// First position in [ks, ke) whose key is >= y, one search per lane (the
+// trip count depends only on ke - ks, up to one step).
+__device__ __forceinline__ int lane_lower_bound(const double* __restrict__ v, int ks, int ke, double y) {
+    int first = ks, len = ke - ks;
+    while (len > 0) {
+        const int half = len >> 1;
+        const bool go = v[first + half] < y;
+        first = go ? first + half + 1 : first;
+        len = go ? len - half - 1 : half;
+    }
+    return first;
+}
+
+// Walk of one strip [ks, ke) (sorted by column 1) for every live lane, from
+// its own start: upward from u and downward from dn, one candidate per
+// direction and iteration, so candidates come in about increasing y
+// distance.  A direction stops at the first candidate whose y distance is
+// >= the lane's current radius T (the strip is sorted, so every later one
+// is further still, and T only decreases); the others are evaluated and
+// offered to the list (an insertion at or above T leaves it unchanged).
+// The next candidate's coordinates are loaded before the current one is
+// evaluated.  Adds the lane's evaluated candidates to `visited`.
+template <int D, int P, bool F32>
+__device__ __forceinline__ void walk_strip(KnnTile<D, P, F32>& w, int ks, int ke, int u, int dn, bool live,
+                                           unsigned& visited) {
+    const double* __restrict__ yc = w.c64[1];
+    const double qy = w.q64[1];
+    bool up = live && u < ke;
+    bool down = live && dn >= ks;
+    double cu[D], cd[D];
+    {
+        const int ju = up ? u : ks, jd = down ? dn : ks;
+#pragma unroll
+        for (int d = 0; d < D; ++d) { cu[d] = w.c64[d][ju]; cd[d] = w.c64[d][jd]; }
+    }
+    while (wave_any(up || down)) {
+        double nu[D], nd[D];
+        {
+            const int ju = up && u + 1 < ke ? u + 1 : ks;
+            const int jd = down && dn - 1 >= ks ? dn - 1 : ks;
+#pragma unroll
+            for (int d = 0; d < D; ++d) { nu[d] = w.c64[d][ju]; nd[d] = w.c64[d][jd]; }
+        }
+        {
+            const double dy = cu[1] - qy;
+            up = up && dy < w.T;
+            double m = dy;
+#pragma unroll
+            for (int d = 0; d < D; ++d)
+                if (d != 1) m = fmax(m, fabs(cu[d] - w.q64[d]));
+            list_insert<P>(w.L, up ? m : INFINITY);
+            w.T = w.L[P - 1];
+            visited += up;
+        }
+        {
+            const double dy = qy - cd[1];
+            down = down && dy < w.T;
+            double m = dy;
+#pragma unroll
+            for (int d = 0; d < D; ++d)
+                if (d != 1) m = fmax(m, fabs(cd[d] - w.q64[d]));
+            list_insert<P>(w.L, down ? m : INFINITY);
+            w.T = w.L[P - 1];
+            visited += down;
+        }
+        ++u;
+        --dn;
+        up = up && u < ke;
+        down = down && dn >= ks;
+#pragma unroll
+        for (int d = 0; d < D; ++d) { cu[d] = nu[d]; cd[d] = nd[d]; }
+    }
+}
+
+// kNN on a grid order by per-lane walks: each lane walks its own strip
+// from its own position, then the strips outward on each side that its
+// radius still reaches on the key axis (x gap to the strip < T), entering
+// each at its own y position (per-lane binary search).  A side stops for
+// the wave once no lane reaches the next strip (strips further out are
+// further in x).  Every candidate that could enter a lane's list is
+// evaluated, in about increasing distance, so lists settle early; a lane
+// evaluates a few dozen candidates instead of the whole tile's window.
+// Returns the lane's number of evaluated candidates (the query itself
+// included).
+template <int D, int P, bool F32>
+__device__ unsigned knn_walk(KnnTile<D, P, F32>& w, const KnnDev& kd, const double* __restrict__ col64,
+                             int lo, int hi, int t0, int i, bool active) {
+    const int S = kd.strip;
+    const double* xk = col64 + kd.base_key;  // base order: column 0 sorted over [lo, hi)
+    const int s = (t0 - lo) / S;
+    const int nstrip = (hi - lo + S - 1) / S;
+    unsigned visited = active ? 1u : 0u;
+    if (active) {
+        list_insert<P>(w.L, 0.0);  // the query itself
+        w.T = w.L[P - 1];
+    }
+    {
+        const int ss = lo + s * S, se = min(hi, ss + S);
+        walk_strip(w, ss, se, i + 1, i - 1, active, visited);
+    }
+    bool ldone = s == 0, rdone = s == nstrip - 1;
+    for (int d = 1; !(ldone && rdone); ++d) {
+        if (!ldone) {
+            const int sl = s - d;
+            if (sl < 0) {
+                ldone = true;
+            } else {
+                const int ks = lo + sl * S, ke = ks + S;  // a full strip (left of s)
+                const bool need = active && w.q64[0] - xk[ke - 1] < w.T;
+                ldone = !wave_any(need);
+                if (!ldone) {
+                    const int p = lane_lower_bound(w.c64[1], ks, ke, w.q64[1]);
+                    walk_strip(w, ks, ke, p, p - 1, need, visited);
+                }
+            }
+        }
+        if (!rdone) {
+            const int sr = s + d;
+            if (sr >= nstrip) {
+                rdone = true;
+            } else {
+                const int ks = lo + sr * S, ke = min(hi, ks + S);
+                const bool need = active && xk[ks] - w.q64[0] < w.T;
+                rdone = !wave_any(need);
+                if (!rdone) {
+                    const int p = lane_lower_bound(w.c64[1], ks, ke, w.q64[1]);
+                    walk_strip(w, ks, ke, p, p - 1, need, visited);
+                }
+            }
+        }
+    }
+    return visited;
+}
+
+// Grid sweep of one tile (already seeded with its own 64 points): the rest
+// of the tile's strip near the tile in y order, then whole strips outward on
+// both sides.  A strip is visited only within the y window of the lanes'
+// current radii; a side stops at the first strip whose x bound is at least
+// every lane's radius away (all strips beyond it are further in x).  Each
+// candidate is visited at most once, and every candidate that could still
+// enter a lane's list is visited.  Returns the number of candidates visited.
+template <int D, int P, bool F32>
+__device__ int grid_sweep(KnnTile<D, P, F32>& w, const KnnDev& kd, const double* __restrict__ col64,
+                          int lo, int hi, int t0, int t1, bool active) {
+    constexpr int SEED = 64;
+    const int S = kd.strip;
+    const double* xk = col64 + kd.base_key;  // base order: column 0 sorted over [lo, hi)
+    const double* yc = w.c64[1];             // column 1, sorted inside each strip
+    const int s = (t0 - lo) / S;
+    const int nstrip = (hi - lo + S - 1) / S;
+    const int ss = lo + s * S, se = min(hi, ss + S);
+    const double yq = w.q64[1];
+    int visited = 0;
+    // Seed: up to SEED neighbours in y on each side of the tile.
+    const int a0 = max(ss, t0 - SEED), b0 = min(se, t1 + SEED);
+    w.range(a0, t0);
+    w.range(t1, b0);
+    visited += (t0 - a0) + (b0 - t1);
+    {
+        int a, b;
+        strip_window(yc, ss, se, yq, w.T, active, a, b);
+        if (a < a0) { w.range(a, a0); visited += a0 - a; }
+        if (b > b0) { w.range(b0, b); visited += b - b0; }
+    }
+    bool ldone = s == 0, rdone = s == nstrip - 1;
+    for (int d = 1; !(ldone && rdone); ++d) {
+        if (!ldone) {
+            const int sl = s - d;
+            if (sl < 0) {
+                ldone = true;
+            } else {
+                const int le = lo + (sl + 1) * S;  // strip sl is [le - S, le)
+                const double xb = xk[le - 1];      // largest x of strip sl
+                // Lanes whose radius reaches this strip on the key axis.
+                const bool need = w.q64[0] - xb < w.T;
+                ldone = !wave_any(need);
+                if (!ldone) {
+                    int a, b;
+                    strip_window(yc, le - S, le, yq, w.T, need, a, b);
+                    w.range(a, b);
+                    visited += b - a;
+                }
+            }
+        }
+        if (!rdone) {
+            const int sr = s + d;
+            if (sr >= nstrip) {
+                rdone = true;
+            } else {
+                const int rs = lo + sr * S;
+                const double xb = xk[rs];          // smallest x of strip sr
+                const bool need = xb - w.q64[0] < w.T;
+                rdone = !wave_any(need);
+                if (!rdone) {
+                    int a, b;
+                    strip_window(yc, rs, min(hi, rs + S), yq, w.T, need, a, b);
+                    w.range(a, b);
+                    visited += b - a;
+                }
+            }
+        }
+    }
+    return visited;
+}
+
+template <int D, int P, bool F32>
+__device__ void knn_item(const KnnArgs& A, const Item& it, const double* __restrict__ col64,
+                         const float* __restrict__ col32, double* __restrict__ res64,
+                         unsigned long long& pairs, unsigned long long& rechecks) {
+    const KnnDev kd = A.descs[it.desc];
+    const OrderDev od = A.orders[kd.order];
+    const int lane = threadIdx.x;
+    const int64_t n = od.n;
+    const int i = it.t0 + lane;
+    const bool active = i < it.t1;
+    const int ncol = od.ncol;
+
+    KnnTile<D, P, F32> w;
+    bool f32ok = true;
+    w.guard = enm_guard(A.maxabs[od.inst], &f32ok);
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+        // Dimensions beyond the order's columns repeat column 0, which
+        // leaves the max-norm unchanged.
+        const int c = d < ncol ? d : 0;
+        w.c64[d] = col64 + od.c64 + c * n;
+        w.c32[d] = col32 + od.c32 + c * n;
+        w.q64[d] = active ? w.c64[d][i] : 0.0;
+        w.q32[d] = (float)w.q64[d];
+    }
+    const int K1 = kd.k + 1;
+#pragma unroll
+    for (int t = 0; t < P; ++t) w.L[t] = (t < P - K1) ? -INFINITY : INFINITY;
+    w.T = active ? INFINITY : -INFINITY;
+    w.thr = active ? INFINITY : -INFINITY;
+    w.rechecks = 0;
+    w.dense = true;   // a tile starts with its nearest candidates
+
+    const int lo = it.lo, hi = it.hi, t0 = it.t0, t1 = it.t1;
+    int visited = t1 - t0;
+    w.range(t0, t1);
+    bool swept = false;
+    if constexpr (D >= 2) {
+        if (kd.strip > 0 && !A.brute) {
+            visited += grid_sweep(w, kd, col64, lo, hi, t0, t1, active);
+            swept = true;
+        }
+    }
+    if (swept) {
+    } else if (A.brute) {
+        w.range(lo, t0);
+        w.range(t1, hi);
+        visited = hi - lo;
+    } else {
+        int left = t0, right = t1;
+        bool ldone = left <= lo, rdone = right >= hi;
+        while (!(ldone && rdone)) {
+            if (!ldone) {
+                const int a = max(lo, left - SWEEP);
+                w.range(a, left);
+                visited += left - a;
+                left = a;
+                const double ck = w.c64[0][left];
+                ldone = left <= lo || wave_all(!(w.q64[0] - ck < w.T));
+            }
+            if (!rdone) {
+                const int b = min(hi, right + SWEEP);
+                w.range(right, b);
+                visited += b - right;
+                right = b;
+                const double ck = w.c64[0][right - 1];
+                rdone = right >= hi || wave_all(!(ck - w.q64[0] < w.T));
+            }
+        }
+    }
+    const double eps = w.L[P - 1];
+    if (active) {
+        if (kd.eps >= 0) res64[kd.eps + i] = eps;
+        if (kd.eps_orig >= 0) res64[kd.eps_orig + A.perm[od.perm + i]] = eps;
+    }
+    pairs += (unsigned long long)(t1 - t0) * (unsigned long long)visited;
+    if (F32) rechecks += wave_sum_u64(w.rechecks);
+}
+
+// One grid-order item by per-lane walks (knn_walk).
+template <int D, int P>
+__device__ void walk_item(const KnnArgs& A, const Item& it, const double* __restrict__ col64,
+                          double* __restrict__ res64, unsigned long long& pairs) {
+    const KnnDev kd = A.descs[it.desc];
+    const OrderDev od = A.orders[kd.order];
+    const int64_t n = od.n;
+    const int i = it.t0 + (int)threadIdx.x;
+    const bool active = i < it.t1;
+    const int ncol = od.ncol;
+    KnnTile<D, P, false> w;
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+        const int c = d < ncol ? d : 0;
+        w.c64[d] = col64 + od.c64 + c * n;
+        w.q64[d] = active ? w.c64[d][i] : 0.0;
+    }
+    const int K1 = kd.k + 1;
+#pragma unroll
+    for (int t = 0; t < P; ++t) w.L[t] = (t < P - K1) ? -INFINITY : INFINITY;
+    w.T = active ? INFINITY : -INFINITY;
+    const unsigned v = knn_walk(w, kd, col64, it.lo, it.hi, it.t0, i, active);
+    const double eps = w.L[P - 1];
+    if (active) {
+        if (kd.eps >= 0) res64[kd.eps + i] = eps;
+        if (kd.eps_orig >= 0) res64[kd.eps_orig + A.perm[od.perm + i]] = eps;
+    }
+    pairs += wave_sum_u64(v);
+}
+
+template <int D, int P>
+__global__ __launch_bounds__(ENM_WAVE)
+void knn_walk_kernel(KnnArgs A, const double* __restrict__ col64, double* __restrict__ res64,
+                     unsigned long long* __restrict__ stats) {
+    unsigned long long pairs = 0;
+    for_each_wave_item(A.qoff, A.counter, A.chunk, [&](int idx) {
+        Item it = A.items[idx];
+        it.desc = uni(it.desc); it.tile = uni(it.tile);
+        it.lo = uni(it.lo); it.hi = uni(it.hi); it.t0 = uni(it.t0); it.t1 = uni(it.t1);
+        walk_item<D, P>(A, it, col64, res64, pairs);
+    });
+    if (threadIdx.x == 0) atomicAdd(stats + ST_KNN_PAIRS, pairs);
+}
+
+template <int D, int P, bool F32>
+__global__ __launch_bounds__(ENM_WAVE)
+void knn_kernel(KnnArgs A, const double* __restrict__ col64, const float* __restrict__ col32,
+                double* __restrict__ res64, unsigned long long* __restrict__ stats) {
+    // Counters accumulate per wave (the same value in every lane) and go
+    // out once at the end: one atomic per item on one address would queue
+    // a million atomics on a single L2 channel.
+    unsigned long long pairs = 0, rechecks = 0;
+    for_each_wave_item(A.qoff, A.counter, A.chunk, [&](int idx) {
+        Item it = A.items[idx];
+        it.desc = uni(it.desc); it.tile = uni(it.tile);
+        it.lo = uni(it.lo); it.hi = uni(it.hi); it.t0 = uni(it.t0); it.t1 = uni(it.t1);
+        if (F32) {
+            const KnnDev& kd = A.descs[it.desc];
+            bool ok = true;
+            (void)enm_guard(A.maxabs[A.orders[kd.order].inst], &ok);
+            if (ok) knn_item<D, P, true>(A, it, col64, col32, res64, pairs, rechecks);
+            else    knn_item<D, P, false>(A, it, col64, col32, res64, pairs, rechecks);
+        } else {
+            knn_item<D, P, false>(A, it, col64, col32, res64, pairs, rechecks);
+        }
+    });
+    if (threadIdx.x == 0) {
+        atomicAdd(stats + ST_KNN_PAIRS, pairs);
+        if (F32) atomicAdd(stats + ST_KNN_RECHECK, rechecks);
+    }
+}
+
+template <int D, int P>
+hipError_t launch_dp(const KnnArgs& a, bool f32, bool walk, int grid, const double* col64, const float* col32,
+                     double* res64, unsigned long long* stats, hipStream_t st) {
+    if constexpr (D >= 2) {
+        if (walk) {
+            hipLaunchKernelGGL((knn_walk_kernel<D, P>), dim3(grid), dim3(ENM_WAVE), 0, st, a, col64, res64, stats);
+            return hipGetLastError();
+        }
+    }
+    if (f32)
+        hipLaunchKernelGGL((knn_kernel<D, P, true>), dim3(grid), dim3(ENM_WAVE), 0, st,
+                           a, col64, col32, res64, stats);
+    else
+        hipLaunchKernelGGL((knn_kernel<D, P, false>), dim3(grid), dim3(ENM_WAVE), 0, st,
+                           a, col64, col32, res64, stats);
+    return hipGetLastError();
+}
+
+template <int D>
+hipError_t launch_d(const KnnArgs& a, int P, bool f32, bool walk, int grid, const double* col64,
+                    const float* col32, double* res64, unsigned long long* stats, hipStream_t st) {
+    switch (P) {
+        case 4: return launch_dp<D, 4>(a, f32, walk, grid, col64, col32, res64, stats, st);
+        case 8: return launch_dp<D, 8>(a, f32, walk, grid, col64, col32, res64, stats, st);
+        case 16: return launch_dp<D, 16>(a, f32, walk, grid, col64, col32, res64, stats, st);
+        case 32: return launch_dp<D, 32>(a, f32, walk, grid, col64, col32, res64, stats, st);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+}  // namespace
+
+#define ENM_KNN_DEFINE(D) \
+    hipError_t enm_knn_launch_d##D(const KnnArgs& a, int P, bool f32, bool walk, int grid, \
+                                   const double* col64, const float* col32, double* res64, \
+                                   unsigned long long* stats, hipStream_t st) { \
+        return launch_d<D>(a, P, f32, walk, grid, col64, col32, res64, stats, st); \
+    }
+#endif

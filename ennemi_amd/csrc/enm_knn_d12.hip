@@ -1,0 +1,5 @@
+// kNN kernels of joint dimension class 12 (see enm_knn.h).
+#define ENM_KNN_KERNELS
+#include "enm_knn.h"
+
+ENM_KNN_DEFINE(12)
