@@ -8,9 +8,10 @@ with NaNs at a random 0-30 % of rows; estimate_mi(y, x, lag=-48..48, k=3,
 drop_nan=True) = 776 KSG estimates with uneven n (about 70,000-100,000).
 
 A step is one run of all 776 estimates with the preprocessed inputs already
-resident in HBM: sort, gather, kNN, counts, digamma means (one enm_run_batch).
-Host validation and rescaling (the reference's NumPy operations) run once
-before the timed region and are reported separately as host_prep_s.
+resident in HBM: sort, gather, kNN, counts, digamma means (one enm_run_batch,
+replayed as one hipGraph).  Host validation and rescaling (the reference's
+NumPy operations) run once before the timed region and are reported
+separately as host_prep_s.
 
 The JSON line also carries
   roofline     the dominant kernel's algorithmic throughput against its peak,
@@ -18,9 +19,14 @@ The JSON line also carries
   comparators  the same batch as plain FP64 one-estimate-per-launch, batched
                FP64, and brute force (no window pruning);
   cpu_baseline the oracle's restatement of ennemi's CPU path (SciPy cKDTree
-               behind a thread pool, as ennemi runs it) on a sample of lags.
+               behind a thread pool of cpu_count() threads, as ennemi runs
+               it) on a sample of lags, with the host's CPU model and the
+               NumPy / SciPy versions;
+  configs      C1, C2, C4, C5a and C5b: ms per step (graph replays), stage
+               times and the kNN / count stages against the FP64 peak;
+  end_to_end   the public estimate_mi call of the headline, host work included.
 
-  python bench.py [--steps K] [--warmup W] [--no-cpu] [--no-compare]
+  python bench.py [--steps K] [--warmup W] [--no-cpu] [--no-compare] [--no-configs]
 """
 
 from __future__ import annotations
@@ -50,12 +56,11 @@ def log(*a):
 
 
 def time_batch(db, steps: int, warmup: int, stream):
-    """Run warmup + timed steps; return (ms/step wall, per-step stats list).
+    """Run warmup + timed steps; return (ms/step wall, stats of the last step).
 
     The timed steps are enqueued back to back (the stream orders them, so
     the host's launch work overlaps the device's), between two device
-    synchronisations.  The per-stage times come from a second pass of the
-    same number of steps, each read back after its own synchronisation."""
+    synchronisations."""
     import torch
     for _ in range(warmup):
         db.launch()
@@ -66,11 +71,17 @@ def time_batch(db, steps: int, warmup: int, stream):
         db.launch()
     torch.cuda.synchronize()
     wall = (time.perf_counter() - t0) * 1e3 / steps
+    return wall, db.stats()
+
+
+def stage_pass(db, steps: int):
+    """Per-stage device times (HIP events between the stages, eager
+    launches): `steps` steps, each read back after its own synchronisation."""
     stats = []
     for _ in range(steps):
         db.launch()
-        stats.append(db.stats())  # synchronises the stream, reads the stage events
-    return wall, stats
+        stats.append(db.stats())
+    return stats
 
 
 def config_batch(name: str):
@@ -86,7 +97,8 @@ def config_batch(name: str):
         return plan_mi(y, x, k=3)[0], "KSG N=10,000 rho=0.6 k=3"
     if name == "c2":
         x, y, z = wl.c2_cmi()
-        b = plan_mi(y, x, k=3, cond=z)[0]
+        # Host-prepared, so that the entropy instance joins the same batch.
+        b = plan_mi(y, x, k=3, cond=z, prep="host")[0]
         e, _ = wl.c2_entropy()
         b.add(nat.KIND_KL, 3, [np.ascontiguousarray(e[:, c]) for c in range(4)])
         return b, "CMI N=100,000 z 3-D k=3 + KL entropy 4-D N=100,000"
@@ -108,24 +120,87 @@ def config_batch(name: str):
     raise ValueError(name)
 
 
+# Element ops per pair evaluation (max-norm: D subtractions with the abs as
+# a free modifier, D-1 maxima, one compare; a count adds the accumulate).
+def knn_ops(d: int) -> int:
+    return 2 * d
+
+
+def count_ops(d: int) -> int:
+    return 2 * d + 1
+
+
+CONFIG_DIMS = {  # (kNN joint dimension, representative multi-D count dimension)
+    "c1": (2, 1), "c2": (5, 4), "c3": (2, 1), "c4": (2, 1), "c5a": (4, 3), "c5b": (3, 3),
+}
+
+
 def run_config(name: str, steps: int) -> dict:
+    """One configuration: graph-replayed batched steps (FP32 filter mode,
+    the default) timed back to back, the same in FP64 mode, and a stage pass
+    with events; kNN and count stages against the FP64 element-op peak."""
     import ennemi_amd._native as nat
     t0 = time.perf_counter()
     batch, desc = config_batch(name)
     prep = time.perf_counter() - t0
-    out = {"config": name, "workload": desc, "n_estimates": len(batch), "host_prep_s": prep}
+    out = {"workload": desc, "n_estimates": len(batch), "host_prep_s": round(prep, 3)}
     for mode, flags in (("fp32_filter", 0), ("fp64", nat.F_FP64)):
-        db = batch.device(flags | nat.F_TIMING)
-        w, st = time_batch(db, steps, 1, db.stream)
-        s = st[-1]
+        db = batch.device(flags | nat.F_GRAPH)
+        w, s = time_batch(db, steps, 1, db.stream)
         out[mode] = {"ms_per_step": round(w, 3), "estimates_per_s": len(batch) / (w * 1e-3),
-                     "stage_ms": {k: round(float(np.mean([x["stage_ms"][k] for x in st])), 3)
-                                  for k in nat.STAGES},
-                     "knn_pairs": s["knn_pairs"], "count_pairs": s["count_pairs"],
-                     "count_queries_1d": s["count_queries_1d"], "sort_keys": s["sort_keys"],
-                     "values": db.results()[:4].tolist()}
+                     "graph_replayed": s["graph_replayed"]}
+        if mode == "fp32_filter":
+            values = db.results()
+        else:
+            out[mode]["values_bitwise_equal_fp32_filter"] = bool(np.array_equal(db.results(), values))
         del db
+    db = batch.device(nat.F_TIMING)
+    st = stage_pass(db, max(2, min(steps, 3)))
+    del db
+    stage = {k: round(float(np.mean([x["stage_ms"][k] for x in st])), 4) for k in nat.STAGES}
+    s = st[-1]
+    dk, dc = CONFIG_DIMS[name]
+    knn_rate = s["knn_pairs"] * knn_ops(dk) / (stage["knn"] * 1e-3) / 1e12 if stage["knn"] > 0 else 0.0
+    cnt_rate = s["count_pairs"] * count_ops(dc) / (stage["count"] * 1e-3) / 1e12 \
+        if stage["count"] > 0 and s["count_pairs"] else 0.0
+    out["stage_ms"] = stage
+    out["work"] = {"knn_pairs": s["knn_pairs"], "knn_rechecks": s["knn_rechecks"],
+                   "count_pairs": s["count_pairs"], "count_rechecks": s["count_rechecks"],
+                   "count_queries_1d": s["count_queries_1d"], "sort_keys": s["sort_keys"],
+                   "n_launches": s["n_launches"]}
+    out["roofline"] = {
+        "knn": {"bound": "valu", "unit": "T ops/s", "ops_per_pair": knn_ops(dk), "achieved": knn_rate,
+                "peak": PEAK_FP64_TOPS, "frac": knn_rate / PEAK_FP64_TOPS},
+        "count": {"bound": "valu" if s["count_pairs"] else "hbm",
+                  "unit": "T ops/s" if s["count_pairs"] else "GB/s",
+                  "ops_per_pair": count_ops(dc) if s["count_pairs"] else None,
+                  "achieved": cnt_rate if s["count_pairs"] else
+                  s["count_queries_1d"] * 20.0 / max(stage["count"], 1e-9) / 1e6,
+                  "peak": PEAK_FP64_TOPS if s["count_pairs"] else PEAK_HBM_GBS},
+    }
+    rc = out["roofline"]["count"]
+    rc["frac"] = rc["achieved"] / rc["peak"]
     return out
+
+
+def cpu_info() -> dict:
+    """The host the CPU baseline ran on."""
+    import scipy
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for ln in f:
+                if ln.startswith("model name"):
+                    model = ln.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except AttributeError:
+        avail = None
+    return {"os_cpu_count": os.cpu_count(), "cpus_available": avail, "cpu_model": model,
+            "numpy": np.__version__, "scipy": scipy.__version__}
 
 
 def main() -> int:
@@ -136,9 +211,13 @@ def main() -> int:
     ap.add_argument("--length", type=int, default=100_096)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-compare", action="store_true")
+    ap.add_argument("--no-configs", action="store_true",
+                    help="skip the other configurations (C1, C2, C4, C5a, C5b) in the line")
+    ap.add_argument("--no-e2e", action="store_true", help="skip the end-to-end public-API timing")
     ap.add_argument("--cpu-lags", type=int, default=24, help="lags in the CPU baseline sample")
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--fp64", action="store_true", help="headline batch in FP64 mode (no FP32 filter)")
+    ap.add_argument("--eager", action="store_true", help="time eager launch sequences instead of graph replays")
     ap.add_argument("--tuning", default=None,
                     help="strip_points,waves_per_cu to use instead of the autotuner")
     ap.add_argument("--config", default=None,
@@ -153,12 +232,15 @@ def main() -> int:
         return 2
 
     import torch
+    import ennemi_amd
     import ennemi_amd._native as nat
     from ennemi_amd._driver import plan_mi
 
     if args.config:
         for name in args.config.split(","):
-            print(json.dumps(run_config(name, max(1, args.steps))), flush=True)
+            r = run_config(name, max(1, args.steps))
+            r["config"] = name
+            print(json.dumps(r), flush=True)
         return 0
 
     if args.copy_calib:
@@ -181,25 +263,34 @@ def main() -> int:
     ns = np.array([t[2] for t in batch._inst])
     log(f"planned {n_est} estimates, n in [{ns.min()}, {ns.max()}], host prep {host_prep_s:.2f} s")
 
-    # Headline: batched, FP32 filter + FP64 recheck, sorted-window pruning.
+    mode_flags = nat.F_FP64 if args.fp64 else 0
     t0 = time.perf_counter()
-    db = batch.device(nat.F_TIMING | (nat.F_FP64 if args.fp64 else 0))
+    tdb = batch.device(mode_flags | nat.F_TIMING)
     upload_s = time.perf_counter() - t0
     if args.tuning:
         sp, wv = (int(v) for v in args.tuning.split(","))
-        db.ctx.set_tuning(sp, wv)
-    tuning = db.ctx.get_tuning()
+        tdb.ctx.set_tuning(sp, wv)
+    tuning = tdb.ctx.get_tuning()
     if not (args.no_autotune or args.tuning):
         t0 = time.perf_counter()
-        tuning = db.autotune()
+        tuning = tdb.autotune()
         log(f"autotuned strip={tuning[0]} waves/CU={tuning[1]} in {time.perf_counter() - t0:.1f} s")
-    wall, stats = time_batch(db, args.steps, args.warmup, db.stream)
+
+    # Headline: the whole batch as one hipGraph replay per step.
+    if args.eager:
+        db = tdb
+    else:
+        db = batch.device(mode_flags | nat.F_GRAPH)
+    wall, st = time_batch(db, args.steps, args.warmup, db.stream)
     values = db.results()
-    st = stats[-1]
+    log(f"headline: {wall:.3f} ms/step ({'eager' if args.eager else 'graph replays'}); stats {st}")
+
+    # Stage times: eager launches with events between the stages.
+    stats = stage_pass(tdb, args.steps)
     stage_mean = {s: float(np.mean([x["stage_ms"][s] for x in stats])) for s in nat.STAGES}
     event_ms = float(np.mean([x["total_ms"] for x in stats]))
-    log(f"headline: {wall:.2f} ms/step wall, {event_ms:.2f} ms events; stages {stage_mean}")
-    log(f"stats {st}")
+    eager_values = tdb.results()
+    log(f"stage pass: {event_ms:.2f} ms events; stages {stage_mean}")
 
     # Roofline of the dominant kernel: the kNN walk (one launch per step for
     # this workload, timed by the events around the kNN stage).  It is
@@ -232,7 +323,8 @@ def main() -> int:
     # kernel (64 B per point for the two orders' columns and permutations);
     # otherwise a block pass and merge passes at 24 B per key each, then the
     # gather (64 B per point).  gather: the grid order's strip sort (24 B)
-    # and copy (44 B); count: 20 B per 1-D query.
+    # and copy (44 B); count: 20 B per 1-D query.  sort_16B: the sort stage
+    # against the plain 16 B per key (key read and written once).
     longest = int(ns.max())
     if 2048 < longest <= 1 << 20:
         sort_bytes = 2.0 * n_points * 44 + n_points * 64.0
@@ -250,26 +342,28 @@ def main() -> int:
     roofline["stages"] = {k: {"ms": round(stage_mean[k], 4),
                               "GB_per_s": stage_bytes[k] / (stage_mean[k] * 1e-3) / 1e9}
                           for k in stage_bytes}
+    roofline["stages"]["sort"]["GB_per_s_at_16B_per_key"] = \
+        st["sort_keys"] * 16.0 / (stage_mean["sort"] * 1e-3) / 1e9
     roofline["stage_ms"] = {k: round(v, 4) for k, v in stage_mean.items()}
 
     comparators = {}
     if not args.no_compare:
         for name, flags, steps in (
                 ("plain_fp64_per_instance", nat.F_FP64 | nat.F_PER_INSTANCE, 2),
-                ("batched_fp64", nat.F_FP64, 3),
+                ("batched_fp64_eager", nat.F_FP64, 3),
                 ("brute_batched_fp32", nat.F_BRUTE, 1),
                 ("brute_plain_fp64_per_instance", nat.F_BRUTE | nat.F_FP64 | nat.F_PER_INSTANCE, 1)):
             cdb = batch.device(flags | nat.F_TIMING)
-            w, cst = time_batch(cdb, steps, 1, cdb.stream)
+            w, _ = time_batch(cdb, steps, 1, cdb.stream)
+            cst = stage_pass(cdb, 1) if not (flags & nat.F_PER_INSTANCE) else []
             cvals = cdb.results()
             same = bool(np.array_equal(cvals, values))
-            s_last = cst[-1]
-            knn_ms = float(np.mean([x["stage_ms"]["knn"] for x in cst]))
             entry = {"ms_per_step": round(w, 3), "estimates_per_s": n_est / (w * 1e-3),
-                     "knn_pairs": s_last["knn_pairs"], "values_bitwise_equal": same}
-            if not (flags & nat.F_PER_INSTANCE):
-                ops = s_last["knn_pairs"] * 4
+                     "knn_pairs": cdb.stats()["knn_pairs"], "values_bitwise_equal": same}
+            if cst:
+                ops = cst[-1]["knn_pairs"] * 4
                 peak = PEAK_FP64_TOPS if flags & nat.F_FP64 else PEAK_FP32_TOPS
+                knn_ms = cst[-1]["stage_ms"]["knn"]
                 entry["knn_ms"] = round(knn_ms, 3)
                 entry["knn_valu_frac"] = ops / (knn_ms * 1e-3) / 1e12 / peak
             comparators[name] = entry
@@ -278,15 +372,35 @@ def main() -> int:
         base = comparators["plain_fp64_per_instance"]["ms_per_step"]
         comparators["speedup_vs_plain_fp64_per_instance"] = base / wall
 
+    # End to end: the public call (validation, windows, NaN drop, rescaling,
+    # upload, the batch, read-back), warm.
+    e2e = None
+    if not args.no_e2e:
+        ennemi_amd.estimate_mi(y, x, lag=lags, k=3, drop_nan=True)
+        t0 = time.perf_counter()
+        out = ennemi_amd.estimate_mi(y, x, lag=lags, k=3, drop_nan=True)
+        e2e_s = time.perf_counter() - t0
+        e2e = {"s": e2e_s, "estimates_per_s": n_est / e2e_s,
+               "values_equal_batch": bool(np.array_equal(out.ravel(), values))}
+        log("end to end", e2e)
+
+    configs = {}
+    if not args.no_configs:
+        for name in ("c1", "c2", "c4", "c5a", "c5b"):
+            t0 = time.perf_counter()
+            configs[name] = run_config(name, 3)
+            log(f"config {name}: {configs[name]['fp32_filter']['ms_per_step']} ms "
+                f"({time.perf_counter() - t0:.1f} s)")
+
     cpu = None
     if not args.no_cpu:
         # The reference's per-task path (_lagged_mi: window, NaN drop, rescale,
         # cKDTree estimator) on the first tasks of the same call, with the
         # call's full lag range fixing the window, fanned out on a thread pool
-        # as ennemi's _map_maybe_parallel does.
+        # of cpu_count() threads as ennemi's _map_maybe_parallel does.
         import concurrent.futures
         from oracle import ennemi_oracle as orc
-        threads = min(16, os.cpu_count() or 1)
+        threads = os.cpu_count() or 1
         n_cpu = args.cpu_lags * x.shape[1]
         params = [(x[:, v], y, int(lags[li]), int(lags.max()), int(lags.min()), 3, None, None,
                    np.zeros(1, dtype=int), False, False, True, True) for (li, v) in indices[:n_cpu]]
@@ -298,7 +412,8 @@ def main() -> int:
                "sample": f"the first {n_cpu} of the 776 tasks (lags {int(lags[0])}.."
                          f"{int(lags[args.cpu_lags - 1])} x 8 columns), end to end per task "
                          f"(window, NaN drop, rescaling, cKDTree), {cpu_s:.1f} s",
-               "max_abs_diff_vs_gpu": float(np.max(np.abs(values[:n_cpu] - cvals)))}
+               "max_abs_diff_vs_gpu": float(np.max(np.abs(values[:n_cpu] - cvals))),
+               "host": cpu_info()}
         log("cpu", cpu)
 
     line = {
@@ -317,17 +432,21 @@ def main() -> int:
         "config": {"workload": "masked lag scan: 8 coupled AR(1) series x 100,096, "
                                "lags -48..48, k=3, drop_nan (776 KSG estimates, N~1e5)",
                    "n_estimates": n_est, "n_min": int(ns.min()), "n_max": int(ns.max()),
-                   "mode": "one batched launch sequence: segmented sorts, strip-grid orders, "
-                           "per-lane FP64 strip walks for the kNN, LDS binary-search 1-D counts "
-                           "(FP32 filter + FP64 recheck only where it pays: brute force, off-grid "
-                           "multi-D counts)",
+                   "mode": "one hipGraph replay per step of the batched launch sequence: "
+                           "segmented sorts, strip-grid orders, per-lane FP64 strip walks for the "
+                           "kNN, LDS binary-search 1-D counts (FP32 filter + FP64 recheck only "
+                           "where it pays: brute force, off-grid multi-D counts)",
+                   "graph_replayed": st["graph_replayed"], "n_launches_eager": stats[-1]["n_launches"],
                    "tuning": {"strip_points": tuning[0], "waves_per_cu": tuning[1]}},
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "end_to_end": e2e,
         "comparators": comparators,
+        "configs": configs,
         "host_prep_s": host_prep_s,
         "upload_s": upload_s,
         "event_ms_per_step": event_ms,
+        "graph_values_equal_eager": bool(np.array_equal(values, eager_values)),
     }
     print(json.dumps(line), flush=True)
     return 0

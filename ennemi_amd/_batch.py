@@ -111,7 +111,8 @@ class DeviceBatch:
         dev = torch.device("cuda", self.ctx.device)
         self.stream = _stream()
         with torch.cuda.stream(self.stream):
-            self.arena = torch.from_numpy(arena).to(dev)
+            # A batch prepared on the device brings its arena along.
+            self.arena = batch.device_arena if arena is None else torch.from_numpy(arena).to(dev)
             self.out = torch.empty(self.n_inst, dtype=torch.float64, device=dev)
             self.offs = batch.point_offsets()
             total = int(self.offs[-1])
@@ -154,6 +155,47 @@ class DeviceBatch:
             eps.append(eps_all[a:b].copy())
             cnts.append(cnt_all[3 * a:3 * b].reshape(3, n).copy())
         return values, eps, cnts
+
+
+class ArenaBatch:
+    """A batch whose float64 arena was written on the device (the columns
+    of device-prepared estimate_mi tasks, enm_prep_write); instances name
+    their columns by arena offset.  Same interface as Batch."""
+
+    def __init__(self, device_arena) -> None:
+        self.device_arena = device_arena
+        self._inst: List[tuple] = []
+
+    def __len__(self) -> int:
+        return len(self._inst)
+
+    def add(self, kind: int, k: int, n: int, offsets: Sequence[int], dx: int, dy: int, dz: int) -> int:
+        self._inst.append((kind, int(k), int(n), dx, dy, dz, 1, 0, [int(o) for o in offsets]))
+        return len(self._inst) - 1
+
+    def pack(self):
+        n_inst = len(self._inst)
+        inst_arr = (nat.EnmInstance * n_inst)()
+        for i, (kind, k, n, dx, dy, dz, n_segs, seg_index, offs) in enumerate(self._inst):
+            e = inst_arr[i]
+            e.kind, e.k, e.n, e.dx, e.dy, e.dz = kind, k, n, dx, dy, dz
+            e.n_segs, e.seg_index = n_segs, seg_index
+            for t, o in enumerate(offs):
+                e.col[t] = o
+        seg_arr = (ctypes.c_int64 * 1)(0)
+        return None, inst_arr, seg_arr
+
+    def point_offsets(self) -> np.ndarray:
+        n = np.array([t[2] for t in self._inst], dtype=np.int64)
+        return np.concatenate([[0], np.cumsum(n)])
+
+    def device(self, flags: int = 0, export: bool = False):
+        return DeviceBatch(self, flags, export)
+
+    def run(self, flags: int = 0, export: bool = False):
+        db = self.device(flags, export)
+        db.launch()
+        return db.results()
 
 
 _side_stream = None

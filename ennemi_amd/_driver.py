@@ -362,10 +362,12 @@ def _estimate_mi(y, x, lag, k, cond, cond_lag, mask, discrete_x, discrete_y, pre
 
 def plan_mi(y, x, lag=0, *, k: int = 3, cond=None, cond_lag=0, mask=None,
             discrete_y: bool = False, discrete_x: bool = False, preprocess: bool = True,
-            drop_nan: bool = False):
+            drop_nan: bool = False, prep: str = "auto"):
     """Validate and rescale every task of an ``estimate_mi`` call and return
     its batch of GPU instances without running it (for timing the device
-    work alone).  Returns (batch, task indices [(lag index, x column)])."""
+    work alone).  Returns (batch, task indices [(lag index, x column)]).
+    prep="host" prepares the tasks with NumPy on the host even where the
+    device could (for comparing the two)."""
     x_arr = np.asarray(x)
     y_arr = np.asarray(y)
     cond_arr, ncond = (np.column_stack((np.asarray(cond),)), None) if cond is not None else (None, 1)
@@ -375,13 +377,14 @@ def plan_mi(y, x, lag=0, *, k: int = 3, cond=None, cond_lag=0, mask=None,
     cond_lag_arr = np.broadcast_to(np.column_stack((cond_lag,)), (lag_arr.shape[0], ncond))
     mask_arr = np.asarray(mask) if mask is not None else None
     tasks, indices, _ = _plan_mi(y_arr, x_arr, lag_arr, k, cond_arr, cond_lag_arr, mask_arr,
-                                 discrete_x, discrete_y, preprocess, drop_nan)
+                                 discrete_x, discrete_y, preprocess, drop_nan, prep)
     if tasks.host_values:
         raise ValueError("plan_mi covers the GPU estimators only (not discrete-discrete tasks)")
     return tasks.batch, indices
 
 
-def _plan_mi(y, x, lag, k, cond, cond_lag, mask, discrete_x, discrete_y, preprocess, drop_nan):
+def _plan_mi(y, x, lag, k, cond, cond_lag, mask, discrete_x, discrete_y, preprocess, drop_nan,
+             prep: str = "auto"):
     _check_parameters(x, y, k, cond, mask)
     min_lag = min(np.min(lag), np.min(cond_lag))
     max_lag = max(np.max(lag), np.max(cond_lag))
@@ -390,6 +393,9 @@ def _plan_mi(y, x, lag, k, cond, cond_lag, mask, discrete_x, discrete_y, preproc
     nvar = 1 if x.ndim == 1 else x.shape[1]
 
     indices = list(itertools.product(range(len(lag)), range(nvar)))
+    if prep != "host" and _device_prep_eligible(x, y, cond, discrete_x, discrete_y):
+        return _plan_mi_device(y, x, lag, k, cond, cond_lag, mask, preprocess, drop_nan,
+                               min_lag, max_lag, indices), indices, nvar
     tasks = _Tasks()
     noise = _Noise()
     for (li, v) in indices:
@@ -399,6 +405,101 @@ def _plan_mi(y, x, lag, k, cond, cond_lag, mask, discrete_x, discrete_y, preproc
         xs, ys, zs = _prepare_lagged(params, noise)
         _add_mi_task(tasks, xs, ys, zs, k, discrete_x, discrete_y)
     return tasks, indices, nvar
+
+
+def _device_prep_eligible(x, y, cond, discrete_x: bool, discrete_y: bool) -> bool:
+    """Tasks whose windows, masks, NaN drop and rescaling run on the device
+    (enm_prep_count / enm_prep_write): continuous float64 (or integer or
+    boolean, which NumPy computes with in float64) x, y and condition."""
+    if discrete_x or discrete_y:
+        return False
+    for a in (x, y) if cond is None else (x, y, cond):
+        if a.dtype.kind not in "fiub" or (a.dtype.kind == "f" and a.dtype != np.float64):
+            return False
+    if cond is not None and cond.shape[1] > nat.PREP_MAXZ:
+        return False
+    # Without a usable GPU the host prepares (and validates) the tasks; the
+    # estimates themselves then fail with NativeUnavailable.
+    try:
+        nat.context()
+    except nat.NativeUnavailable:
+        return False
+    return True
+
+
+def _plan_mi_device(y, x, lag, k, cond, cond_lag, mask, preprocess, drop_nan, min_lag, max_lag,
+                    indices):
+    """_plan_mi with each task prepared on the device: the raw series go up
+    once, the kept rows of every task are counted there (the host validates
+    with those counts, in task order, with the reference's messages), and
+    the compacted, rescaled columns are written straight into the batch's
+    arena."""
+    import torch
+    from ._batch import ArenaBatch, _stream
+    ctx = nat.context()
+    dev = torch.device("cuda", ctx.device)
+    st = _stream()
+    n_rows = len(y)
+    nvar = 1 if x.ndim == 1 else x.shape[1]
+    dz = 0 if cond is None else cond.shape[1]
+    series = [y] + ([x] if x.ndim == 1 else [x[:, v] for v in range(nvar)]) + \
+        ([] if cond is None else [cond[:, c] for c in range(dz)])
+    raw = np.empty(n_rows * len(series), dtype=np.float64)
+    for i, col in enumerate(series):
+        raw[i * n_rows:(i + 1) * n_rows] = col
+    min_lag = min(min_lag, 0)
+    max_lag = max(max_lag, 0)
+    w = n_rows - max_lag + min_lag
+    n_tasks = len(indices)
+    tasks = (nat.EnmPrepTask * n_tasks)()
+    for t, (li, v) in enumerate(indices):
+        T = tasks[t]
+        T.x_col, T.y_col = (1 + v) * n_rows, 0
+        T.x0, T.y0, T.w, T.dz = max_lag - int(lag[li]), max_lag, w, dz
+        for c in range(dz):
+            T.z_col[c] = (1 + nvar + c) * n_rows
+            T.z0[c] = max_lag - int(cond_lag[li][c])
+    flags = (nat.P_DROP_NAN if drop_nan else 0) | (nat.P_RESCALE if preprocess else 0)
+    h_n = np.zeros(n_tasks, dtype=np.int64)
+    h_nan = np.zeros(n_tasks, dtype=np.uint32)
+    with torch.cuda.stream(st):
+        d_raw = torch.from_numpy(raw).to(dev, non_blocking=False)
+        d_mask = torch.from_numpy(np.ascontiguousarray(mask, dtype=np.uint8)).to(dev) \
+            if mask is not None else None
+    with ctx.lock:
+        ctx.prep_count(tasks, n_tasks, d_raw.data_ptr(), d_mask.data_ptr() if d_mask is not None else 0,
+                       flags, h_n, h_nan, st.cuda_stream)
+    for t in range(n_tasks):
+        if h_n[t] <= k:
+            raise ValueError(K_TOO_LARGE_MSG)
+        if h_nan[t]:
+            raise ValueError(NANS_LEFT_MSG)
+    per = h_n * (2 + dz)
+    arena_off = np.concatenate([[0], np.cumsum(per)[:-1]]).astype(np.int64)
+    noise_len = int(per.max()) if preprocess else 0
+    with torch.cuda.stream(st):
+        arena = torch.empty(int(per.sum()), dtype=torch.float64, device=dev)
+        d_const = torch.zeros(n_tasks, dtype=torch.int32, device=dev)
+        d_noise = torch.from_numpy(np.random.default_rng(NOISE_SEED).normal(0.0, 1e-10, noise_len)).to(dev) \
+            if preprocess else None
+    with ctx.lock:
+        ctx.prep_write(tasks, n_tasks, d_raw.data_ptr(), d_mask.data_ptr() if d_mask is not None else 0,
+                       d_noise.data_ptr() if d_noise is not None else 0, noise_len, flags, arena_off,
+                       arena.data_ptr(), d_const.data_ptr(), st.cuda_stream)
+    st.synchronize()
+    const = d_const.cpu().numpy() if preprocess else np.zeros(n_tasks, dtype=np.int32)
+    for t in range(n_tasks):
+        for bit in (1, 2, 4):
+            if const[t] & bit and (bit != 4 or dz > 0):
+                warn(PREPROCESS_CONSTANT_DATA_WARNING)
+    tasks_out = _Tasks()
+    tasks_out.batch = ArenaBatch(arena)
+    kind = nat.KIND_KSG if dz == 0 else nat.KIND_CMI
+    for t in range(n_tasks):
+        n = int(h_n[t])
+        offs = [int(arena_off[t]) + c * n for c in range(2 + dz)]
+        tasks_out.gpu(tasks_out.batch.add(kind, k, n, offs, 1, 1, dz))
+    return tasks_out
 
 
 # ---------------------------------------------------------------------------

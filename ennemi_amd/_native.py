@@ -29,7 +29,8 @@ F_TIMING = 0x10
 STAGES = ("fill", "sort", "gather", "knn", "count", "combine")
 
 EXPORTED_SYMBOLS = ("enm_init", "enm_shutdown", "enm_last_error", "enm_run_batch",
-                    "enm_get_stats", "enm_autotune", "enm_set_tuning", "enm_get_tuning")
+                    "enm_prep_count", "enm_prep_write", "enm_unique_rows", "enm_get_stats",
+                    "enm_autotune", "enm_set_tuning", "enm_get_tuning")
 
 
 class NativeUnavailable(RuntimeError):
@@ -41,6 +42,26 @@ class EnmInstance(ctypes.Structure):
                 ("dx", ctypes.c_int32), ("dy", ctypes.c_int32), ("dz", ctypes.c_int32),
                 ("n_segs", ctypes.c_int32), ("seg_index", ctypes.c_int64),
                 ("col", ctypes.c_int64 * MAX_COLS)]
+
+
+PREP_MAXZ = MAX_COLS - 2
+P_DROP_NAN = 0x1
+P_RESCALE = 0x2
+
+
+class EnmPrepTask(ctypes.Structure):
+    _fields_ = [("x_col", ctypes.c_int64), ("y_col", ctypes.c_int64),
+                ("x0", ctypes.c_int64), ("y0", ctypes.c_int64), ("w", ctypes.c_int64),
+                ("dz", ctypes.c_int32), ("pad", ctypes.c_int32),
+                ("z_col", ctypes.c_int64 * PREP_MAXZ), ("z0", ctypes.c_int64 * PREP_MAXZ)]
+
+
+class EnmRowset(ctypes.Structure):
+    _fields_ = [("n", ctypes.c_int64), ("ncol", ctypes.c_int32), ("pad", ctypes.c_int32),
+                ("col", ctypes.c_int64 * MAX_COLS)]
+
+
+E_UNSUPPORTED = -4
 
 
 class EnmStats(ctypes.Structure):
@@ -75,6 +96,14 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
                                   ctypes.POINTER(ctypes.c_int64), vp, vp, vp, vp,
                                   ctypes.c_uint32, vp]
     lib.enm_run_batch.restype = ctypes.c_int
+    lib.enm_prep_count.argtypes = [vp, ctypes.POINTER(EnmPrepTask), ctypes.c_int32, vp, vp,
+                                   ctypes.c_uint32, vp, vp, vp]
+    lib.enm_prep_count.restype = ctypes.c_int
+    lib.enm_prep_write.argtypes = [vp, ctypes.POINTER(EnmPrepTask), ctypes.c_int32, vp, vp, vp,
+                                   ctypes.c_int64, ctypes.c_uint32, vp, vp, vp, vp]
+    lib.enm_prep_write.restype = ctypes.c_int
+    lib.enm_unique_rows.argtypes = [vp, ctypes.POINTER(EnmRowset), ctypes.c_int32, vp, vp, vp, vp, vp]
+    lib.enm_unique_rows.restype = ctypes.c_int
     lib.enm_get_stats.argtypes = [vp, ctypes.POINTER(EnmStats)]
     lib.enm_get_stats.restype = ctypes.c_int
     lib.enm_autotune.argtypes = [vp, ctypes.POINTER(EnmInstance), ctypes.c_int32,
@@ -117,6 +146,30 @@ class Context:
         rc = self.lib.enm_run_batch(self.ctx, inst_arr, n_inst, seg_arr, arena_ptr, out_ptr,
                                     eps_ptr or None, cnt_ptr or None, flags, stream_ptr or None)
         self.check(rc, "enm_run_batch")
+
+    def prep_count(self, tasks, n_tasks: int, raw_ptr: int, mask_ptr: int, flags: int,
+                   h_n: np.ndarray, h_nan: np.ndarray, stream_ptr: int = 0) -> None:
+        rc = self.lib.enm_prep_count(self.ctx, tasks, n_tasks, raw_ptr, mask_ptr or None, flags,
+                                     h_n.ctypes.data, h_nan.ctypes.data, stream_ptr or None)
+        self.check(rc, "enm_prep_count")
+
+    def prep_write(self, tasks, n_tasks: int, raw_ptr: int, mask_ptr: int, noise_ptr: int,
+                   noise_len: int, flags: int, arena_off: np.ndarray, arena_ptr: int,
+                   const_ptr: int, stream_ptr: int = 0) -> None:
+        rc = self.lib.enm_prep_write(self.ctx, tasks, n_tasks, raw_ptr, mask_ptr or None,
+                                     noise_ptr or None, noise_len, flags, arena_off.ctypes.data,
+                                     arena_ptr, const_ptr, stream_ptr or None)
+        self.check(rc, "enm_prep_write")
+
+    def unique_rows(self, sets, n_sets: int, values_ptr: int, codes_ptr: int, counts_ptr: int,
+                    h_distinct: np.ndarray, stream_ptr: int = 0) -> int:
+        """enm_unique_rows; returns its status (0, or E_UNSUPPORTED when a row
+        set's keys would not be exact), raising on any other failure."""
+        rc = self.lib.enm_unique_rows(self.ctx, sets, n_sets, values_ptr, codes_ptr, counts_ptr,
+                                      h_distinct.ctypes.data, stream_ptr or None)
+        if rc != E_UNSUPPORTED:
+            self.check(rc, "enm_unique_rows")
+        return rc
 
     def stats(self) -> dict:
         st = EnmStats()

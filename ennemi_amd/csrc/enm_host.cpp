@@ -38,17 +38,32 @@ namespace {
 
 std::string g_init_error;
 
+// A device buffer: owned (hipMalloc) or borrowed from the static workspace.
 struct DevBuf {
     void* p = nullptr;
     size_t bytes = 0;
-    ~DevBuf() { if (p) (void)hipFree(p); }
+    bool borrowed = false;
+    ~DevBuf() { release(); }
+    void release() {
+        if (p && !borrowed) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+        borrowed = false;
+    }
     hipError_t ensure(size_t b) {
-        if (b <= bytes && p) return hipSuccess;
-        if (p) { (void)hipFree(p); p = nullptr; bytes = 0; }
+        if (b <= bytes && p && !borrowed) return hipSuccess;
+        release();
         if (b == 0) b = 256;
         hipError_t e = hipMalloc(&p, b);
         if (e == hipSuccess) bytes = b;
+        else p = nullptr;
         return e;
+    }
+    void borrow(void* q, size_t b) {
+        release();
+        p = q;
+        bytes = b;
+        borrowed = true;
     }
     template <class T> T* as() const { return reinterpret_cast<T*>(p); }
 };
@@ -65,6 +80,7 @@ struct Group {
     int qoff;       // first of its ENM_NQ + 1 queue offsets in the offset table
     int inst;       // instance (per-instance mode) or -1
     int walk;       // kNN items on grid orders (per-lane walk kernel)
+    int k1max;      // largest k+1 of its kNN items (sizes heap lists)
 };
 
 struct ExportSpace {
@@ -124,8 +140,9 @@ struct Plan {
     // multi-D counts off the grid (band filter).  Only then are they made.
     bool need32 = false;
     // device
+    int64_t heap_bytes = 0;   // global heap lists of kNN groups with k+1 > ENM_LIST_LDS_MAX
     DevBuf tables, sk0, si0, sk1, si1, split, col64, col32, perm, res64, res32, maxabs, counters, stats,
-        deferred, samp_k0, samp_k1, samp_i0, samp_i1, samp_split, bid, ss_counts, ss_tiles, ss_ctl;
+        deferred, heap, samp_k0, samp_k1, samp_i0, samp_i1, samp_split, bid, ss_counts, ss_tiles, ss_ctl;
     size_t off_segs = 0, off_gitems = 0, off_orders = 0, off_kdescs = 0, off_cdescs = 0, off_insts = 0,
            off_cls = 0, off_kitems = 0, off_citems = 0, off_segs2 = 0, off_gitems2 = 0, off_rdescs = 0,
            off_ritems = 0, off_sitems = 0, off_qoffs = 0, off_ssegs = 0, off_samp_segs = 0, off_cons = 0,
@@ -155,6 +172,24 @@ struct enm_ctx {
     std::list<std::unique_ptr<Plan>> cache;   // most recent first
     Plan* last = nullptr;
     hipStream_t last_stream = nullptr;
+    // Device-side task preparation: the state enm_prep_count leaves for
+    // enm_prep_write, and their workspace.
+    struct Prep {
+        std::vector<PrepTaskDev> tasks;
+        std::vector<int64_t> blk0, n;
+        int64_t n_blocks = 0;
+        const void* raw = nullptr;
+        const void* mask = nullptr;
+        uint32_t drop_nan = 0;
+        bool counted = false;
+        std::vector<char> blob1, blob2;   // host tables of the last uploads
+        DevBuf tables1, tables2, blk_count, blk_off, nan, nout, chunk_val, stats;
+    } prep;
+    // Workspace of enm_unique_rows.
+    struct Unique {
+        DevBuf tables, k0, k1, i0, i1, split, ccodes, cstarts, ccounts, cdist, rkeys, rstarts, rdist;
+        std::vector<char> blob1, blob2;
+    } uq;
 };
 
 namespace {
@@ -346,7 +381,13 @@ struct PendingItem {
     int inst;
     int dclass, pclass;
     int walk = 0;   // kNN on a grid order: the per-lane walk kernel
+    int k1 = 0;     // k+1 of its instance (kNN items)
 };
+
+int grid_for(enm_ctx* ctx, int n_items) {
+    const int g = ctx->num_cus * ctx->waves_per_cu;
+    return std::max(1, std::min(n_items, g));
+}
 
 int plan_batch(enm_ctx* ctx, Plan& P, const enm_instance* inst, int n_inst, const int64_t* seg_len) {
     Planner pl(P, inst, seg_len);
@@ -493,7 +534,7 @@ int plan_batch(enm_ctx* ctx, Plan& P, const enm_instance* inst, int n_inst, cons
         const int dcl = enm_knn_dim_class(D);
         const int pcl = enm_knn_list_class(I.k + 1);
         if (dcl < 0) return bad("joint dimension above 16");
-        if (pcl < 0) return fail(ctx, ENM_E_UNSUPPORTED, "k above 31 is not supported by the register lists");
+        if (I.k > 0x3fffffff) return bad("k out of range");
         int ktiles = 0;
         std::vector<PendingItem> mine;
         for (auto rg : pl.ranges(K)) {
@@ -508,6 +549,7 @@ int plan_batch(enm_ctx* ctx, Plan& P, const enm_instance* inst, int n_inst, cons
                 // Per-lane walks on two-column grids; with more columns the
                 // gridded two prune less and tiles share candidates.
                 pi.inst = ii; pi.dclass = dcl; pi.pclass = pcl; pi.walk = grid && D == 2 ? 1 : 0;
+                pi.k1 = I.k + 1;
                 mine.push_back(pi);
             }
         }
@@ -635,6 +677,12 @@ int plan_batch(enm_ctx* ctx, Plan& P, const enm_instance* inst, int n_inst, cons
             g.counter = (int)P.groups.size() * (ENM_NQ + ENM_NDEF) * ENM_CSTRIDE;
             g.inst = P.per_instance ? pend[i].inst : -1;
             g.walk = pend[i].walk;
+            g.k1max = 1;
+            for (size_t t = i; t < j; ++t) g.k1max = std::max(g.k1max, pend[t].k1);
+            if (!is_count && g.pclass == ENM_LIST_HEAP_GLOBAL) {
+                const int hg = enm_knn_heap_grid(grid_for(ctx, g.n_items), g.k1max);
+                P.heap_bytes = std::max<int64_t>(P.heap_bytes, (int64_t)hg * ENM_TILE * g.k1max * 8);
+            }
             const size_t blk = std::max<size_t>(64, (j - i + 4 * ENM_NQ - 1) / (4 * ENM_NQ));
             struct Block { int64_t cost; size_t a, b; };
             std::vector<Block> blocks;
@@ -756,7 +804,8 @@ size_t append(std::vector<char>& blob, const std::vector<T>& v) {
     return off;
 }
 
-int allocate_and_upload(enm_ctx* ctx, Plan& P, hipStream_t st) {
+// The plan's device tables, packed into one host blob (256-byte aligned).
+std::vector<char> pack_tables(Plan& P) {
     std::vector<char> blob;
     P.off_segs = append(blob, P.segs);
     P.off_gitems = append(blob, P.gitems);
@@ -778,6 +827,53 @@ int allocate_and_upload(enm_ctx* ctx, Plan& P, hipStream_t st) {
     P.off_cons = append(blob, P.cons);
     P.off_qbase = append(blob, P.qbase);
     P.table_bytes = blob.size();
+    return blob;
+}
+
+struct BufSpec {
+    DevBuf* buf;
+    size_t bytes;
+    const char* what;
+};
+
+// Every device buffer of a plan and its size in bytes.
+std::vector<BufSpec> workspace_layout(Plan& P) {
+    std::vector<BufSpec> v;
+    const int64_t se = std::max(P.sort_elems, P.sort_elems2);
+    // FP32 column copies exist only when a kernel reads them.
+    const bool with32 = P.f32 && P.need32;
+    v.push_back({&P.tables, P.table_bytes, "tables"});
+    v.push_back({&P.sk0, (size_t)se * 8, "sort keys"});
+    v.push_back({&P.sk1, (size_t)se * 8, "sort keys"});
+    v.push_back({&P.si0, (size_t)se * 4, "sort index"});
+    v.push_back({&P.si1, (size_t)se * 4, "sort index"});
+    v.push_back({&P.split, (size_t)std::max(P.ntiles, P.ntiles2) * 4 + 4, "merge splits"});
+    v.push_back({&P.col64, (size_t)P.n_col64 * 8, "columns"});
+    v.push_back({&P.col32, with32 ? (size_t)P.n_col32 * 4 : 256, "FP32 columns"});
+    v.push_back({&P.perm, (size_t)P.n_perm * 4, "permutations"});
+    v.push_back({&P.res64, (size_t)P.n_res64 * 8, "radii"});
+    v.push_back({&P.res32, (size_t)P.n_res32 * 4, "counts"});
+    v.push_back({&P.maxabs, (size_t)P.n_inst * 8, "maxima"});
+    v.push_back({&P.counters, P.groups.size() * (ENM_NQ + ENM_NDEF) * ENM_CSTRIDE * 4 + 4, "counters"});
+    v.push_back({&P.deferred, (size_t)P.deferred_seg * ENM_NDEF * 4, "deferred queries"});
+    v.push_back({&P.stats, ST_N * 8, "stats"});
+    if (P.heap_bytes > 0) v.push_back({&P.heap, (size_t)P.heap_bytes, "heap lists"});
+    if (P.sample_sort) {
+        v.push_back({&P.samp_k0, (size_t)P.n_samp * 8 + 8, "samples"});
+        v.push_back({&P.samp_k1, (size_t)P.n_samp * 8 + 8, "samples"});
+        v.push_back({&P.samp_i0, (size_t)P.n_samp * 4 + 4, "samples"});
+        v.push_back({&P.samp_i1, (size_t)P.n_samp * 4 + 4, "samples"});
+        v.push_back({&P.samp_split, (size_t)P.samp_tiles * 4 + 4, "sample splits"});
+        v.push_back({&P.bid, (size_t)P.sort_elems * 2 + 2, "buckets"});
+        v.push_back({&P.ss_counts, (size_t)P.n_counts * 4 + 4, "bucket counts"});
+        v.push_back({&P.ss_tiles, (size_t)P.n_buckets * sizeof(SsTile), "bucket runs"});
+        v.push_back({&P.ss_ctl, 2 * ENM_NQ * ENM_CSTRIDE * 4, "run queue counters"});
+    }
+    return v;
+}
+
+int allocate_and_upload(enm_ctx* ctx, Plan& P, hipStream_t st) {
+    std::vector<char> blob = pack_tables(P);
     hipError_t e;
     if (P.upload_done) {
         e = hipEventSynchronize(P.upload_done);
@@ -791,38 +887,11 @@ int allocate_and_upload(enm_ctx* ctx, Plan& P, hipStream_t st) {
     if (e != hipSuccess) return hip_fail(ctx, e, "hipHostMalloc");
     std::memcpy(P.pinned, blob.data(), P.table_bytes);
 
-    auto need = [&](DevBuf& b, size_t bytes, const char* what) -> int {
-        hipError_t er = b.ensure(bytes);
-        if (er != hipSuccess) return fail(ctx, ENM_E_NOMEM, std::string("hipMalloc failed for ") + what);
-        return ENM_OK;
-    };
-    int rc;
-    if ((rc = need(P.tables, P.table_bytes, "tables"))) return rc;
-    const int64_t se = std::max(P.sort_elems, P.sort_elems2);
-    if ((rc = need(P.sk0, se * 8, "sort keys"))) return rc;
-    if ((rc = need(P.sk1, se * 8, "sort keys"))) return rc;
-    if ((rc = need(P.si0, se * 4, "sort index"))) return rc;
-    if ((rc = need(P.si1, se * 4, "sort index"))) return rc;
-    if ((rc = need(P.split, (size_t)std::max(P.ntiles, P.ntiles2) * 4 + 4, "merge splits"))) return rc;
-    if ((rc = need(P.col64, P.n_col64 * 8, "columns"))) return rc;
-    if ((rc = need(P.col32, P.n_col32 * 4, "columns"))) return rc;
-    if ((rc = need(P.perm, P.n_perm * 4, "permutations"))) return rc;
-    if ((rc = need(P.res64, P.n_res64 * 8, "radii"))) return rc;
-    if ((rc = need(P.res32, P.n_res32 * 4, "counts"))) return rc;
-    if ((rc = need(P.maxabs, (size_t)P.n_inst * 8, "maxima"))) return rc;
-    if ((rc = need(P.counters, P.groups.size() * (ENM_NQ + ENM_NDEF) * ENM_CSTRIDE * 4 + 4, "counters"))) return rc;
-    if ((rc = need(P.deferred, (size_t)P.deferred_seg * ENM_NDEF * 4, "deferred queries"))) return rc;
-    if ((rc = need(P.stats, ST_N * 8, "stats"))) return rc;
-    if (P.sample_sort) {
-        if ((rc = need(P.samp_k0, P.n_samp * 8 + 8, "samples"))) return rc;
-        if ((rc = need(P.samp_k1, P.n_samp * 8 + 8, "samples"))) return rc;
-        if ((rc = need(P.samp_i0, P.n_samp * 4 + 4, "samples"))) return rc;
-        if ((rc = need(P.samp_i1, P.n_samp * 4 + 4, "samples"))) return rc;
-        if ((rc = need(P.samp_split, (size_t)P.samp_tiles * 4 + 4, "sample splits"))) return rc;
-        if ((rc = need(P.bid, P.sort_elems * 2 + 2, "buckets"))) return rc;
-        if ((rc = need(P.ss_counts, P.n_counts * 4 + 4, "bucket counts"))) return rc;
-        if ((rc = need(P.ss_tiles, P.n_buckets * sizeof(SsTile), "bucket runs"))) return rc;
-        if ((rc = need(P.ss_ctl, 2 * ENM_NQ * ENM_CSTRIDE * 4, "run queue counters"))) return rc;
+    for (const BufSpec& b : workspace_layout(P)) {
+        if (b.buf->ensure(b.bytes) != hipSuccess) {
+            (void)hipGetLastError();
+            return fail(ctx, ENM_E_NOMEM, std::string("hipMalloc failed for ") + b.what);
+        }
     }
     e = hipMemcpyAsync(P.tables.p, P.pinned, P.table_bytes, hipMemcpyHostToDevice, st);
     if (e != hipSuccess) return hip_fail(ctx, e, "hipMemcpyAsync(tables)");
@@ -834,11 +903,6 @@ int allocate_and_upload(enm_ctx* ctx, Plan& P, hipStream_t st) {
 template <class T>
 const T* tab(const Plan& P, size_t off) {
     return reinterpret_cast<const T*>(P.tables.as<char>() + off);
-}
-
-int grid_for(enm_ctx* ctx, int n_items) {
-    const int g = ctx->num_cus * ctx->waves_per_cu;
-    return std::max(1, std::min(n_items, g));
 }
 
 // Enqueue the whole launch sequence of a plan on `st`.  With `timing`, an
@@ -962,7 +1026,10 @@ int enqueue(enm_ctx* ctx, Plan& P, const double* arena, double* out, double* d_e
                                counters + g.counter,
                                tab<KnnDev>(P, P.off_kdescs), orders, maxabs, P.perm.as<int>(),
                                P.col64.as<double>(), P.col32.as<float>(), P.res64.as<double>(), stats,
-                               g.dclass, g.pclass, P.f32 && P.brute, P.brute, g.walk, grid_for(ctx, g.n_items), st);
+                               g.dclass, g.pclass, P.f32 && P.brute, P.brute, g.walk,
+                               g.pclass == ENM_LIST_HEAP_GLOBAL ? enm_knn_heap_grid(grid_for(ctx, g.n_items), g.k1max)
+                                                                 : grid_for(ctx, g.n_items),
+                               g.k1max, P.heap.as<double>(), st);
             ++nl;
             return chk(e, "knn");
         }
@@ -1182,6 +1249,332 @@ int enm_run_batch(enm_ctx* ctx, const enm_instance* inst, int32_t n_inst, const 
     }
     ctx->last = up.get();
     ctx->cache.push_front(std::move(up));
+    return ENM_OK;
+}
+
+int enm_prep_count(enm_ctx* ctx, const enm_prep_task* tasks, int32_t n_tasks, const double* d_raw,
+                   const uint8_t* d_mask, uint32_t flags, int64_t* h_n, uint32_t* h_nan, void* stream) {
+    if (!ctx) { g_init_error = "ctx is NULL"; return ENM_E_ARG; }
+    ctx->err.clear();
+    auto& S = ctx->prep;
+    S.counted = false;
+    if (!tasks || n_tasks <= 0) return fail(ctx, ENM_E_ARG, "no tasks");
+    if (!d_raw || !h_n || !h_nan) return fail(ctx, ENM_E_ARG, "d_raw, h_n and h_nan are required");
+    hipError_t e = hipSetDevice(ctx->device);
+    if (e != hipSuccess) return hip_fail(ctx, e, "hipSetDevice");
+    hipStream_t st = (hipStream_t)stream;
+    S.tasks.assign(n_tasks, PrepTaskDev{});
+    S.blk0.assign(n_tasks, 0);
+    int64_t nb = 0;
+    for (int t = 0; t < n_tasks; ++t) {
+        const enm_prep_task& T = tasks[t];
+        if (T.w < 1 || T.w > 0x7fffffffLL || T.dz < 0 || T.dz > ENM_PREP_MAXZ || T.x0 < 0 || T.y0 < 0) {
+            char msg[96];
+            std::snprintf(msg, sizeof msg, "task %d: bad window or condition columns", t);
+            return fail(ctx, ENM_E_ARG, msg);
+        }
+        PrepTaskDev& D = S.tasks[t];
+        std::memset(&D, 0, sizeof D);
+        D.x_col = T.x_col; D.y_col = T.y_col; D.x0 = T.x0; D.y0 = T.y0; D.w = T.w; D.dz = T.dz;
+        for (int c = 0; c < T.dz; ++c) { D.z_col[c] = T.z_col[c]; D.z0[c] = T.z0[c]; }
+        D.nblk = (int32_t)((T.w + ENM_PREP_BLOCK - 1) / ENM_PREP_BLOCK);
+        S.blk0[t] = nb;
+        nb += D.nblk;
+    }
+    S.n_blocks = nb;
+    S.blob1.clear();
+    const size_t off_t = append(S.blob1, S.tasks);
+    const size_t off_b = append(S.blob1, S.blk0);
+    if (S.tables1.ensure(S.blob1.size()) != hipSuccess || S.blk_count.ensure(nb * 4 + 4) != hipSuccess ||
+        S.blk_off.ensure(nb * 8 + 8) != hipSuccess || S.nan.ensure((size_t)n_tasks * 4) != hipSuccess ||
+        S.nout.ensure((size_t)n_tasks * 8) != hipSuccess) {
+        (void)hipGetLastError();
+        return fail(ctx, ENM_E_NOMEM, "hipMalloc failed for the preparation workspace");
+    }
+    e = hipMemcpyAsync(S.tables1.p, S.blob1.data(), S.blob1.size(), hipMemcpyHostToDevice, st);
+    if (e != hipSuccess) return hip_fail(ctx, e, "hipMemcpyAsync(prep tables)");
+    const PrepTaskDev* dt = reinterpret_cast<const PrepTaskDev*>(S.tables1.as<char>() + off_t);
+    const int64_t* db0 = reinterpret_cast<const int64_t*>(S.tables1.as<char>() + off_b);
+    e = enm_launch_prep_count(dt, db0, n_tasks, nb, d_raw, d_mask, (flags & ENM_P_DROP_NAN) ? 1 : 0,
+                              S.blk_count.as<int>(), S.blk_off.as<int64_t>(), S.nan.as<unsigned>(),
+                              S.nout.as<int64_t>(), st);
+    if (e != hipSuccess) return hip_fail(ctx, e, "prep count");
+    e = hipMemcpyAsync(h_n, S.nout.p, (size_t)n_tasks * 8, hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(h_nan, S.nan.p, (size_t)n_tasks * 4, hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e != hipSuccess) return hip_fail(ctx, e, "prep count read-back");
+    S.n.assign(h_n, h_n + n_tasks);
+    S.raw = d_raw;
+    S.mask = d_mask;
+    S.drop_nan = flags & ENM_P_DROP_NAN;
+    S.counted = true;
+    return ENM_OK;
+}
+
+int enm_prep_write(enm_ctx* ctx, const enm_prep_task* tasks, int32_t n_tasks, const double* d_raw,
+                   const uint8_t* d_mask, const double* d_noise, int64_t noise_len, uint32_t flags,
+                   const int64_t* arena_off, double* d_arena, uint32_t* d_const, void* stream) {
+    if (!ctx) { g_init_error = "ctx is NULL"; return ENM_E_ARG; }
+    ctx->err.clear();
+    auto& S = ctx->prep;
+    if (!S.counted || (int)S.tasks.size() != n_tasks || S.raw != d_raw || S.mask != d_mask ||
+        S.drop_nan != (flags & ENM_P_DROP_NAN))
+        return fail(ctx, ENM_E_ARG, "enm_prep_write must follow enm_prep_count of the same tasks and inputs");
+    if (!tasks || !arena_off || !d_arena || !d_const) return fail(ctx, ENM_E_ARG, "missing argument");
+    const bool rescale = (flags & ENM_P_RESCALE) != 0;
+    hipError_t e = hipSetDevice(ctx->device);
+    if (e != hipSuccess) return hip_fail(ctx, e, "hipSetDevice");
+    hipStream_t st = (hipStream_t)stream;
+    std::vector<PrepSumDev> sums;
+    std::vector<int> chunk_sum_of, chunk_index;
+    std::vector<int64_t> sblk0(n_tasks, 0);
+    int64_t nsb = 0, need_noise = 0;
+    int stat = 0;
+    for (int t = 0; t < n_tasks; ++t) {
+        const enm_prep_task& T = tasks[t];
+        PrepTaskDev& D = S.tasks[t];
+        if (T.x_col != D.x_col || T.y_col != D.y_col || T.x0 != D.x0 || T.y0 != D.y0 || T.w != D.w || T.dz != D.dz)
+            return fail(ctx, ENM_E_ARG, "enm_prep_write must follow enm_prep_count of the same tasks");
+        D.arena_off = arena_off[t];
+        D.n = S.n[t];
+        D.stat = stat;
+        D.flags = 0;
+        const int64_t n = D.n;
+        need_noise = std::max<int64_t>(need_noise, n * (2 + D.dz));
+        sblk0[t] = nsb;
+        nsb += (n + 255) / 256;
+        // x, y and a one-column condition: pairwise sums; the columns of a
+        // multi-column condition: row-by-row sums.
+        const int narr = 2 + D.dz;
+        for (int a = 0; a < narr; ++a) {
+            PrepSumDev sd;
+            std::memset(&sd, 0, sizeof sd);
+            sd.src = D.arena_off + a * n;
+            sd.n = n;
+            sd.stat = stat + 2 * a;
+            sd.sequential = (a >= 2 && D.dz >= 2) ? 1 : 0;
+            sd.chunk0 = (int32_t)chunk_sum_of.size();
+            sd.n_chunks = 0;
+            if (!sd.sequential && n > 0) {
+                sd.n_chunks = (int32_t)((n + 8191) / 8192);
+                for (int c = 0; c < sd.n_chunks; ++c) {
+                    chunk_sum_of.push_back((int)sums.size());
+                    chunk_index.push_back(c);
+                }
+            }
+            sums.push_back(sd);
+        }
+        stat += 2 * narr;
+    }
+    if (rescale && (!d_noise || noise_len < need_noise)) {
+        char msg[128];
+        std::snprintf(msg, sizeof msg, "the noise holds %lld values, the tasks draw up to %lld",
+                      (long long)noise_len, (long long)need_noise);
+        return fail(ctx, ENM_E_ARG, msg);
+    }
+    S.blob2.clear();
+    const size_t o_t = append(S.blob2, S.tasks);
+    const size_t o_b = append(S.blob2, S.blk0);
+    const size_t o_s = append(S.blob2, sums);
+    const size_t o_cs = append(S.blob2, chunk_sum_of);
+    const size_t o_ci = append(S.blob2, chunk_index);
+    const size_t o_sb = append(S.blob2, sblk0);
+    if (S.tables2.ensure(S.blob2.size()) != hipSuccess ||
+        S.chunk_val.ensure(chunk_sum_of.size() * 8 + 8) != hipSuccess ||
+        S.stats.ensure((size_t)stat * 8 + 8) != hipSuccess) {
+        (void)hipGetLastError();
+        return fail(ctx, ENM_E_NOMEM, "hipMalloc failed for the preparation workspace");
+    }
+    e = hipMemcpyAsync(S.tables2.p, S.blob2.data(), S.blob2.size(), hipMemcpyHostToDevice, st);
+    if (e != hipSuccess) return hip_fail(ctx, e, "hipMemcpyAsync(prep tables)");
+    char* base = S.tables2.as<char>();
+    e = enm_launch_prep_write(reinterpret_cast<PrepTaskDev*>(base + o_t), reinterpret_cast<const int64_t*>(base + o_b),
+                              n_tasks, S.n_blocks, d_raw, d_mask, S.drop_nan ? 1 : 0, rescale ? 1 : 0,
+                              S.blk_off.as<int64_t>(), reinterpret_cast<const PrepSumDev*>(base + o_s),
+                              (int)sums.size(), reinterpret_cast<const int*>(base + o_cs),
+                              reinterpret_cast<const int*>(base + o_ci), (int)chunk_sum_of.size(),
+                              S.chunk_val.as<double>(), S.stats.as<double>(),
+                              reinterpret_cast<const int64_t*>(base + o_sb), nsb, d_noise, d_const, d_arena, st);
+    if (e != hipSuccess) return hip_fail(ctx, e, "prep write");
+    if (!rescale) {
+        e = enm_launch_fill_i32(reinterpret_cast<int*>(d_const), n_tasks, 0, st);
+        if (e != hipSuccess) return hip_fail(ctx, e, "prep write");
+    }
+    S.counted = false;
+    return ENM_OK;
+}
+
+int enm_unique_rows(enm_ctx* ctx, const enm_rowset* sets, int32_t n_sets, const double* d_values,
+                    int32_t* d_codes, int64_t* d_counts, int64_t* h_distinct, void* stream) {
+    if (!ctx) { g_init_error = "ctx is NULL"; return ENM_E_ARG; }
+    ctx->err.clear();
+    if (!sets || n_sets <= 0 || !d_values || !d_codes || !d_counts || !h_distinct)
+        return fail(ctx, ENM_E_ARG, "missing argument");
+    hipError_t e = hipSetDevice(ctx->device);
+    if (e != hipSuccess) return hip_fail(ctx, e, "hipSetDevice");
+    hipStream_t st = (hipStream_t)stream;
+    auto& U = ctx->uq;
+    // Distinct columns, each sorted once.
+    std::map<std::pair<int64_t, int64_t>, int> colid;
+    std::vector<SortSeg> segs1;
+    std::vector<UqSeg> useg1;
+    std::vector<int64_t> set_off(n_sets);
+    int64_t tot1 = 0, tot2 = 0, maxlen1 = 0, maxlen2 = 0, out_tot = 0;
+    int tiles1 = 0;
+    for (int s = 0; s < n_sets; ++s) {
+        const enm_rowset& R = sets[s];
+        if (R.n < 1 || R.n > 0x7fffffffLL || R.ncol < 1 || R.ncol > ENM_MAX_COLS)
+            return fail(ctx, ENM_E_ARG, "bad row set");
+        set_off[s] = out_tot;
+        out_tot += R.n;
+        if (R.ncol > 1) { tot2 += R.n; maxlen2 = std::max(maxlen2, R.n); }
+        for (int c = 0; c < R.ncol; ++c) {
+            auto key = std::make_pair((int64_t)R.col[c], (int64_t)R.n);
+            if (colid.count(key)) continue;
+            const int id = (int)segs1.size();
+            colid[key] = id;
+            SortSeg sg;
+            sg.src = R.col[c];
+            sg.sbase = tot1;
+            sg.len = (int32_t)R.n;
+            sg.first_tile = tiles1;
+            sg.p0 = 0;
+            sg.pad = 0;
+            tiles1 += (int)((R.n + ENM_SORT_TILE - 1) / ENM_SORT_TILE);
+            UqSeg u;
+            std::memset(&u, 0, sizeof u);
+            u.sbase = tot1;
+            u.n = R.n;
+            u.code_off = tot1;
+            u.start_off = tot1 + id;   // n + 1 slots each
+            u.count_off = tot1;
+            u.slot = id;
+            segs1.push_back(sg);
+            useg1.push_back(u);
+            tot1 += R.n;
+            maxlen1 = std::max(maxlen1, R.n);
+        }
+    }
+    const int ncols = (int)segs1.size();
+    const int64_t sort_n = std::max(tot1, tot2);
+    const int tiles_max = std::max<int>(tiles1, (int)((tot2 + ENM_SORT_TILE - 1) / ENM_SORT_TILE) + n_sets);
+    auto ensure_all = [&]() -> bool {
+        return U.k0.ensure(sort_n * 8) == hipSuccess && U.k1.ensure(sort_n * 8) == hipSuccess &&
+               U.i0.ensure(sort_n * 4) == hipSuccess && U.i1.ensure(sort_n * 4) == hipSuccess &&
+               U.split.ensure((size_t)tiles_max * 4 + 4) == hipSuccess &&
+               U.ccodes.ensure(tot1 * 4) == hipSuccess && U.cstarts.ensure((tot1 + ncols) * 8) == hipSuccess &&
+               U.ccounts.ensure(tot1 * 8) == hipSuccess && U.cdist.ensure((size_t)ncols * 8) == hipSuccess &&
+               U.rkeys.ensure(tot2 * 8 + 8) == hipSuccess && U.rstarts.ensure((tot2 + n_sets) * 8 + 8) == hipSuccess &&
+               U.rdist.ensure((size_t)n_sets * 8) == hipSuccess;
+    };
+    if (!ensure_all()) {
+        (void)hipGetLastError();
+        return fail(ctx, ENM_E_NOMEM, "hipMalloc failed for the distinct-row workspace");
+    }
+    U.blob1.clear();
+    const size_t o_s1 = append(U.blob1, segs1), o_u1 = append(U.blob1, useg1);
+    if (U.tables.ensure(U.blob1.size()) != hipSuccess) return fail(ctx, ENM_E_NOMEM, "hipMalloc failed for tables");
+    e = hipMemcpyAsync(U.tables.p, U.blob1.data(), U.blob1.size(), hipMemcpyHostToDevice, st);
+    if (e != hipSuccess) return hip_fail(ctx, e, "hipMemcpyAsync(tables)");
+    int fb = 0, nl = 0;
+    e = enm_launch_sort(reinterpret_cast<const SortSeg*>(U.tables.as<char>() + o_s1), ncols, tiles1, maxlen1,
+                        ENM_SORT_TILE, d_values, nullptr, U.k0.as<double>(), U.i0.as<int>(), U.k1.as<double>(),
+                        U.i1.as<int>(), U.split.as<int>(), &fb, &nl, st);
+    if (e == hipSuccess)
+        e = enm_launch_uq_codes(reinterpret_cast<const UqSeg*>(U.tables.as<char>() + o_u1), ncols,
+                                fb ? U.k1.as<double>() : U.k0.as<double>(), fb ? U.i1.as<int>() : U.i0.as<int>(),
+                                U.ccodes.as<int>(), U.cstarts.as<int64_t>(), U.ccounts.as<int64_t>(),
+                                U.cdist.as<int64_t>(), st);
+    std::vector<int64_t> cdist(ncols);
+    if (e == hipSuccess) e = hipMemcpyAsync(cdist.data(), U.cdist.p, (size_t)ncols * 8, hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e != hipSuccess) return hip_fail(ctx, e, "distinct values of the columns");
+    // Multi-column row sets: mixed-radix keys, sorted and numbered.
+    std::vector<UqRowKey> rks;
+    std::vector<SortSeg> segs2;
+    std::vector<UqSeg> useg2;
+    std::vector<int> multi_set;
+    int64_t koff = 0;
+    int tiles2 = 0;
+    for (int s = 0; s < n_sets; ++s) {
+        const enm_rowset& R = sets[s];
+        if (R.ncol == 1) continue;
+        UqRowKey rk;
+        std::memset(&rk, 0, sizeof rk);
+        rk.n = R.n;
+        rk.key_off = koff;
+        rk.ncol = R.ncol;
+        double prod = 1.0;
+        for (int c = 0; c < R.ncol; ++c) {
+            const int id = colid[std::make_pair((int64_t)R.col[c], (int64_t)R.n)];
+            rk.code_off[c] = useg1[id].code_off;
+            rk.radix[c] = cdist[id];
+            prod *= (double)cdist[id];
+        }
+        if (prod > 9007199254740992.0)
+            return fail(ctx, ENM_E_UNSUPPORTED, "too many distinct value combinations for float64 row keys");
+        SortSeg sg;
+        sg.src = koff;
+        sg.sbase = koff;
+        sg.len = (int32_t)R.n;
+        sg.first_tile = tiles2;
+        sg.p0 = 0;
+        sg.pad = 0;
+        tiles2 += (int)((R.n + ENM_SORT_TILE - 1) / ENM_SORT_TILE);
+        UqSeg u;
+        std::memset(&u, 0, sizeof u);
+        u.sbase = koff;
+        u.n = R.n;
+        u.code_off = set_off[s];
+        u.start_off = koff + (int64_t)useg2.size();
+        u.count_off = set_off[s];
+        u.slot = (int32_t)useg2.size();
+        rks.push_back(rk);
+        segs2.push_back(sg);
+        useg2.push_back(u);
+        multi_set.push_back(s);
+        koff += R.n;
+    }
+    std::vector<int64_t> rdist(useg2.size());
+    if (!rks.empty()) {
+        U.blob2.clear();
+        const size_t o_r = append(U.blob2, rks), o_s2 = append(U.blob2, segs2), o_u2 = append(U.blob2, useg2);
+        if (U.tables.ensure(std::max(U.blob1.size(), U.blob2.size())) != hipSuccess)
+            return fail(ctx, ENM_E_NOMEM, "hipMalloc failed for tables");
+        e = hipMemcpyAsync(U.tables.p, U.blob2.data(), U.blob2.size(), hipMemcpyHostToDevice, st);
+        const char* tb = U.tables.as<char>();
+        if (e == hipSuccess)
+            e = enm_launch_uq_keys(reinterpret_cast<const UqRowKey*>(tb + o_r), (int)rks.size(), maxlen2,
+                                   U.ccodes.as<int>(), U.rkeys.as<double>(), st);
+        fb = 0;
+        if (e == hipSuccess)
+            e = enm_launch_sort(reinterpret_cast<const SortSeg*>(tb + o_s2), (int)segs2.size(), tiles2, maxlen2,
+                                ENM_SORT_TILE, U.rkeys.as<double>(), nullptr, U.k0.as<double>(), U.i0.as<int>(),
+                                U.k1.as<double>(), U.i1.as<int>(), U.split.as<int>(), &fb, &nl, st);
+        if (e == hipSuccess)
+            e = enm_launch_uq_codes(reinterpret_cast<const UqSeg*>(tb + o_u2), (int)useg2.size(),
+                                    fb ? U.k1.as<double>() : U.k0.as<double>(), fb ? U.i1.as<int>() : U.i0.as<int>(),
+                                    d_codes, U.rstarts.as<int64_t>(), d_counts, U.rdist.as<int64_t>(), st);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(rdist.data(), U.rdist.p, rdist.size() * 8, hipMemcpyDeviceToHost, st);
+        if (e != hipSuccess) return hip_fail(ctx, e, "distinct rows");
+    }
+    // Single-column row sets take their column's codes and counts.
+    for (int s = 0; s < n_sets; ++s) {
+        const enm_rowset& R = sets[s];
+        if (R.ncol != 1) continue;
+        const int id = colid[std::make_pair((int64_t)R.col[0], (int64_t)R.n)];
+        e = hipMemcpyAsync(d_codes + set_off[s], U.ccodes.as<int>() + useg1[id].code_off, (size_t)R.n * 4,
+                           hipMemcpyDeviceToDevice, st);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(d_counts + set_off[s], U.ccounts.as<int64_t>() + useg1[id].count_off,
+                               (size_t)cdist[id] * 8, hipMemcpyDeviceToDevice, st);
+        if (e != hipSuccess) return hip_fail(ctx, e, "hipMemcpyAsync(codes)");
+        h_distinct[s] = cdist[id];
+    }
+    e = hipStreamSynchronize(st);
+    if (e != hipSuccess) return hip_fail(ctx, e, "hipStreamSynchronize");
+    for (size_t t = 0; t < multi_set.size(); ++t) h_distinct[multi_set[t]] = rdist[t];
     return ENM_OK;
 }
 

@@ -35,6 +35,13 @@
 #define ENM_SORT_TILE 2048     // elements per block-sort tile and merge chunk
 #define ENM_SORT_THREADS 256
 #define ENM_MAX_SPACES 3
+// kNN list classes beyond the register lists of 4..32 slots: a per-lane
+// max-heap of k+1 values in LDS (k+1 <= ENM_LIST_LDS_MAX) or in a global
+// buffer (any larger k+1).
+#define ENM_LIST_HEAP_LDS 0
+#define ENM_LIST_HEAP_GLOBAL -1
+#define ENM_LIST_LDS_MAX 128
+#define ENM_HEAP_GLOBAL_BYTES (256ll << 20)  // most global heap storage of one launch
 
 // One contiguous key range to sort: keys at arena[src + t], t in [0, len),
 // carried with their instance point index p0 + t.
@@ -194,3 +201,50 @@ struct InstDev {
 
 // Device counters (enm_stats order).
 enum { ST_KNN_PAIRS = 0, ST_KNN_RECHECK, ST_CNT_PAIRS, ST_CNT_RECHECK, ST_CNT_1D, ST_SORT_KEYS, ST_N };
+
+// Device-side preparation of estimate_mi tasks (enm_prep_count / enm_prep_write).
+#define ENM_PREP_BLOCK 2048     // window rows per compaction block
+#define ENM_PREP_MAXZ 14        // condition columns of one task (ENM_MAX_COLS - 2)
+
+struct PrepTaskDev {
+    int64_t x_col, y_col;       // raw offsets of the x and y series
+    int64_t x0, y0;             // first row of each window (the mask is aligned with y)
+    int64_t w;                  // window rows
+    int64_t z_col[ENM_PREP_MAXZ], z0[ENM_PREP_MAXZ];
+    int32_t dz, nblk;           // condition columns; compaction blocks
+    int64_t arena_off;          // arena offset of its x column (then y, then z columns, n each)
+    int64_t n;                  // kept rows
+    int64_t stat;               // its first stats slot: mean, std of x, y, then each z column
+    int64_t noise_x, noise_y, noise_z;   // noise offsets (set on the device)
+    uint32_t flags, pad;        // bit 0/1/2: x/y/z constant (not rescaled)
+};
+
+// One array whose mean and standard deviation NumPy's way are needed.
+struct PrepSumDev {
+    int64_t src;        // arena offset
+    int64_t n;
+    int32_t stat;       // stats slot of the mean (the std follows)
+    int32_t sequential; // 1: accumulated row by row (a column of a multi-column condition)
+    int32_t chunk0;     // pairwise arrays: first of its 8192-value chunks
+    int32_t n_chunks;
+};
+
+// Distinct rows (enm_unique_rows): one sorted segment of keys.
+struct UqSeg {
+    int64_t sbase;      // sort-buffer position of its first key
+    int64_t n;
+    int64_t code_off;   // codes of its points, by point index
+    int64_t start_off;  // run starts (n + 1 slots)
+    int64_t count_off;  // points per code
+    int32_t slot;       // its distinct-count slot
+    int32_t pad;
+};
+
+// The mixed-radix keys of a multi-column row set.
+struct UqRowKey {
+    int64_t n;
+    int64_t key_off;            // where its keys go
+    int32_t ncol, pad;
+    int64_t code_off[16];       // each column's codes
+    int64_t radix[16];          // each column's distinct count
+};

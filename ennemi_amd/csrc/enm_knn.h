@@ -60,6 +60,8 @@ struct KnnArgs {
     const unsigned long long* maxabs;
     const int* perm;
     int brute;
+    double* heap;       // global heap lists (list class ENM_LIST_HEAP_GLOBAL): per block
+    int64_t heap_stride;  // doubles per block in `heap` (64 lanes x the group's largest k+1)
 };
 
 // Launch of the kNN kernels of one joint dimension D (one translation unit
@@ -79,13 +81,79 @@ constexpr int SWEEP = 64; // candidates per side per sweep step
 
 
 
+// The K1 = k+1 smallest distances of a lane.  Register lists (P = 4, 8,
+// 16, 32 slots) keep them sorted ascending, the lowest P-K1 slots pinned at
+// -inf, so the last slot is the K1-th smallest.  For K1 above 32 a max-heap
+// of exactly K1 values per lane (initially +inf) keeps the same multiset:
+// a value below the root replaces it and sinks, so the root is the K1-th
+// smallest.  The heap lives in LDS (P = ENM_LIST_HEAP_LDS, up to
+// ENM_LIST_LDS_MAX values per lane) or in a global buffer
+// (P = ENM_LIST_HEAP_GLOBAL), lane-interleaved: slot t of a lane at
+// h[t * 64].  Either way top() is the current radius.
+template <int P>
+struct RegList {
+    double L[P];
+    __device__ __forceinline__ void init(int K1, double*) {
+#pragma unroll
+        for (int t = 0; t < P; ++t) L[t] = (t < P - K1) ? -INFINITY : INFINITY;
+    }
+    __device__ __forceinline__ void insert(double v) { list_insert<P>(L, v); }
+    __device__ __forceinline__ double top() const { return L[P - 1]; }
+};
+
+struct HeapList {
+    double* h;
+    int K1;
+    double root;
+    __device__ __forceinline__ void init(int k1, double* base) {
+        h = base;
+        K1 = k1;
+        for (int t = 0; t < K1; ++t) h[t * ENM_WAVE] = INFINITY;
+        root = INFINITY;
+    }
+    __device__ __forceinline__ void insert(double v) {
+        if (!(v < root)) return;
+        int i = 0;
+        for (;;) {
+            int c = 2 * i + 1;
+            if (c >= K1) break;
+            double cv = h[c * ENM_WAVE];
+            if (c + 1 < K1) {
+                const double c2 = h[(c + 1) * ENM_WAVE];
+                if (c2 > cv) { ++c; cv = c2; }
+            }
+            if (!(cv > v)) break;
+            h[i * ENM_WAVE] = cv;
+            i = c;
+        }
+        h[i * ENM_WAVE] = v;
+        root = h[0];
+    }
+    __device__ __forceinline__ double top() const { return root; }
+};
+
+template <int P> struct ListOf { using type = RegList<P>; };
+template <> struct ListOf<ENM_LIST_HEAP_LDS> { using type = HeapList; };
+template <> struct ListOf<ENM_LIST_HEAP_GLOBAL> { using type = HeapList; };
+
+// Dynamic LDS of the heap kernels (64 lanes x K1 doubles).
+extern __shared__ double enm_heap_lds[];
+
+// This lane's heap storage (list classes with a heap), else nullptr.
+template <int P>
+__device__ __forceinline__ double* heap_base(const KnnArgs& A) {
+    if constexpr (P == ENM_LIST_HEAP_LDS) return enm_heap_lds + threadIdx.x;
+    else if constexpr (P == ENM_LIST_HEAP_GLOBAL) return A.heap + blockIdx.x * A.heap_stride + threadIdx.x;
+    else return nullptr;
+}
+
 template <int D, int P, bool F32>
 struct KnnTile {
     const double* c64[D];
     const float* c32[D];
     double q64[D];
     float q32[D];
-    double L[P];
+    typename ListOf<P>::type L;
     double T;
     float thr;
     double guard;
@@ -102,8 +170,8 @@ struct KnnTile {
         for (int d = 1; d < D; ++d) e = fmax(e, fabs(c64[d][j] - q64[d]));
         ++rechecks;
         if (e < T) {
-            list_insert<P>(L, e);
-            T = L[P - 1];
+            L.insert(e);
+            T = L.top();
             set_thr();
         }
     }
@@ -143,13 +211,13 @@ struct KnnTile {
         for (int c = 1; c < CH; ++c) mn = fmin(mn, dd[c]);
         if (!wave_any(mn < T)) return false;
         bool merged = false;
-        if constexpr (P <= 8) {
+        if constexpr (P > 0 && P <= 8) {
             // After a chunk that improved some list (dense regime) most of
             // the 8 candidates matter to some lane: one network merge.
             if (dense) {
                 sort8(dd);
-                list_merge8<P>(L, dd);
-                T = L[P - 1];
+                list_merge8<P>(L.L, dd);
+                T = L.top();
                 merged = true;
             }
         }
@@ -158,8 +226,8 @@ struct KnnTile {
 #pragma unroll
             for (int c = 0; c < CH; ++c) {
                 if (wave_any(dd[c] < T) && dd[c] < T) {
-                    list_insert<P>(L, dd[c]);
-                    T = L[P - 1];
+                    L.insert(dd[c]);
+                    T = L.top();
                 }
             }
         }
@@ -212,8 +280,8 @@ struct KnnTile {
 #pragma unroll
             for (int d = 1; d < D; ++d) m = fmax(m, fabs(c64[d][j] - q64[d]));
             if (m < T) {
-                list_insert<P>(L, m);
-                T = L[P - 1];
+                L.insert(m);
+                T = L.top();
             }
         }
     }
@@ -264,6 +332,37 @@ __device__ __forceinline__ int lane_lower_bound(const double* __restrict__ v, in
     return first;
 }
 
+// Every live lane's lower bound lane_lower_bound(v, ks, ke, y), found by
+// the whole wave: the bounds [a, b) of the live lanes' y range in the strip
+// (wave_bounds, two rounds of independent probes for a 1024-point strip),
+// then, when that window holds at most 64 keys (the usual case: a wave's
+// 64 queries are consecutive in y, and so are their entry points), each
+// lane counts the window keys below its y by a binary search over the
+// window held one key per lane (shuffles, no memory round trips).  Wider
+// windows fall back to a per-lane search inside the window.  The result is
+// the exact lower bound of every live lane; other lanes get ks.
+__device__ __forceinline__ int wave_lower_bounds(const double* __restrict__ v, int ks, int ke, double y, bool live) {
+    const double lo = uni_f64(wave_min(live ? y : INFINITY));
+    const double hi = uni_f64(wave_max(live ? y : -INFINITY));
+    if (!(lo <= hi)) return ks;
+    int a, b;
+    wave_bounds(v, ks, ke, lo, hi, a, b);
+    const int L = b - a;
+    const int lane = (int)(threadIdx.x & (ENM_WAVE - 1));
+    if (L <= ENM_WAVE) {
+        const double key = lane < L ? v[a + lane] : INFINITY;
+        int pos = 0;
+#pragma unroll
+        for (int step = ENM_WAVE; step > 0; step >>= 1) {
+            const int idx = pos + step - 1;
+            const double kv = __shfl(key, idx < ENM_WAVE ? idx : ENM_WAVE - 1, ENM_WAVE);
+            if (idx < L && kv < y) pos += step;
+        }
+        return live ? a + pos : ks;
+    }
+    return live ? lane_lower_bound(v, a, b, y) : ks;
+}
+
 // Walk of one strip [ks, ke) (sorted by column 1) for every live lane, from
 // its own start: upward from u and downward from dn, one candidate per
 // direction and iteration, so candidates come in about increasing y
@@ -301,8 +400,8 @@ __device__ __forceinline__ void walk_strip(KnnTile<D, P, F32>& w, int ks, int ke
 #pragma unroll
             for (int d = 0; d < D; ++d)
                 if (d != 1) m = fmax(m, fabs(cu[d] - w.q64[d]));
-            list_insert<P>(w.L, up ? m : INFINITY);
-            w.T = w.L[P - 1];
+            w.L.insert(up ? m : INFINITY);
+            w.T = w.L.top();
             visited += up;
         }
         {
@@ -312,8 +411,8 @@ __device__ __forceinline__ void walk_strip(KnnTile<D, P, F32>& w, int ks, int ke
 #pragma unroll
             for (int d = 0; d < D; ++d)
                 if (d != 1) m = fmax(m, fabs(cd[d] - w.q64[d]));
-            list_insert<P>(w.L, down ? m : INFINITY);
-            w.T = w.L[P - 1];
+            w.L.insert(down ? m : INFINITY);
+            w.T = w.L.top();
             visited += down;
         }
         ++u;
@@ -344,8 +443,8 @@ __device__ unsigned knn_walk(KnnTile<D, P, F32>& w, const KnnDev& kd, const doub
     const int nstrip = (hi - lo + S - 1) / S;
     unsigned visited = active ? 1u : 0u;
     if (active) {
-        list_insert<P>(w.L, 0.0);  // the query itself
-        w.T = w.L[P - 1];
+        w.L.insert(0.0);  // the query itself
+        w.T = w.L.top();
     }
     {
         const int ss = lo + s * S, se = min(hi, ss + S);
@@ -362,7 +461,7 @@ __device__ unsigned knn_walk(KnnTile<D, P, F32>& w, const KnnDev& kd, const doub
                 const bool need = active && w.q64[0] - xk[ke - 1] < w.T;
                 ldone = !wave_any(need);
                 if (!ldone) {
-                    const int p = lane_lower_bound(w.c64[1], ks, ke, w.q64[1]);
+                    const int p = wave_lower_bounds(w.c64[1], ks, ke, w.q64[1], need);
                     walk_strip(w, ks, ke, p, p - 1, need, visited);
                 }
             }
@@ -376,7 +475,7 @@ __device__ unsigned knn_walk(KnnTile<D, P, F32>& w, const KnnDev& kd, const doub
                 const bool need = active && xk[ks] - w.q64[0] < w.T;
                 rdone = !wave_any(need);
                 if (!rdone) {
-                    const int p = lane_lower_bound(w.c64[1], ks, ke, w.q64[1]);
+                    const int p = wave_lower_bounds(w.c64[1], ks, ke, w.q64[1], need);
                     walk_strip(w, ks, ke, p, p - 1, need, visited);
                 }
             }
@@ -482,8 +581,7 @@ __device__ void knn_item(const KnnArgs& A, const Item& it, const double* __restr
         w.q32[d] = (float)w.q64[d];
     }
     const int K1 = kd.k + 1;
-#pragma unroll
-    for (int t = 0; t < P; ++t) w.L[t] = (t < P - K1) ? -INFINITY : INFINITY;
+    w.L.init(K1, heap_base<P>(A));
     w.T = active ? INFINITY : -INFINITY;
     w.thr = active ? INFINITY : -INFINITY;
     w.rechecks = 0;
@@ -526,7 +624,7 @@ __device__ void knn_item(const KnnArgs& A, const Item& it, const double* __restr
             }
         }
     }
-    const double eps = w.L[P - 1];
+    const double eps = w.L.top();
     if (active) {
         if (kd.eps >= 0) res64[kd.eps + i] = eps;
         if (kd.eps_orig >= 0) res64[kd.eps_orig + A.perm[od.perm + i]] = eps;
@@ -553,11 +651,10 @@ __device__ void walk_item(const KnnArgs& A, const Item& it, const double* __rest
         w.q64[d] = active ? w.c64[d][i] : 0.0;
     }
     const int K1 = kd.k + 1;
-#pragma unroll
-    for (int t = 0; t < P; ++t) w.L[t] = (t < P - K1) ? -INFINITY : INFINITY;
+    w.L.init(K1, heap_base<P>(A));
     w.T = active ? INFINITY : -INFINITY;
     const unsigned v = knn_walk(w, kd, col64, it.lo, it.hi, it.t0, i, active);
-    const double eps = w.L[P - 1];
+    const double eps = w.L.top();
     if (active) {
         if (kd.eps >= 0) res64[kd.eps + i] = eps;
         if (kd.eps_orig >= 0) res64[kd.eps_orig + A.perm[od.perm + i]] = eps;
@@ -610,17 +707,20 @@ void knn_kernel(KnnArgs A, const double* __restrict__ col64, const float* __rest
 template <int D, int P>
 hipError_t launch_dp(const KnnArgs& a, bool f32, bool walk, int grid, const double* col64, const float* col32,
                      double* res64, unsigned long long* stats, hipStream_t st) {
+    // LDS heaps: 64 lanes x the group's largest K1 doubles per workgroup.
+    const unsigned shmem = P == ENM_LIST_HEAP_LDS ? (unsigned)(a.heap_stride * sizeof(double)) : 0u;
     if constexpr (D >= 2) {
         if (walk) {
-            hipLaunchKernelGGL((knn_walk_kernel<D, P>), dim3(grid), dim3(ENM_WAVE), 0, st, a, col64, res64, stats);
+            hipLaunchKernelGGL((knn_walk_kernel<D, P>), dim3(grid), dim3(ENM_WAVE), shmem, st, a, col64, res64,
+                               stats);
             return hipGetLastError();
         }
     }
     if (f32)
-        hipLaunchKernelGGL((knn_kernel<D, P, true>), dim3(grid), dim3(ENM_WAVE), 0, st,
+        hipLaunchKernelGGL((knn_kernel<D, P, true>), dim3(grid), dim3(ENM_WAVE), shmem, st,
                            a, col64, col32, res64, stats);
     else
-        hipLaunchKernelGGL((knn_kernel<D, P, false>), dim3(grid), dim3(ENM_WAVE), 0, st,
+        hipLaunchKernelGGL((knn_kernel<D, P, false>), dim3(grid), dim3(ENM_WAVE), shmem, st,
                            a, col64, col32, res64, stats);
     return hipGetLastError();
 }
@@ -633,6 +733,10 @@ hipError_t launch_d(const KnnArgs& a, int P, bool f32, bool walk, int grid, cons
         case 8: return launch_dp<D, 8>(a, f32, walk, grid, col64, col32, res64, stats, st);
         case 16: return launch_dp<D, 16>(a, f32, walk, grid, col64, col32, res64, stats, st);
         case 32: return launch_dp<D, 32>(a, f32, walk, grid, col64, col32, res64, stats, st);
+        case ENM_LIST_HEAP_LDS: return launch_dp<D, ENM_LIST_HEAP_LDS>(a, f32, walk, grid, col64, col32, res64,
+                                                                      stats, st);
+        case ENM_LIST_HEAP_GLOBAL: return launch_dp<D, ENM_LIST_HEAP_GLOBAL>(a, f32, walk, grid, col64, col32,
+                                                                            res64, stats, st);
         default: return hipErrorInvalidValue;
     }
 }

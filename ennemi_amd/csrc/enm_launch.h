@@ -58,7 +58,10 @@ hipError_t enm_launch_knn(const Item* items, const int* qoff, int n_items, unsig
                           const unsigned long long* maxabs, const int* perm,
                           const double* col64, const float* col32, double* res64,
                           unsigned long long* stats, int dclass, int pclass, bool f32, bool brute,
-                          bool walk, int grid, hipStream_t st);
+                          bool walk, int grid, int k1max, double* heap, hipStream_t st);
+// Grid of a heap-list kNN launch (ENM_LIST_HEAP_GLOBAL) whose heaps fit
+// ENM_HEAP_GLOBAL_BYTES: at most `grid` blocks of 64 x k1max doubles.
+int enm_knn_heap_grid(int grid, int k1max);
 
 int enm_count_dim_class(int d);
 hipError_t enm_launch_count(const Item* items, const int* qoff, int n_items, unsigned* counter,
@@ -78,3 +81,18 @@ hipError_t enm_launch_reduce_pt(const RedItem* items, int n_items, const RedDev*
                                 hipStream_t st);
 hipError_t enm_launch_export_eps(const double* src, const int* perm, int64_t n, double* dst, hipStream_t st);
 hipError_t enm_launch_export_cnt(const int* src, const int* perm, int64_t n, int* dst, hipStream_t st);
+
+hipError_t enm_launch_prep_count(const PrepTaskDev* tasks, const int64_t* blk0, int n_tasks, int64_t n_blocks,
+                                 const double* raw, const unsigned char* mask, int drop_nan, int* blk_count,
+                                 int64_t* blk_off, unsigned* nan_flag, int64_t* n_out, hipStream_t st);
+hipError_t enm_launch_prep_write(PrepTaskDev* tasks, const int64_t* blk0, int n_tasks, int64_t n_blocks,
+                                 const double* raw, const unsigned char* mask, int drop_nan, int rescale,
+                                 const int64_t* blk_off, const PrepSumDev* sums, int n_sums,
+                                 const int* chunk_sum_of, const int* chunk_index, int n_chunks,
+                                 double* chunk_val, double* stats, const int64_t* sblk0, int64_t n_sblocks,
+                                 const double* noise, unsigned* const_flag, double* arena, hipStream_t st);
+
+hipError_t enm_launch_uq_codes(const UqSeg* segs, int nsegs, const double* skeys, const int* sidx, int* codes,
+                               int64_t* starts, int64_t* counts, int64_t* distinct, hipStream_t st);
+hipError_t enm_launch_uq_keys(const UqRowKey* rk, int n_rk, int64_t max_n, const int* ccodes, double* keys,
+                              hipStream_t st);

@@ -1,0 +1,390 @@
+// Device-side preparation of estimate_mi tasks: the lag windows, mask and
+// NaN compaction, and the rescaling of ennemi's _lagged_mi
+// (ennemi/_driver.py:795-806 windows, :834-854 _apply_masks, :871-902
+// _rescale_data), producing the float64 instance columns of a batch directly
+// in HBM from the raw series.
+//
+// Every value is bit-identical to NumPy's:
+//   * compaction keeps the rows in order (a stable prefix scan per block);
+//   * x.mean() and x.std() of a column (and of a one-column condition) sum
+//     in NumPy's order: 8192-element chunks, each by its pairwise recursion
+//     (halves cut at a multiple of 8, leaves of at most 128 elements summed
+//     by 8 interleaved accumulators), the chunk sums added in order to -0.0;
+//   * zs.mean(axis=0) and zs.std(axis=0) of a condition of two or more
+//     columns accumulate each column row by row;
+//   * std = sqrt(sum((v - mean)^2) / n); the scaled value is
+//     ((v - mean) / std) + noise, each operation rounded once (the library
+//     is built without FP contraction).
+// The noise is the reference's: every task draws from a fresh
+// default_rng(2_718281828) in the order x, y, z (z row-major), so its draws
+// are prefixes of one stream, which the host generates once per call.  A
+// variable with std < 1e-20 is left as it is and draws nothing (for the
+// condition: if any of its columns is constant).
+#include "enm_device.h"
+#include "enm_launch.h"
+
+namespace {
+
+constexpr int PB_THREADS = 256;
+constexpr int PB_ROWS = ENM_PREP_BLOCK;          // rows per block
+constexpr int PB_ROUNDS = PB_ROWS / PB_THREADS;  // rows per thread
+constexpr int PW_CHUNK = 8192;                   // NumPy's reduction buffer
+constexpr int PW_LEAF = 128;                     // NumPy's pairwise block
+
+__device__ __forceinline__ bool row_valid(const PrepTaskDev& t, const double* __restrict__ raw,
+                                          const unsigned char* __restrict__ mask, int drop_nan, int64_t r,
+                                          bool& has_nan) {
+    if (mask && !mask[t.y0 + r]) {
+        has_nan = false;
+        return false;
+    }
+    bool nan = isnan(raw[t.x_col + t.x0 + r]) || isnan(raw[t.y_col + t.y0 + r]);
+    for (int c = 0; c < t.dz; ++c) nan = nan || isnan(raw[t.z_col[c] + t.z0[c] + r]);
+    has_nan = nan;
+    return !(drop_nan && nan);
+}
+
+__device__ __forceinline__ int block_task(const int64_t* __restrict__ blk0, int n_tasks, int64_t b) {
+    int lo = 0, hi = n_tasks - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (blk0[mid] <= b) lo = mid; else hi = mid - 1;
+    }
+    return lo;
+}
+
+// Phase 1: kept rows per block, and a NaN among the masked rows per task.
+__global__ __launch_bounds__(PB_THREADS)
+void prep_count_kernel(const PrepTaskDev* __restrict__ tasks, const int64_t* __restrict__ blk0, int n_tasks,
+                       const double* __restrict__ raw, const unsigned char* __restrict__ mask, int drop_nan,
+                       int* __restrict__ blk_count, unsigned* __restrict__ nan_flag) {
+    __shared__ int s_cnt[PB_THREADS / ENM_WAVE];
+    __shared__ int s_nan;
+    const int64_t b = blockIdx.x;
+    const int ti = block_task(blk0, n_tasks, b);
+    const PrepTaskDev t = tasks[ti];
+    const int64_t r0 = (b - blk0[ti]) * PB_ROWS;
+    if (threadIdx.x == 0) s_nan = 0;
+    __syncthreads();
+    int cnt = 0;
+    bool any_nan = false;
+    for (int j = 0; j < PB_ROUNDS; ++j) {
+        const int64_t r = r0 + j * PB_THREADS + threadIdx.x;
+        if (r < t.w) {
+            bool nan;
+            cnt += row_valid(t, raw, mask, drop_nan, r, nan) ? 1 : 0;
+            any_nan = any_nan || nan;
+        }
+    }
+    const int lane = threadIdx.x & (ENM_WAVE - 1), wv = threadIdx.x / ENM_WAVE;
+    for (int off = 32; off > 0; off >>= 1) cnt += __shfl_xor(cnt, off, ENM_WAVE);
+    if (lane == 0) s_cnt[wv] = cnt;
+    if (wave_any(any_nan) && lane == 0) s_nan = 1;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int tot = 0;
+        for (int w = 0; w < PB_THREADS / ENM_WAVE; ++w) tot += s_cnt[w];
+        blk_count[b] = tot;
+        if (s_nan && !drop_nan) atomicOr(nan_flag + ti, 1u);
+    }
+}
+
+// Per task: block offsets (exclusive scan of the block counts, in place in
+// blk_off) and the kept rows n.  One wave per task.
+__global__ __launch_bounds__(ENM_WAVE)
+void prep_scan_kernel(const PrepTaskDev* __restrict__ tasks, const int64_t* __restrict__ blk0, int n_tasks,
+                      const int* __restrict__ blk_count, int64_t* __restrict__ blk_off, int64_t* __restrict__ n_out) {
+    const int ti = blockIdx.x;
+    if (ti >= n_tasks) return;
+    const int64_t b0 = blk0[ti], nb = tasks[ti].nblk;
+    const int lane = threadIdx.x;
+    int64_t carry = 0;
+    for (int64_t base = 0; base < nb; base += ENM_WAVE) {
+        const int64_t v = base + lane < nb ? blk_count[b0 + base + lane] : 0;
+        int64_t inc = v;
+        for (int off = 1; off < ENM_WAVE; off <<= 1) {
+            const int64_t o = __shfl_up(inc, off, ENM_WAVE);
+            if (lane >= off) inc += o;
+        }
+        if (base + lane < nb) blk_off[b0 + base + lane] = carry + inc - v;
+        carry += __shfl(inc, ENM_WAVE - 1, ENM_WAVE);
+    }
+    if (lane == 0) n_out[ti] = carry;
+}
+
+// Phase 2a: the kept rows of every block, in row order, to the task's
+// arena columns (x, y, then the condition columns, n values each).
+__global__ __launch_bounds__(PB_THREADS)
+void prep_compact_kernel(const PrepTaskDev* __restrict__ tasks, const int64_t* __restrict__ blk0, int n_tasks,
+                         const double* __restrict__ raw, const unsigned char* __restrict__ mask, int drop_nan,
+                         const int64_t* __restrict__ blk_off, double* __restrict__ arena) {
+    __shared__ int s_wcnt[PB_THREADS / ENM_WAVE];
+    const int64_t b = blockIdx.x;
+    const int ti = block_task(blk0, n_tasks, b);
+    const PrepTaskDev t = tasks[ti];
+    const int64_t r0 = (b - blk0[ti]) * PB_ROWS;
+    const int lane = threadIdx.x & (ENM_WAVE - 1), wv = threadIdx.x / ENM_WAVE;
+    int64_t pos = blk_off[b];
+    for (int j = 0; j < PB_ROUNDS; ++j) {
+        const int64_t r = r0 + j * PB_THREADS + threadIdx.x;
+        bool nan;
+        const bool keep = r < t.w && row_valid(t, raw, mask, drop_nan, r, nan);
+        const unsigned long long m = __builtin_amdgcn_ballot_w64(keep);
+        const int below = __popcll(m & ((1ull << lane) - 1ull));
+        if (lane == 0) s_wcnt[wv] = __popcll(m);
+        __syncthreads();
+        int before = 0, total = 0;
+        for (int w = 0; w < PB_THREADS / ENM_WAVE; ++w) {
+            before += w < wv ? s_wcnt[w] : 0;
+            total += s_wcnt[w];
+        }
+        __syncthreads();
+        if (keep) {
+            const int64_t p = pos + before + below;
+            double* out = arena + t.arena_off;
+            out[p] = raw[t.x_col + t.x0 + r];
+            out[t.n + p] = raw[t.y_col + t.y0 + r];
+            for (int c = 0; c < t.dz; ++c) out[(2 + c) * t.n + p] = raw[t.z_col[c] + t.z0[c] + r];
+        }
+        pos += total;
+    }
+}
+
+// One term of a sum: the value, or its squared deviation from `mean`.
+struct Term {
+    const double* v;
+    double mean;
+    bool dev;
+    __device__ __forceinline__ double operator()(int64_t i) const {
+        if (!dev) return v[i];
+        const double d = v[i] - mean;
+        return d * d;
+    }
+};
+
+// NumPy's pairwise_sum leaf: fewer than 8 values summed from 0.0; up to 128
+// by 8 interleaved accumulators combined as ((r0+r1)+(r2+r3))+((r4+r5)+(r6+r7)),
+// the remainder added after.
+__device__ __forceinline__ double pw_leaf(const Term& f, int64_t lo, int n) {
+    if (n < 8) {
+        double r = 0.0;
+        for (int i = 0; i < n; ++i) r += f(lo + i);
+        return r;
+    }
+    double r[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) r[j] = f(lo + j);
+    int i = 8;
+    for (; i < n - (n % 8); i += 8) {
+#pragma unroll
+        for (int j = 0; j < 8; ++j) r[j] += f(lo + i + j);
+    }
+    double res = ((r[0] + r[1]) + (r[2] + r[3])) + ((r[4] + r[5]) + (r[6] + r[7]));
+    for (; i < n; ++i) res += f(lo + i);
+    return res;
+}
+
+// The leaves of NumPy's pairwise recursion over n <= PW_CHUNK values, in
+// depth-first order: a node of more than 128 values splits at half its
+// length rounded down to a multiple of 8.  Writes (offset, length) of each
+// leaf and returns their number (at most 128 for n <= 8192).
+__device__ int pw_leaves(int n, int* lo_out, int* len_out) {
+    int st_lo[16], st_len[16];
+    int sp = 0, nl = 0;
+    st_lo[0] = 0;
+    st_len[0] = n;
+    while (sp >= 0) {
+        const int lo = st_lo[sp], len = st_len[sp];
+        --sp;
+        if (len <= PW_LEAF) {
+            lo_out[nl] = lo;
+            len_out[nl] = len;
+            ++nl;
+            continue;
+        }
+        int n2 = len / 2;
+        n2 -= n2 % 8;
+        // Right pushed first so the left one is visited first.
+        ++sp; st_lo[sp] = lo + n2; st_len[sp] = len - n2;
+        ++sp; st_lo[sp] = lo; st_len[sp] = n2;
+    }
+    return nl;
+}
+
+// The same recursion, combining leaf sums (in depth-first order, consumed
+// from `leaf`) as NumPy does: left + right at every node.
+__device__ double pw_combine(int n, const double* leaf) {
+    int st_len[16];
+    double st_left[16];
+    bool st_right[16];
+    int sp = 0, next = 0;
+    st_len[0] = n;
+    st_right[0] = false;
+    for (;;) {
+        while (st_len[sp] > PW_LEAF) {
+            int n2 = st_len[sp] / 2;
+            n2 -= n2 % 8;
+            st_len[sp + 1] = n2;
+            st_right[sp + 1] = false;
+            ++sp;
+        }
+        double val = leaf[next++];
+        for (;;) {
+            if (sp == 0) return val;
+            if (!st_right[sp]) {
+                // A left child is done: keep it, continue with its sibling.
+                st_left[sp - 1] = val;
+                st_len[sp] = st_len[sp - 1] - st_len[sp];
+                st_right[sp] = true;
+                break;
+            }
+            val = st_left[sp - 1] + val;
+            --sp;
+        }
+    }
+}
+
+// Phase 2b: one wave per (array, chunk): the pairwise sum of a chunk of
+// values (pass 0) or of squared deviations from the array's mean (pass 1).
+// Lanes sum the leaves, lane 0 combines them in NumPy's order.
+__global__ __launch_bounds__(ENM_WAVE)
+void prep_chunk_kernel(const PrepSumDev* __restrict__ sums, const int* __restrict__ chunk_sum_of,
+                       const int* __restrict__ chunk_index, int n_chunks, const double* __restrict__ arena,
+                       const double* __restrict__ stats, int pass, double* __restrict__ chunk_val) {
+    __shared__ int s_lo[PW_LEAF], s_len[PW_LEAF];
+    __shared__ double s_leaf[PW_LEAF];
+    __shared__ int s_nl;
+    const int c = blockIdx.x;
+    if (c >= n_chunks) return;
+    const PrepSumDev sd = sums[chunk_sum_of[c]];
+    const int64_t first = (int64_t)chunk_index[c] * PW_CHUNK;
+    const int m = (int)min((int64_t)PW_CHUNK, sd.n - first);
+    if (threadIdx.x == 0) s_nl = pw_leaves(m, s_lo, s_len);
+    __syncthreads();
+    const Term f{arena + sd.src + first, pass ? stats[sd.stat] : 0.0, pass != 0};
+    for (int l = threadIdx.x; l < s_nl; l += ENM_WAVE) s_leaf[l] = pw_leaf(f, s_lo[l], s_len[l]);
+    __syncthreads();
+    if (threadIdx.x == 0) chunk_val[c] = pw_combine(m, s_leaf);
+}
+
+// Phase 2c: per array, the chunk sums added in order to -0.0 (pairwise
+// arrays), or the whole column accumulated row by row (columns of a
+// multi-column condition); then mean = sum / n (pass 0) or
+// std = sqrt(sum / n) (pass 1).  One thread per array.
+__global__ void prep_finish_kernel(const PrepSumDev* __restrict__ sums, int n_sums, const double* __restrict__ arena,
+                                   const double* __restrict__ chunk_val, int pass, double* __restrict__ stats) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n_sums) return;
+    const PrepSumDev sd = sums[s];
+    double acc;
+    if (sd.sequential) {
+        const double* v = arena + sd.src;
+        const double mean = pass ? stats[sd.stat] : 0.0;
+        acc = 0.0;
+        for (int64_t i = 0; i < sd.n; ++i) {
+            double t = v[i];
+            if (pass) {
+                t = t - mean;
+                t = t * t;
+            }
+            acc = i == 0 ? t : acc + t;
+        }
+    } else {
+        acc = -0.0;
+        for (int c = 0; c < sd.n_chunks; ++c) acc += chunk_val[sd.chunk0 + c];
+    }
+    const double q = acc / (double)sd.n;
+    if (pass == 0) stats[sd.stat] = q;
+    else stats[sd.stat + 1] = sqrt(q);
+}
+
+// Phase 2d: which variables are rescaled, where their noise starts, and
+// their flag bits (bit 0 x, bit 1 y, bit 2 z constant).  One thread per task.
+__global__ void prep_noise_kernel(PrepTaskDev* __restrict__ tasks, int n_tasks, const double* __restrict__ stats,
+                                  unsigned* __restrict__ const_flag) {
+    const int ti = blockIdx.x * blockDim.x + threadIdx.x;
+    if (ti >= n_tasks) return;
+    PrepTaskDev& t = tasks[ti];
+    unsigned fl = 0;
+    if (fabs(stats[t.stat + 1]) < 1e-20) fl |= 1u;
+    if (fabs(stats[t.stat + 3]) < 1e-20) fl |= 2u;
+    for (int c = 0; c < t.dz; ++c)
+        if (fabs(stats[t.stat + 4 + 2 * c + 1]) < 1e-20) fl |= 4u;
+    int64_t drawn = 0;
+    t.noise_x = drawn;
+    if (!(fl & 1u)) drawn += t.n;
+    t.noise_y = drawn;
+    if (!(fl & 2u)) drawn += t.n;
+    t.noise_z = drawn;
+    t.flags = fl;
+    const_flag[ti] = fl;
+}
+
+// Phase 2e: (v - mean) / std + noise for every rescaled value, in place.
+__global__ __launch_bounds__(PB_THREADS)
+void prep_scale_kernel(const PrepTaskDev* __restrict__ tasks, const int64_t* __restrict__ sblk0, int n_tasks,
+                       const double* __restrict__ stats, const double* __restrict__ noise, double* __restrict__ arena) {
+    const int64_t b = blockIdx.x;
+    const int ti = block_task(sblk0, n_tasks, b);
+    const PrepTaskDev t = tasks[ti];
+    const int64_t i = (b - sblk0[ti]) * PB_THREADS + threadIdx.x;
+    if (i >= t.n) return;
+    double* col = arena + t.arena_off;
+    if (!(t.flags & 1u)) {
+        const double v = (col[i] - stats[t.stat]) / stats[t.stat + 1];
+        col[i] = v + noise[t.noise_x + i];
+    }
+    if (!(t.flags & 2u)) {
+        const double v = (col[t.n + i] - stats[t.stat + 2]) / stats[t.stat + 3];
+        col[t.n + i] = v + noise[t.noise_y + i];
+    }
+    if (!(t.flags & 4u)) {
+        for (int c = 0; c < t.dz; ++c) {
+            const double mean = stats[t.stat + 4 + 2 * c], sd = stats[t.stat + 5 + 2 * c];
+            const double v = (col[(2 + c) * t.n + i] - mean) / sd;
+            col[(2 + c) * t.n + i] = v + noise[t.noise_z + i * t.dz + c];
+        }
+    }
+}
+
+}  // namespace
+
+hipError_t enm_launch_prep_count(const PrepTaskDev* tasks, const int64_t* blk0, int n_tasks, int64_t n_blocks,
+                                 const double* raw, const unsigned char* mask, int drop_nan, int* blk_count,
+                                 int64_t* blk_off, unsigned* nan_flag, int64_t* n_out, hipStream_t st) {
+    if (n_tasks <= 0) return hipSuccess;
+    hipError_t e = enm_launch_fill_i32(reinterpret_cast<int*>(nan_flag), n_tasks, 0, st);
+    if (e != hipSuccess) return e;
+    if (n_blocks > 0)
+        hipLaunchKernelGGL(prep_count_kernel, dim3((unsigned)n_blocks), dim3(PB_THREADS), 0, st, tasks, blk0, n_tasks,
+                           raw, mask, drop_nan, blk_count, nan_flag);
+    hipLaunchKernelGGL(prep_scan_kernel, dim3((unsigned)n_tasks), dim3(ENM_WAVE), 0, st, tasks, blk0, n_tasks,
+                       blk_count, blk_off, n_out);
+    return hipGetLastError();
+}
+
+hipError_t enm_launch_prep_write(PrepTaskDev* tasks, const int64_t* blk0, int n_tasks, int64_t n_blocks,
+                                 const double* raw, const unsigned char* mask, int drop_nan, int rescale,
+                                 const int64_t* blk_off, const PrepSumDev* sums, int n_sums,
+                                 const int* chunk_sum_of, const int* chunk_index, int n_chunks,
+                                 double* chunk_val, double* stats, const int64_t* sblk0, int64_t n_sblocks,
+                                 const double* noise, unsigned* const_flag, double* arena, hipStream_t st) {
+    if (n_tasks <= 0) return hipSuccess;
+    if (n_blocks > 0)
+        hipLaunchKernelGGL(prep_compact_kernel, dim3((unsigned)n_blocks), dim3(PB_THREADS), 0, st, tasks, blk0,
+                           n_tasks, raw, mask, drop_nan, blk_off, arena);
+    if (!rescale) return hipGetLastError();
+    for (int pass = 0; pass < 2; ++pass) {
+        if (n_chunks > 0)
+            hipLaunchKernelGGL(prep_chunk_kernel, dim3((unsigned)n_chunks), dim3(ENM_WAVE), 0, st, sums,
+                               chunk_sum_of, chunk_index, n_chunks, arena, stats, pass, chunk_val);
+        hipLaunchKernelGGL(prep_finish_kernel, dim3((unsigned)((n_sums + 63) / 64)), dim3(64), 0, st, sums, n_sums,
+                           arena, chunk_val, pass, stats);
+    }
+    hipLaunchKernelGGL(prep_noise_kernel, dim3((unsigned)((n_tasks + 63) / 64)), dim3(64), 0, st, tasks, n_tasks,
+                       stats, const_flag);
+    if (n_sblocks > 0)
+        hipLaunchKernelGGL(prep_scale_kernel, dim3((unsigned)n_sblocks), dim3(PB_THREADS), 0, st, tasks, sblk0,
+                           n_tasks, stats, noise, arena);
+    return hipGetLastError();
+}

@@ -148,6 +148,90 @@ int enm_run_batch(enm_ctx* ctx,
                   uint32_t flags,
                   void* stream);
 
+/*
+ * Device-side preparation of estimate_mi tasks, replacing per task of
+ *
+ *   _lagged_mi windows        ennemi/_driver.py:795-806
+ *   _apply_masks              ennemi/_driver.py:834-854
+ *   _rescale_data             ennemi/_driver.py:871-902
+ *
+ * for continuous x, y and condition.  The raw series live in one float64
+ * device array d_raw; a task names its x and y series and condition columns
+ * by offset and the first row of each window (x0 = max_lag - lag,
+ * y0 = max_lag, z0[c] = max_lag - cond_lag[c]; all windows have w rows, the
+ * mask (one byte per row of the full series, NULL for none) is read at
+ * y0 .. y0+w).  Two calls:
+ *
+ *   enm_prep_count   kept rows n[t] of every task (mask, then, with
+ *                    ENM_P_DROP_NAN, rows with a NaN in x, y or any
+ *                    condition column dropped) and h_nan[t] = 1 when a NaN
+ *                    is left among them; synchronous (host outputs are
+ *                    written before it returns).  The host validates with
+ *                    these (k < n, no NaN left) before going on.
+ *   enm_prep_write   the kept rows of task t, in row order, to
+ *                    d_arena[arena_off[t] ...]: x (n values), y (n), then
+ *                    each condition column (n each); with ENM_P_RESCALE each
+ *                    variable standardised bit for bit as NumPy does in
+ *                    _rescale_data, plus the noise d_noise[...] (the first
+ *                    values of default_rng(2_718281828).normal(0, 1e-10, .),
+ *                    at least max_t n[t] * (2 + dz[t]) of them).
+ *                    d_const[t] (device) receives bit 0/1/2 when x/y/the
+ *                    condition was constant (std < 1e-20: left unscaled, no
+ *                    noise drawn), for the caller's warnings.  Must follow
+ *                    the enm_prep_count of the same tasks, d_raw, d_mask and
+ *                    ENM_P_DROP_NAN on the same context.  Asynchronous.
+ */
+typedef struct enm_prep_task {
+    int64_t x_col, y_col;               /* d_raw offsets of the x and y series   */
+    int64_t x0, y0;                     /* first row of the x and y windows      */
+    int64_t w;                          /* window rows                           */
+    int32_t dz;                         /* condition columns (0 .. 14)           */
+    int32_t pad;
+    int64_t z_col[ENM_MAX_COLS - 2];    /* d_raw offsets of the condition series */
+    int64_t z0[ENM_MAX_COLS - 2];       /* first row of each condition window    */
+} enm_prep_task;
+
+#define ENM_P_DROP_NAN 0x1u
+#define ENM_P_RESCALE  0x2u
+
+int enm_prep_count(enm_ctx* ctx, const enm_prep_task* tasks, int32_t n_tasks,
+                   const double* d_raw, const uint8_t* d_mask, uint32_t flags,
+                   int64_t* h_n, uint32_t* h_nan, void* stream);
+int enm_prep_write(enm_ctx* ctx, const enm_prep_task* tasks, int32_t n_tasks,
+                   const double* d_raw, const uint8_t* d_mask,
+                   const double* d_noise, int64_t noise_len, uint32_t flags,
+                   const int64_t* arena_off, double* d_arena, uint32_t* d_const,
+                   void* stream);
+
+/*
+ * enm_unique_rows: the distinct rows of row sets and their counts, as
+ * np.unique(rows, axis=0, return_inverse=True, return_counts=True) gives
+ * them, for the discrete plug-in estimators
+ *
+ *   _estimate_discrete_entropy             ennemi/_entropy_estimators.py:44-56
+ *   _estimate_discrete_mi                  ennemi/_entropy_estimators.py:257-289
+ *   _estimate_conditional_discrete_mi      ennemi/_entropy_estimators.py:291-320
+ *
+ * A row set is ncol float64 columns of n values in d_values (numeric data;
+ * -0.0 and 0.0 are one value; no NaN).  Its rows are numbered by their
+ * place among the distinct rows in lexicographic order (column 0 first):
+ * for row set s, placed at o_s = n_0 + ... + n_{s-1},
+ *   d_codes[o_s + p]   the number of point p's distinct row (int32),
+ *   d_counts[o_s + r]  the points of distinct row r (int64, r < h_distinct[s]).
+ * Synchronous.  ENM_E_UNSUPPORTED when the product of the distinct counts
+ * of a multi-column row set's columns exceeds 2^53 (nothing is written).
+ */
+typedef struct enm_rowset {
+    int64_t n;                     /* rows                                   */
+    int32_t ncol;                  /* columns (1 .. ENM_MAX_COLS)            */
+    int32_t pad;
+    int64_t col[ENM_MAX_COLS];     /* d_values offset of each column         */
+} enm_rowset;
+
+int enm_unique_rows(enm_ctx* ctx, const enm_rowset* sets, int32_t n_sets,
+                    const double* d_values, int32_t* d_codes, int64_t* d_counts,
+                    int64_t* h_distinct, void* stream);
+
 /* Counters of the last enm_run_batch; synchronises the stream it used. */
 int enm_get_stats(enm_ctx* ctx, enm_stats* out);
 

@@ -1,7 +1,7 @@
 """Write the golden vectors in this folder.
 
 Run in the build container only (the reference is not present on the GPU
-machines):  python tests/golden/make_golden.py
+machines):  python tests/golden/make_golden.py [discrete]
 
 Two kinds of vectors are written, both as small ``.npz`` fixtures:
 
@@ -11,6 +11,8 @@ Two kinds of vectors are written, both as small ``.npz`` fixtures:
   its source tree.  These pin the composed driver path (lag windowing, masks,
   NaN drop, rescaling with the fixed-seed noise, task order, normalisation and
   the psi(0) broadcasting) bit for bit, which no closed form pins.
+* ``discrete_cases.npz`` -- the discrete plug-in estimators (entropy, MI,
+  conditional MI) on numeric, boolean, mixed-type and string values.
 * ``knn_cases.npz`` -- per-point k-th-neighbour radii and ball counts from
   ``scipy.spatial.cKDTree`` called exactly as ennemi calls it, on inputs with
   exact ties, duplicates and negative radii.
@@ -212,9 +214,61 @@ def knn_cases():
     return cases
 
 
+def discrete_cases():
+    """The discrete plug-in estimators: entropy, MI and conditional MI of
+    discrete variables, through the public functions and the estimator-level
+    ones, with numeric, mixed-type, boolean and string values."""
+    ennemi, est = _ref()
+    cases = {}
+
+    def put(name, **arrays):
+        for key, val in arrays.items():
+            cases[f"{name}/{key}"] = np.asarray(val)
+
+    rng = np.random.default_rng(3003)
+    n = 2000
+    z1 = rng.integers(0, 4, n)
+    z2 = rng.integers(0, 3, n)
+    x = (z1 + rng.integers(0, 3, n)) % 5
+    y = (x + z2 + rng.integers(0, 2, n)) % 4
+    xf = x * 0.5 - 1.0
+    lags = np.array([0, 1, 3])
+    with warnings.catch_warnings():
+        warnings.simplefilter("ignore")
+        put("disc_cond1", y=y, x=np.column_stack([x, xf]), lag=lags, cond=z1,
+            out=ennemi.estimate_mi(y, np.column_stack([x, xf]), lag=lags, cond=z1,
+                                   discrete_x=True, discrete_y=True))
+        zz = np.column_stack([z1, z2])
+        put("disc_cond2", y=y, x=x, lag=lags, cond=zz,
+            out=ennemi.estimate_mi(y, x, lag=lags, cond=zz, discrete_x=True, discrete_y=True))
+        put("est_cond_disc", x=xf, y=y, cond=zz,
+            out=est._estimate_conditional_discrete_mi(xf, y, zz))
+        put("est_disc_mi_mixed", x=xf, y=y, out=est._estimate_discrete_mi(xf, y))
+        xb = x > 2
+        put("est_disc_mi_bool", x=xb, y=y, out=est._estimate_discrete_mi(xb, y))
+        xs = np.array(["a", "bb", "c", "dd", "e"])[x]
+        put("est_disc_mi_str", x=xs, y=y, out=est._estimate_discrete_mi(xs, y))
+        put("est_disc_entropy_2d", x=np.column_stack([x, z1, z2]),
+            out=est._estimate_discrete_entropy(np.column_stack([x, z1, z2])))
+        put("entropy_discrete_multidim", x=np.column_stack([xf, z2]),
+            out=ennemi.estimate_entropy(np.column_stack([xf, z2]), discrete=True, multidim=True))
+        put("entropy_discrete_cond", x=x, cond=z1,
+            out=ennemi.estimate_entropy(x, cond=z1, discrete=True))
+        # Many distinct values: the warnings fire, the estimate is still defined.
+        xm = rng.integers(0, 900, n)
+        put("est_disc_mi_many", x=xm, y=y, out=est._estimate_discrete_mi(xm, y))
+        put("est_cond_disc_many", x=xm, y=y, cond=z1,
+            out=est._estimate_conditional_discrete_mi(xm, y, z1))
+    return cases
+
+
 def main():
-    np.savez_compressed(os.path.join(HERE, "knn_cases.npz"), **knn_cases())
-    np.savez_compressed(os.path.join(HERE, "driver_cases.npz"), **driver_cases())
+    if sys.argv[1:] == ["discrete"]:
+        np.savez_compressed(os.path.join(HERE, "discrete_cases.npz"), **discrete_cases())
+    else:
+        np.savez_compressed(os.path.join(HERE, "knn_cases.npz"), **knn_cases())
+        np.savez_compressed(os.path.join(HERE, "driver_cases.npz"), **driver_cases())
+        np.savez_compressed(os.path.join(HERE, "discrete_cases.npz"), **discrete_cases())
     print("wrote", sorted(os.listdir(HERE)))
 
 

@@ -131,3 +131,59 @@ def test_c5b_ross_sampled():
     ref = float(orc.psi(n) + orc.psi(3) - np.mean(orc.psi(n_full.astype(float)))
                 - np.sum(np.dot(orc.psi(counts), counts / n)))
     assert abs(vals[0] - ref) <= TOL
+
+
+# ---------------------------------------------------------------------------
+# C3: the batched masked lag scan (the headline workload)
+# ---------------------------------------------------------------------------
+
+@pytest.mark.parametrize("length", [100_096, 200_000])
+def test_c3_lag_scan(length):
+    """estimate_mi(y, X, lag=-48..48, k=3, drop_nan=True) on the eight AR(1)
+    series: 16 tasks spread over lags and columns equal the oracle's cKDTree
+    path within 1e-12, and three instances' radii and counts equal cKDTree's
+    bit for bit on 2,000 sampled points of the oracle's own preprocessed
+    arrays."""
+    import ennemi_amd
+    from ennemi_amd._driver import plan_mi
+    y, x = wl.lag_scan(length)
+    lags = np.arange(-48, 49)
+    out = ennemi_amd.estimate_mi(y, x, lag=lags, k=3, drop_nan=True)
+    assert out.shape == (97, 8) and np.isfinite(out).all()
+    rng = np.random.default_rng(length)
+    picks = [(0, 0), (96, 7), (48, 0), (48, 5), (1, 1), (95, 2)] + \
+        [(int(rng.integers(97)), int(rng.integers(8))) for _ in range(10)]
+    prepared = {}
+    for li, j in picks:
+        p = (x[:, j], y, lags[li], 48, -48, 3, None, None, np.asarray([0]), False, False, True, True)
+        w, prep = orc.lagged_task(p, backend="kdtree")
+        prepared[(li, j)] = prep
+        assert abs(out[li, j] - w) <= TOL, (li, j, out[li, j], w)
+    batch, indices = plan_mi(y, x, lag=lags, k=3, drop_nan=True)
+    vals, eps, cnts = batch.run(export=True)
+    np.testing.assert_array_equal(vals.reshape(97, 8), out)
+    for li, j in picks[:3]:
+        i = indices.index((li, j))
+        xs, ys, _ = prepared[(li, j)]
+        xs = np.ravel(xs)
+        ys = np.ravel(ys)
+        assert len(eps[i]) == len(xs)
+        _sample_check(np.column_stack([xs, ys]), 3, eps[i], [(xs[:, None], cnts[i][0]), (ys[:, None], cnts[i][1])],
+                      np.random.default_rng(li * 8 + j))
+
+
+# ---------------------------------------------------------------------------
+# C4 with preprocess=False: exact ties in the rounded columns
+# ---------------------------------------------------------------------------
+
+def test_c4_pairwise_exact_ties_sampled_pairs():
+    import ennemi_amd
+    data = wl.c4()
+    out = ennemi_amd.pairwise_mi(data, k=3, preprocess=False)
+    np.testing.assert_array_equal(out, out.T)
+    # Pairs with one or two rounded columns (0, 10, ..., 60) and others.
+    pairs = [(0, 10), (10, 20), (30, 40), (0, 1), (20, 55), (5, 60), (17, 33), (48, 50)]
+    for i, j in pairs:
+        p = (data[:, i], data[:, j], 0, 0, 0, 3, None, None, np.asarray(0), False, False, False, False)
+        w, _ = orc.lagged_task(p, backend="kdtree")
+        assert out[i, j] == w or abs(out[i, j] - w) <= TOL, (i, j, out[i, j], w)

@@ -388,3 +388,56 @@ def test_sample_sorted_modes_graphs_and_exports_agree():
         np.testing.assert_array_equal(db.results(), ref[0])
         torch.ones(1000, device="cuda").sum().item()
     assert db.stats()["graph_replayed"] == 1
+
+
+# ---------------------------------------------------------------------------
+# Tuning invariance: every shape the autotuner may pick gives the same bits
+# ---------------------------------------------------------------------------
+
+AUTOTUNE_STRIPS = [0, 256, 512, 1024, 2048, 4096]
+AUTOTUNE_WAVES = [8, 16, 32]
+
+
+def _mixed_tuning_batch():
+    from ennemi_amd._batch import Batch
+    from ennemi_amd import _estimators as est
+    nat = _nat()
+    rng = np.random.default_rng(2024)
+    b = Batch()
+    a = _data(31, 9000, 4, rounding=None)
+    t = _data(32, 6000, 3, rounding=1)
+    b.add(nat.KIND_KSG, 3, _cols(a[:, :1]), _cols(a[:, 1:2]))
+    b.add(nat.KIND_KSG, 4, _cols(t[:, :2]), _cols(t[:, 2:]))
+    b.add(nat.KIND_CMI, 3, _cols(a[:, :1]), _cols(a[:, 1:2]), _cols(a[:, 2:]))
+    b.add(nat.KIND_CMI, 5, _cols(t[:, :1]), _cols(t[:, 1:2]), _cols(t[:, 2:]))
+    b.add(nat.KIND_KL, 3, _cols(a[:, :3]))
+    b.add(nat.KIND_KL, 40, _cols(t[:, :2]))
+    y = rng.choice(4, size=7000, p=[0.5, 0.3, 0.15, 0.05])
+    x = rng.normal(size=(7000, 2)) + y[:, None]
+    values, counts, order, grouped = est._grouped(y, _cols(x))
+    b.add(nat.KIND_ROSS, 3, grouped, class_sizes=counts)
+    z = [np.ascontiguousarray(np.sin(g)) for g in grouped]
+    b.add(nat.KIND_CROSS, 3, grouped[:1], z=z, class_sizes=counts)
+    return b
+
+
+@pytest.mark.parametrize("flags_name", ["f32", "fp64"])
+def test_every_autotuner_shape_gives_identical_results(flags_name):
+    nat = _nat()
+    flags = _modes()[flags_name]
+    ctx = nat.context()
+    old = ctx.get_tuning()
+    b = _mixed_tuning_batch()
+    try:
+        ctx.set_tuning(1024, 32)
+        ref_vals, ref_eps, ref_cnt = b.run(flags=flags, export=True)
+        for strip in AUTOTUNE_STRIPS:
+            for waves in AUTOTUNE_WAVES:
+                ctx.set_tuning(strip, waves)
+                vals, eps, cnt = b.run(flags=flags, export=True)
+                np.testing.assert_array_equal(vals, ref_vals, err_msg=f"strip={strip} waves={waves}")
+                for i in range(len(vals)):
+                    np.testing.assert_array_equal(eps[i], ref_eps[i], err_msg=f"eps {i} strip={strip}")
+                    np.testing.assert_array_equal(cnt[i], ref_cnt[i], err_msg=f"counts {i} strip={strip}")
+    finally:
+        ctx.set_tuning(*old)

@@ -16,6 +16,7 @@ plants each one and checks that it does.
 """
 
 import math
+import warnings
 
 import numpy as np
 import pytest
@@ -231,3 +232,46 @@ def test_planted_mistakes_are_caught(golden_knn, golden_driver, monkeypatch):
                                                else true_psi(np.asarray(x, dtype=float))))
     out = orc.estimate_single_mi(g["est_single_mi/x"], g["est_single_mi/y"], 3)
     assert abs(out - float(g["est_single_mi/out"])) > 1e-9
+
+
+@pytest.fixture(scope="module")
+def golden_discrete():
+    import os
+    return np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "discrete_cases.npz"),
+                   allow_pickle=False)
+
+
+def test_oracle_discrete_estimators_match_reference(golden_discrete):
+    """The oracle's plug-in estimators against the reference's outputs
+    (tests/golden/discrete_cases.npz): conditional MI with one- and
+    two-column conditions, mixed types, booleans, strings, entropy of rows."""
+    g = golden_discrete
+    with warnings.catch_warnings():
+        warnings.simplefilter("ignore")
+        assert orc.estimate_conditional_discrete_mi(g["est_cond_disc/x"], g["est_cond_disc/y"],
+                                                    g["est_cond_disc/cond"]) == float(g["est_cond_disc/out"])
+        assert orc.estimate_conditional_discrete_mi(g["est_cond_disc_many/x"], g["est_cond_disc_many/y"],
+                                                    g["est_cond_disc_many/cond"]) == \
+            float(g["est_cond_disc_many/out"])
+        for name in ("est_disc_mi_mixed", "est_disc_mi_bool", "est_disc_mi_str", "est_disc_mi_many"):
+            assert orc.estimate_discrete_mi(g[f"{name}/x"], g[f"{name}/y"]) == float(g[f"{name}/out"]), name
+        assert abs(orc.estimate_discrete_entropy(g["est_disc_entropy_2d/x"])
+                   - float(g["est_disc_entropy_2d/out"])) <= 1e-14
+        for name in ("disc_cond1", "disc_cond2"):
+            out = orc.estimate_mi(g[f"{name}/y"], g[f"{name}/x"], g[f"{name}/lag"], cond=g[f"{name}/cond"],
+                                  discrete_x=True, discrete_y=True, threads=1)
+            _assert_same(out, g[f"{name}/out"])
+        _assert_same(orc.estimate_entropy(g["entropy_discrete_multidim/x"], discrete=True, multidim=True),
+                     g["entropy_discrete_multidim/out"])
+        _assert_same(orc.estimate_entropy(g["entropy_discrete_cond/x"], cond=g["entropy_discrete_cond/cond"],
+                                          discrete=True), g["entropy_discrete_cond/out"])
+
+
+def test_planted_mistake_in_conditional_discrete_mi_is_caught(golden_discrete, monkeypatch):
+    """Weighting each condition row by 1 instead of n_z must fail the pin."""
+    g = golden_discrete
+    x, y, z = g["est_cond_disc/x"], g["est_cond_disc/y"], g["est_cond_disc/cond"]
+    _, inv, cnt = np.unique(z, axis=0, return_inverse=True, return_counts=True)
+    inv = np.ravel(inv)
+    wrong = sum(orc.estimate_discrete_mi(x[inv == i], y[inv == i]) for i in range(len(cnt))) / len(cnt)
+    assert abs(wrong - float(g["est_cond_disc/out"])) > 1e-6
