@@ -29,8 +29,10 @@ F_TIMING = 0x10
 STAGES = ("fill", "sort", "gather", "knn", "count", "combine")
 
 EXPORTED_SYMBOLS = ("enm_init", "enm_shutdown", "enm_last_error", "enm_run_batch",
-                    "enm_prep_count", "enm_prep_write", "enm_unique_rows", "enm_get_stats",
+                    "enm_run_batch_static", "enm_prep_count", "enm_prep_write", "enm_unique_rows", "enm_get_stats",
                     "enm_autotune", "enm_set_tuning", "enm_get_tuning")
+# Workspace of enm_run_batch_static, bytes (ENM_STATIC_WORKSPACE_BYTES).
+STATIC_WORKSPACE_BYTES = 256 << 20
 
 
 class NativeUnavailable(RuntimeError):
@@ -96,6 +98,8 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
                                   ctypes.POINTER(ctypes.c_int64), vp, vp, vp, vp,
                                   ctypes.c_uint32, vp]
     lib.enm_run_batch.restype = ctypes.c_int
+    lib.enm_run_batch_static.argtypes = lib.enm_run_batch.argtypes
+    lib.enm_run_batch_static.restype = ctypes.c_int
     lib.enm_prep_count.argtypes = [vp, ctypes.POINTER(EnmPrepTask), ctypes.c_int32, vp, vp,
                                    ctypes.c_uint32, vp, vp, vp]
     lib.enm_prep_count.restype = ctypes.c_int
@@ -146,6 +150,14 @@ class Context:
         rc = self.lib.enm_run_batch(self.ctx, inst_arr, n_inst, seg_arr, arena_ptr, out_ptr,
                                     eps_ptr or None, cnt_ptr or None, flags, stream_ptr or None)
         self.check(rc, "enm_run_batch")
+
+    def run_static(self, inst_arr, n_inst: int, seg_arr, arena_ptr: int, out_ptr: int,
+                   eps_ptr: int = 0, cnt_ptr: int = 0, flags: int = 0, stream_ptr: int = 0) -> None:
+        """enm_run_batch_static: the same batch with no device allocation
+        (workspace in the library's static array)."""
+        rc = self.lib.enm_run_batch_static(self.ctx, inst_arr, n_inst, seg_arr, arena_ptr, out_ptr,
+                                           eps_ptr or None, cnt_ptr or None, flags, stream_ptr or None)
+        self.check(rc, "enm_run_batch_static")
 
     def prep_count(self, tasks, n_tasks: int, raw_ptr: int, mask_ptr: int, flags: int,
                    h_n: np.ndarray, h_nan: np.ndarray, stream_ptr: int = 0) -> None:

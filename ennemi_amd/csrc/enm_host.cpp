@@ -25,6 +25,7 @@
 #include <list>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <set>
 #include <string>
 #include <tuple>
@@ -163,6 +164,18 @@ struct Plan {
 
 }  // namespace
 
+// Workspace of enm_run_batch_static.  Every GPU that loads this code object
+// holds its own copy until the process ends.  In host code the name is only
+// a stand-in: the copy of the current device is at the address
+// hipGetSymbolAddress gives (kept per device in g_scratch), and that address
+// is what kernels and copies receive.
+static __device__ __align__(256) unsigned char scratch[ENM_STATIC_WORKSPACE_BYTES];
+
+namespace {
+std::mutex g_scratch_mu;
+std::map<int, void*> g_scratch;   // device -> address of its scratch copy
+}  // namespace
+
 struct enm_ctx {
     int device = 0;
     int num_cus = 256;
@@ -190,6 +203,9 @@ struct enm_ctx {
         DevBuf tables, k0, k1, i0, i1, split, ccodes, cstarts, ccounts, cdist, rkeys, rstarts, rdist;
         std::vector<char> blob1, blob2;
     } uq;
+    void* scratch = nullptr;          // this device's static workspace
+    std::string scratch_err;          // why it is unavailable, if it is
+    std::unique_ptr<Plan> static_plan;   // plan of the last enm_run_batch_static
 };
 
 namespace {
@@ -1158,6 +1174,25 @@ int enm_init(int device, enm_ctx** out_ctx) {
     enm_ctx* c = new enm_ctx();
     c->device = device;
     c->num_cus = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
+    {
+        // The static workspace's address on this device, looked up once per
+        // device with the device current.
+        std::lock_guard<std::mutex> guard(g_scratch_mu);
+        auto it = g_scratch.find(device);
+        if (it != g_scratch.end()) {
+            c->scratch = it->second;
+        } else {
+            void* p = nullptr;
+            e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+            if (e == hipSuccess && p) {
+                g_scratch[device] = p;
+                c->scratch = p;
+            } else {
+                c->scratch_err = std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e);
+                (void)hipGetLastError();
+            }
+        }
+    }
     *out_ctx = c;
     return ENM_OK;
 }
@@ -1249,6 +1284,64 @@ int enm_run_batch(enm_ctx* ctx, const enm_instance* inst, int32_t n_inst, const 
     }
     ctx->last = up.get();
     ctx->cache.push_front(std::move(up));
+    return ENM_OK;
+}
+
+int enm_run_batch_static(enm_ctx* ctx, const enm_instance* inst, int32_t n_inst, const int64_t* seg_len,
+                         const double* d_arena, double* d_out, double* d_eps, int32_t* d_counts, uint32_t flags,
+                         void* stream) {
+    if (!ctx) { g_init_error = "ctx is NULL"; return ENM_E_ARG; }
+    ctx->err.clear();
+    if (!inst || n_inst <= 0) return fail(ctx, ENM_E_ARG, "no instances");
+    if (!d_arena || !d_out) return fail(ctx, ENM_E_ARG, "d_arena and d_out must be device pointers");
+    if (flags & (ENM_F_GRAPH | ENM_F_TIMING))
+        return fail(ctx, ENM_E_ARG, "enm_run_batch_static takes neither ENM_F_GRAPH nor ENM_F_TIMING");
+    if (!ctx->scratch) return fail(ctx, ENM_E_HIP, "static workspace unavailable: " + ctx->scratch_err);
+    hipError_t e = hipSetDevice(ctx->device);
+    if (e != hipSuccess) return hip_fail(ctx, e, "hipSetDevice");
+    hipStream_t st = (hipStream_t)stream;
+    if (ctx->last_stream != st && ctx->last_stream != nullptr) {
+        e = hipStreamSynchronize(ctx->last_stream);
+        if (e != hipSuccess) return hip_fail(ctx, e, "hipStreamSynchronize");
+    }
+    ctx->last_stream = st;
+    std::unique_ptr<Plan> up(new Plan());
+    Plan& P = *up;
+    P.f32 = (flags & ENM_F_FP64) == 0;
+    P.brute = (flags & ENM_F_BRUTE) != 0;
+    P.need32 = P.brute;
+    P.per_instance = (flags & ENM_F_PER_INSTANCE) != 0;
+    P.want_eps = d_eps != nullptr;
+    int rc = plan_batch(ctx, P, inst, n_inst, seg_len);
+    if (rc) return rc;
+    const std::vector<char> blob = pack_tables(P);
+    const std::vector<BufSpec> layout = workspace_layout(P);
+    auto rounded = [](size_t b) { return (std::max<size_t>(b, 256) + 255) & ~(size_t)255; };
+    size_t total = 0;
+    for (const BufSpec& b : layout) total += rounded(b.bytes);
+    if (total > (size_t)ENM_STATIC_WORKSPACE_BYTES) {
+        char msg[200];
+        std::snprintf(msg, sizeof msg,
+                      "the batch needs %zu bytes of workspace; enm_run_batch_static holds at most %llu",
+                      total, (unsigned long long)ENM_STATIC_WORKSPACE_BYTES);
+        return fail(ctx, ENM_E_NOMEM, msg);
+    }
+    char* base = static_cast<char*>(ctx->scratch);
+    size_t off = 0;
+    for (const BufSpec& b : layout) {
+        b.buf->borrow(base + off, b.bytes);
+        off += rounded(b.bytes);
+    }
+    // The tables go up from pageable host memory; the copy is complete
+    // before the blob is released.
+    e = hipMemcpyAsync(P.tables.p, blob.data(), P.table_bytes, hipMemcpyHostToDevice, st);
+    if (e != hipSuccess) return hip_fail(ctx, e, "hipMemcpyAsync(tables)");
+    e = hipStreamSynchronize(st);
+    if (e != hipSuccess) return hip_fail(ctx, e, "hipStreamSynchronize");
+    rc = enqueue(ctx, P, d_arena, d_out, d_eps, d_counts, st, false);
+    if (rc) return rc;
+    ctx->static_plan = std::move(up);
+    ctx->last = ctx->static_plan.get();
     return ENM_OK;
 }
 

@@ -149,6 +149,39 @@ int enm_run_batch(enm_ctx* ctx,
                   void* stream);
 
 /*
+ * enm_run_batch_static: enm_run_batch without any device allocation.
+ *
+ * No counterpart in the reference.  It is for callers that cannot allocate
+ * device memory (code under hipGraph capture, tight serving loops, harnesses
+ * that forbid hipMalloc).  Same arguments, results and exports as
+ * enm_run_batch, with these differences:
+ *   - the whole workspace (plan tables, sort buffers, orders, radii, counts)
+ *     lives in one static device array of ENM_STATIC_WORKSPACE_BYTES, one
+ *     copy per GPU, held from the first enm_init on that GPU until the
+ *     process ends; a batch that needs more returns ENM_E_NOMEM and says how
+ *     much it needs (about 140 bytes per point for KSG, so about 1.9 million
+ *     points per call);
+ *   - nothing is cached: every call plans the batch, copies its tables from
+ *     pageable host memory and synchronises the stream once before
+ *     enqueueing the kernels (the call still returns before they finish);
+ *   - ENM_F_GRAPH and ENM_F_TIMING are rejected (ENM_E_ARG).
+ * Every call on one GPU shares that GPU's array, which keeps the last call's
+ * contents: calls that overlap on one GPU (from two contexts, or two streams
+ * of one context) race, so serialise them.  enm_get_stats reports the last
+ * call, as for enm_run_batch.
+ */
+#define ENM_STATIC_WORKSPACE_BYTES (256ull << 20)
+int enm_run_batch_static(enm_ctx* ctx,
+                         const enm_instance* inst, int32_t n_inst,
+                         const int64_t* seg_len,
+                         const double* d_arena,
+                         double* d_out,
+                         double* d_eps,
+                         int32_t* d_counts,
+                         uint32_t flags,
+                         void* stream);
+
+/*
  * Device-side preparation of estimate_mi tasks, replacing per task of
  *
  *   _lagged_mi windows        ennemi/_driver.py:795-806
