@@ -219,7 +219,7 @@ def main() -> int:
     ap.add_argument("--fp64", action="store_true", help="headline batch in FP64 mode (no FP32 filter)")
     ap.add_argument("--eager", action="store_true", help="time eager launch sequences instead of graph replays")
     ap.add_argument("--tuning", default=None,
-                    help="strip_points,waves_per_cu to use instead of the autotuner")
+                    help="strip_points,waves_per_cu[,list_slots,items_per_ticket] instead of the autotuner")
     ap.add_argument("--config", default=None,
                     help="time another configuration instead (c1 c2 c3 c4 c5a c5b); one JSON line each")
     ap.add_argument("--copy-calib", action="store_true",
@@ -268,13 +268,14 @@ def main() -> int:
     tdb = batch.device(mode_flags | nat.F_TIMING)
     upload_s = time.perf_counter() - t0
     if args.tuning:
-        sp, wv = (int(v) for v in args.tuning.split(","))
-        tdb.ctx.set_tuning(sp, wv)
-    tuning = tdb.ctx.get_tuning()
+        tdb.ctx.set_tuning_ex(*(int(v) for v in args.tuning.split(",")))
     if not (args.no_autotune or args.tuning):
         t0 = time.perf_counter()
-        tuning = tdb.autotune()
-        log(f"autotuned strip={tuning[0]} waves/CU={tuning[1]} in {time.perf_counter() - t0:.1f} s")
+        tdb.autotune()
+        log(f"autotuned in {time.perf_counter() - t0:.1f} s")
+    # The shape the headline's instances run with (KSG, two columns, k = 3).
+    tuning = tdb.ctx.tuning_for(2, 3, int(ns.max()))
+    log(f"tuning: strip={tuning[0]} waves/CU={tuning[1]} list slots={tuning[2]} items/ticket={tuning[3]}")
 
     # Headline: the whole batch as one hipGraph replay per step.
     if args.eager:
@@ -437,7 +438,8 @@ def main() -> int:
                            "kNN, LDS binary-search 1-D counts (FP32 filter + FP64 recheck only "
                            "where it pays: brute force, off-grid multi-D counts)",
                    "graph_replayed": st["graph_replayed"], "n_launches_eager": stats[-1]["n_launches"],
-                   "tuning": {"strip_points": tuning[0], "waves_per_cu": tuning[1]}},
+                   "tuning": {"strip_points": tuning[0], "waves_per_cu": tuning[1],
+                              "list_slots": tuning[2], "items_per_ticket": tuning[3]}},
         "roofline": roofline,
         "cpu_baseline": cpu,
         "end_to_end": e2e,

@@ -30,7 +30,8 @@ STAGES = ("fill", "sort", "gather", "knn", "count", "combine")
 
 EXPORTED_SYMBOLS = ("enm_init", "enm_shutdown", "enm_last_error", "enm_run_batch",
                     "enm_run_batch_static", "enm_prep_count", "enm_prep_write", "enm_unique_rows", "enm_get_stats",
-                    "enm_autotune", "enm_set_tuning", "enm_get_tuning")
+                    "enm_autotune", "enm_set_tuning", "enm_set_tuning_ex", "enm_get_tuning",
+                    "enm_get_tuning_for")
 # Workspace of enm_run_batch_static, bytes (ENM_STATIC_WORKSPACE_BYTES).
 STATIC_WORKSPACE_BYTES = 256 << 20
 
@@ -64,6 +65,14 @@ class EnmRowset(ctypes.Structure):
 
 
 E_UNSUPPORTED = -4
+
+
+class EnmTuning(ctypes.Structure):
+    _fields_ = [("strip_points", ctypes.c_int32), ("waves_per_cu", ctypes.c_int32),
+                ("list_slots", ctypes.c_int32), ("items_per_ticket", ctypes.c_int32)]
+
+    def as_tuple(self) -> Tuple[int, int, int, int]:
+        return (self.strip_points, self.waves_per_cu, self.list_slots, self.items_per_ticket)
 
 
 class EnmStats(ctypes.Structure):
@@ -117,6 +126,11 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
     lib.enm_set_tuning.restype = ctypes.c_int
     lib.enm_get_tuning.argtypes = [vp, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32)]
     lib.enm_get_tuning.restype = ctypes.c_int
+    lib.enm_set_tuning_ex.argtypes = [vp, ctypes.POINTER(EnmTuning)]
+    lib.enm_set_tuning_ex.restype = ctypes.c_int
+    lib.enm_get_tuning_for.argtypes = [vp, ctypes.c_int32, ctypes.c_int32, ctypes.c_int64,
+                                       ctypes.POINTER(EnmTuning)]
+    lib.enm_get_tuning_for.restype = ctypes.c_int
     return lib
 
 
@@ -196,6 +210,16 @@ class Context:
         w = ctypes.c_int32()
         self.check(self.lib.enm_get_tuning(self.ctx, ctypes.byref(q), ctypes.byref(w)), "enm_get_tuning")
         return q.value, w.value
+
+    def set_tuning_ex(self, strip_points: int, waves_per_cu: int, list_slots: int = 0,
+                      items_per_ticket: int = 0) -> None:
+        t = EnmTuning(strip_points, waves_per_cu, list_slots, items_per_ticket)
+        self.check(self.lib.enm_set_tuning_ex(self.ctx, ctypes.byref(t)), "enm_set_tuning_ex")
+
+    def tuning_for(self, dims: int, k: int, n: int) -> Tuple[int, int, int, int]:
+        t = EnmTuning()
+        self.check(self.lib.enm_get_tuning_for(self.ctx, dims, k, n, ctypes.byref(t)), "enm_get_tuning_for")
+        return t.as_tuple()
 
     def autotune(self, inst_arr, n_inst, seg_arr, arena_ptr, out_ptr, flags=0, stream_ptr=0) -> None:
         self.check(self.lib.enm_autotune(self.ctx, inst_arr, n_inst, seg_arr, arena_ptr, out_ptr,

@@ -559,10 +559,10 @@ hipError_t enm_launch_count(const Item* items, const int* qoff, int n_items, uns
                             const double* col64, const float* col32, double* res64, int* res32,
                             unsigned long long* stats, int dclass, bool f32, bool brute,
                             int grid, unsigned* deferred, unsigned* n_deferred, unsigned seg_cap,
-                            hipStream_t st) {
+                            int ticket, hipStream_t st) {
     if (n_items <= 0) return hipSuccess;
     // Tickets of up to ENM_KCHUNK items, at least four per wave of the grid.
-    const int chunk = std::max(1, std::min(ENM_KCHUNK, n_items / (4 * std::max(1, grid))));
+    const int chunk = ticket > 0 ? ticket : std::max(1, std::min(ENM_KCHUNK, n_items / (4 * std::max(1, grid))));
     CountArgs a{items, qoff, counter, chunk, descs, orders, maxabs, perm, psi_tab, brute ? 1 : 0};
     if (dclass == 1) {
         // `grid` counts waves; a 1-D workgroup has four.

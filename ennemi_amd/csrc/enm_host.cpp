@@ -82,6 +82,9 @@ struct Group {
     int inst;       // instance (per-instance mode) or -1
     int walk;       // kNN items on grid orders (per-lane walk kernel)
     int k1max;      // largest k+1 of its kNN items (sizes heap lists)
+    int waves;      // persistent grid: waves per CU
+    int ticket;     // items per queue ticket (0: automatic)
+    int smax;       // strip items: the largest strip (LDS slots)
 };
 
 struct ExportSpace {
@@ -151,6 +154,7 @@ struct Plan {
     void* pinned = nullptr;
     hipEvent_t upload_done = nullptr;
     hipGraphExec_t graph = nullptr;
+    const void* graph_ptrs[4] = {};   // arena, out, eps, counts the graph was captured with
     int n_launch = 0;
     bool timing = false;       // events recorded by the last enqueue
     hipEvent_t ev[ENM_N_STAGES + 1] = {};
@@ -180,8 +184,12 @@ struct enm_ctx {
     int device = 0;
     int num_cus = 256;
     std::string err;
-    int strip = 1024;       // grid strip size of the kNN order (0: 1-D key order)
-    int waves_per_cu = 32;
+    // Tile shapes: the default, and the shapes enm_autotune picked per
+    // (joint dimension class, list class, floor(log2 n)).  Plans are cached
+    // by tuning_gen, which every change bumps.
+    enm_tuning tuning{1024, 32, 0, 0};
+    std::map<std::tuple<int, int, int>, enm_tuning> tuned;
+    uint64_t tuning_gen = 0;
     std::list<std::unique_ptr<Plan>> cache;   // most recent first
     Plan* last = nullptr;
     hipStream_t last_stream = nullptr;
@@ -291,6 +299,7 @@ struct Planner {
         od.base_perm = -1;
         od.base_c64 = -1;
         od.base_c32 = -1;
+        od.xy = -1;
         for (size_t c = 0; c < cols_src.size(); ++c) od.src[c] = cols_src[c];
         P.n_col64 += od.ncol * I.n;
         P.n_col32 += od.ncol * I.n;
@@ -323,6 +332,7 @@ struct Planner {
         if (it != order_ids.end()) return it->second;
         const OrderDev bo = P.orders[base];
         OrderDev od = bo;
+        od.xy = -1;
         od.c64 = P.n_col64;
         od.c32 = P.n_col32;
         od.perm = P.n_perm;
@@ -336,9 +346,6 @@ struct Planner {
         P.orders.push_back(od);
         order_ids[key] = id;
         order_group.push_back(order_group[base]);
-        // Strips up to 2048 keys are sorted by one block each, with the
-        // smallest tile that holds a strip.
-        P.tile2 = S <= 512 ? 512 : S <= 1024 ? 1024 : ENM_SORT_TILE;
         for (auto rg : ranges(base)) {
             for (int st = rg.first; st < rg.second; st += S) {
                 const int len = std::min(S, rg.second - st);
@@ -398,17 +405,42 @@ struct PendingItem {
     int dclass, pclass;
     int walk = 0;   // kNN on a grid order: the per-lane walk kernel
     int k1 = 0;     // k+1 of its instance (kNN items)
+    int waves = 32, ticket = 0;   // its instance's tile shape
+    int strip = 0;  // strip items: the strip size
 };
 
-int grid_for(enm_ctx* ctx, int n_items) {
-    const int g = ctx->num_cus * ctx->waves_per_cu;
+int grid_for(const enm_ctx* ctx, int waves_per_cu, int n_items) {
+    const int g = ctx->num_cus * waves_per_cu;
     return std::max(1, std::min(n_items, g));
+}
+
+std::tuple<int, int, int> shape_key(int dims, int k, int64_t n) {
+    int b = 0;
+    while (b < 62 && ((int64_t)1 << (b + 1)) <= n) ++b;
+    return std::make_tuple(enm_knn_dim_class(dims), enm_knn_list_class(k + 1), b);
+}
+
+// The tile shape the planner uses for an instance of `dims` joint columns.
+enm_tuning tuning_for(const enm_ctx* ctx, int dims, int k, int64_t n) {
+    auto it = ctx->tuned.find(shape_key(dims, k, n));
+    return it != ctx->tuned.end() ? it->second : ctx->tuning;
 }
 
 int plan_batch(enm_ctx* ctx, Plan& P, const enm_instance* inst, int n_inst, const int64_t* seg_len) {
     Planner pl(P, inst, seg_len);
     std::vector<PendingItem> kpend, cpend;
     P.n_inst = n_inst;
+    // Every instance's tile shape; grid strips up to 2048 points are sorted
+    // by one block each, with the smallest block tile that holds the
+    // largest strip.
+    std::vector<enm_tuning> tunes(n_inst);
+    int max_strip = 0;
+    for (int ii = 0; ii < n_inst; ++ii) {
+        const int cols = std::max(0, inst[ii].dx) + std::max(0, inst[ii].dy) + std::max(0, inst[ii].dz);
+        tunes[ii] = tuning_for(ctx, cols, inst[ii].k, inst[ii].n);
+        if (cols >= 2) max_strip = std::max(max_strip, (int)tunes[ii].strip_points);
+    }
+    P.tile2 = max_strip <= 512 ? 512 : max_strip <= 1024 ? 1024 : ENM_SORT_TILE;
     int64_t point_off = 0;
     for (int ii = 0; ii < n_inst; ++ii) {
         const enm_instance& I = inst[ii];
@@ -479,9 +511,16 @@ int plan_batch(enm_ctx* ctx, Plan& P, const enm_instance* inst, int n_inst, cons
         A = pl.order(ii, acols, segA);
         // The kNN runs on a grid order (strips of the key-sorted order, each
         // sorted by the second column) when there are two or more columns.
-        const int S = ctx->strip;
+        const enm_tuning& T = tunes[ii];
+        const int S = T.strip_points;
         const bool grid = !P.brute && S > 0 && acols.size() >= 2;
         const int K = grid ? pl.grid_order(A, S) : A;
+        if (grid && acols.size() == 2 && P.orders[K].xy < 0) {
+            // The per-lane walk reads (column 0, column 1) pairs.
+            P.n_col64 += P.n_col64 & 1;
+            P.orders[K].xy = P.n_col64;
+            P.n_col64 += 2 * I.n;
+        }
 
         struct SpaceSpec { int order; std::vector<int> cols; bool segmented; };
         std::vector<SpaceSpec> spaces;
@@ -536,7 +575,7 @@ int plan_batch(enm_ctx* ctx, Plan& P, const enm_instance* inst, int n_inst, cons
         std::memset(&kd, 0, sizeof(kd));
         kd.order = K;
         kd.inst = ii;
-        kd.strip = grid ? ctx->strip : 0;
+        kd.strip = grid ? S : 0;
         kd.base_key = P.orders[A].c64;
         kd.k = I.k;
         kd.kl = I.kind == ENM_KIND_KL;
@@ -548,12 +587,31 @@ int plan_batch(enm_ctx* ctx, Plan& P, const enm_instance* inst, int n_inst, cons
         const int kdi = (int)P.kdescs.size();
         const int D = (int)acols.size();
         const int dcl = enm_knn_dim_class(D);
-        const int pcl = enm_knn_list_class(I.k + 1);
+        // A register list may be given more slots than k+1 needs (the
+        // extra ones stay pinned at -inf).
+        const int pcl = I.k + 1 <= 32 && T.list_slots > 0 ? enm_knn_list_class(std::max(I.k + 1, (int)T.list_slots))
+                                                          : enm_knn_list_class(I.k + 1);
         if (dcl < 0) return bad("joint dimension above 16");
         if (I.k > 0x3fffffff) return bad("k out of range");
         int ktiles = 0;
         std::vector<PendingItem> mine;
+        // Two-column grid orders with register lists and strips that three
+        // LDS slots hold: a strip per workgroup.
+        const bool strip_items = grid && D == 2 && pcl > 0 && S <= ENM_STRIP_LDS_MAX;
         for (auto rg : pl.ranges(K)) {
+            if (strip_items) {
+                for (int t0 = rg.first; t0 < rg.second; t0 += S) {
+                    PendingItem pi;
+                    pi.it = Item{kdi, ktiles++, rg.first, rg.second, t0, std::min(t0 + S, rg.second)};
+                    pi.cost = (std::min(t0 + S, rg.second) - t0 + ENM_TILE - 1) / ENM_TILE;
+                    pi.inst = ii; pi.dclass = dcl; pi.pclass = pcl; pi.walk = 2;
+                    pi.k1 = I.k + 1;
+                    pi.waves = T.waves_per_cu; pi.ticket = T.items_per_ticket;
+                    pi.strip = S;
+                    mine.push_back(pi);
+                }
+                continue;
+            }
             // Tiles never straddle a strip: strips start at lo + s*strip and
             // the strip size is a multiple of the tile size.
             for (int t0 = rg.first; t0 < rg.second; t0 += ENM_TILE) {
@@ -566,6 +624,7 @@ int plan_batch(enm_ctx* ctx, Plan& P, const enm_instance* inst, int n_inst, cons
                 // gridded two prune less and tiles share candidates.
                 pi.inst = ii; pi.dclass = dcl; pi.pclass = pcl; pi.walk = grid && D == 2 ? 1 : 0;
                 pi.k1 = I.k + 1;
+                pi.waves = T.waves_per_cu; pi.ticket = T.items_per_ticket;
                 mine.push_back(pi);
             }
         }
@@ -619,6 +678,7 @@ int plan_batch(enm_ctx* ctx, Plan& P, const enm_instance* inst, int n_inst, cons
                                  std::min(t0 + step, rg.second)};
                     pi.cost = P.brute && ccl != 1 ? (int64_t)(rg.second - rg.first) : (int64_t)step;
                     pi.inst = ii; pi.dclass = ccl; pi.pclass = 0;
+                    pi.waves = T.waves_per_cu; pi.ticket = T.items_per_ticket;
                     cpend.push_back(pi);
                 }
                 if (ccl == 1)
@@ -694,9 +754,16 @@ int plan_batch(enm_ctx* ctx, Plan& P, const enm_instance* inst, int n_inst, cons
             g.inst = P.per_instance ? pend[i].inst : -1;
             g.walk = pend[i].walk;
             g.k1max = 1;
-            for (size_t t = i; t < j; ++t) g.k1max = std::max(g.k1max, pend[t].k1);
+            g.smax = 0;
+            for (size_t t = i; t < j; ++t) {
+                g.k1max = std::max(g.k1max, pend[t].k1);
+                g.smax = std::max(g.smax, pend[t].strip);
+            }
+            // The group runs with the shape of its costliest item's instance.
+            g.waves = pend[i].waves;
+            g.ticket = pend[i].ticket;
             if (!is_count && g.pclass == ENM_LIST_HEAP_GLOBAL) {
-                const int hg = enm_knn_heap_grid(grid_for(ctx, g.n_items), g.k1max);
+                const int hg = enm_knn_heap_grid(grid_for(ctx, g.waves, g.n_items), g.k1max);
                 P.heap_bytes = std::max<int64_t>(P.heap_bytes, (int64_t)hg * ENM_TILE * g.k1max * 8);
             }
             const size_t blk = std::max<size_t>(64, (j - i + 4 * ENM_NQ - 1) / (4 * ENM_NQ));
@@ -1038,14 +1105,22 @@ int enqueue(enm_ctx* ctx, Plan& P, const double* arena, double* out, double* d_e
     const double* psi_tab = P.res64.as<double>() + P.psi_tab;
     auto run_group = [&](const Group& g) -> int {
         if (!g.is_count) {
+            int grid = g.pclass == ENM_LIST_HEAP_GLOBAL ? enm_knn_heap_grid(grid_for(ctx, g.waves, g.n_items), g.k1max)
+                                                        : grid_for(ctx, g.waves, g.n_items);
+            if (g.walk == 2) {
+                // Strip workgroups: as many per CU as the waves per CU and
+                // the LDS (three slots of the largest strip) allow.
+                const int nt = std::min(512, ((g.smax + 63) / 64) * 64);
+                const int by_waves = std::max(1, g.waves * 64 / nt);
+                const int by_lds = std::max(1, (160 * 1024) / (3 * g.smax * 16));
+                grid = std::max(1, std::min(g.n_items, ctx->num_cus * std::min(by_waves, by_lds)));
+            }
             e = enm_launch_knn(tab<Item>(P, P.off_kitems) + g.first, qoffs + g.qoff, g.n_items,
                                counters + g.counter,
                                tab<KnnDev>(P, P.off_kdescs), orders, maxabs, P.perm.as<int>(),
                                P.col64.as<double>(), P.col32.as<float>(), P.res64.as<double>(), stats,
-                               g.dclass, g.pclass, P.f32 && P.brute, P.brute, g.walk,
-                               g.pclass == ENM_LIST_HEAP_GLOBAL ? enm_knn_heap_grid(grid_for(ctx, g.n_items), g.k1max)
-                                                                 : grid_for(ctx, g.n_items),
-                               g.k1max, P.heap.as<double>(), st);
+                               g.dclass, g.pclass, P.f32 && P.brute, P.brute, g.walk, grid,
+                               g.k1max, P.heap.as<double>(), g.ticket, g.smax, st);
             ++nl;
             return chk(e, "knn");
         }
@@ -1053,9 +1128,9 @@ int enqueue(enm_ctx* ctx, Plan& P, const double* arena, double* out, double* d_e
                              counters + g.counter, tab<CountDev>(P, P.off_cdescs), orders, maxabs,
                              P.perm.as<int>(), psi_tab, P.col64.as<double>(), P.col32.as<float>(),
                              P.res64.as<double>(), P.res32.as<int>(), stats, g.dclass, P.f32, P.brute,
-                             grid_for(ctx, g.n_items * (g.dclass == 1 ? ENM_C1_THREADS / 64 : 1)),
+                             grid_for(ctx, g.waves, g.n_items * (g.dclass == 1 ? ENM_C1_THREADS / 64 : 1)),
                              P.deferred.as<unsigned>(), counters + g.counter + ENM_NQ * ENM_CSTRIDE,
-                             P.deferred_seg, st);
+                             P.deferred_seg, g.ticket, st);
         nl += g.dclass == 1 ? 2 : 1;
         return chk(e, "count");
     };
@@ -1131,16 +1206,19 @@ int enqueue(enm_ctx* ctx, Plan& P, const double* arena, double* out, double* d_e
     return ENM_OK;
 }
 
-std::string plan_key(const enm_instance* inst, int n_inst, const int64_t* seg_len, const void* arena,
-                     const void* out, const void* eps, const void* cnt, uint32_t flags, int q, int w) {
+// Plans are cached by the shape of the call: the instances (column offsets,
+// not the arena's address), class sizes, flags, which exports are asked for
+// and the tuning.  Device pointers are kernel arguments of every enqueue, so
+// a plan serves any arena and outputs; a captured graph holds the pointers
+// it was captured with and is captured again when they change.
+std::string plan_key(const enm_instance* inst, int n_inst, const int64_t* seg_len, bool eps, bool cnt,
+                     uint32_t flags, uint64_t gen) {
     std::string k;
     k.append((const char*)inst, sizeof(enm_instance) * (size_t)n_inst);
     for (int i = 0; i < n_inst; ++i)
         if (inst[i].n_segs > 1)
             k.append((const char*)(seg_len + inst[i].seg_index), sizeof(int64_t) * (size_t)inst[i].n_segs);
-    const void* ptrs[4] = {arena, out, eps, cnt};
-    k.append((const char*)ptrs, sizeof ptrs);
-    int misc[3] = {(int)flags, q, w};
+    const uint64_t misc[4] = {flags, gen, eps ? 1u : 0u, cnt ? 1u : 0u};
     k.append((const char*)misc, sizeof misc);
     return k;
 }
@@ -1227,24 +1305,45 @@ int enm_run_batch(enm_ctx* ctx, const enm_instance* inst, int32_t n_inst, const 
     }
     ctx->last_stream = st;
     const bool use_graph = (flags & ENM_F_GRAPH) != 0;
-    const std::string key = plan_key(inst, n_inst, seg_len, d_arena, d_out, d_eps, d_counts,
-                                     flags & ~(uint32_t)ENM_F_TIMING, ctx->strip, ctx->waves_per_cu);
-    // Reuse a cached plan for an identical call.
+    const bool timing = (flags & ENM_F_TIMING) != 0;
+    const std::string key = plan_key(inst, n_inst, seg_len, d_eps != nullptr, d_counts != nullptr,
+                                     flags & ~(uint32_t)ENM_F_TIMING, ctx->tuning_gen);
+    const void* ptrs[4] = {d_arena, d_out, d_eps, d_counts};
+    // Capture the plan's launch sequence with these pointers and launch it.
+    auto capture_and_launch = [&](Plan& P) -> int {
+        if (P.graph) { (void)hipGraphExecDestroy(P.graph); P.graph = nullptr; }
+        hipGraph_t graph = nullptr;
+        hipError_t er = hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal);
+        if (er != hipSuccess) return hip_fail(ctx, er, "hipStreamBeginCapture");
+        int r = enqueue(ctx, P, d_arena, d_out, d_eps, d_counts, st, false);
+        hipError_t e2 = hipStreamEndCapture(st, &graph);
+        if (r) { if (graph) (void)hipGraphDestroy(graph); return r; }
+        if (e2 != hipSuccess) return hip_fail(ctx, e2, "hipStreamEndCapture");
+        er = hipGraphInstantiate(&P.graph, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        if (er != hipSuccess) { P.graph = nullptr; return hip_fail(ctx, er, "hipGraphInstantiate"); }
+        std::memcpy(P.graph_ptrs, ptrs, sizeof ptrs);
+        er = hipGraphLaunch(P.graph, st);
+        if (er != hipSuccess) return hip_fail(ctx, er, "hipGraphLaunch");
+        return ENM_OK;
+    };
+    // Reuse a cached plan of the same shape.
     for (auto it = ctx->cache.begin(); it != ctx->cache.end(); ++it) {
         if ((*it)->key == key) {
             Plan* P = it->get();
             if (it != ctx->cache.begin()) ctx->cache.splice(ctx->cache.begin(), ctx->cache, it);
             ctx->last = P;
-            if (use_graph && P->graph) {
-                P->timing = false;
-                e = hipGraphLaunch(P->graph, st);
-                if (e != hipSuccess) return hip_fail(ctx, e, "hipGraphLaunch");
-                P->n_launch = 0;
-                return ENM_OK;
-            }
             P->timing = false;
-            int rc = enqueue(ctx, *P, d_arena, d_out, d_eps, d_counts, st, (flags & ENM_F_TIMING) != 0);
-            return rc;
+            if (use_graph) {
+                if (P->graph && std::memcmp(P->graph_ptrs, ptrs, sizeof ptrs) == 0) {
+                    e = hipGraphLaunch(P->graph, st);
+                    if (e != hipSuccess) return hip_fail(ctx, e, "hipGraphLaunch");
+                    P->n_launch = 0;
+                    return ENM_OK;
+                }
+                return capture_and_launch(*P);
+            }
+            return enqueue(ctx, *P, d_arena, d_out, d_eps, d_counts, st, timing);
         }
     }
     std::unique_ptr<Plan> up(new Plan());
@@ -1258,28 +1357,27 @@ int enm_run_batch(enm_ctx* ctx, const enm_instance* inst, int32_t n_inst, const 
     int rc = plan_batch(ctx, P, inst, n_inst, seg_len);
     if (rc) return rc;
     // Evict before allocating so a cache full of large plans frees memory first.
-    while (ctx->cache.size() >= kMaxCachedPlans) {
-        if (ctx->cache.back().get() == ctx->last) ctx->last = nullptr;
-        (void)hipStreamSynchronize(st);
-        ctx->cache.pop_back();
-    }
+    auto evict = [&](size_t keep) {
+        while (ctx->cache.size() > keep) {
+            if (ctx->cache.back().get() == ctx->last) ctx->last = nullptr;
+            (void)hipStreamSynchronize(st);
+            ctx->cache.pop_back();
+        }
+    };
+    evict(kMaxCachedPlans - 1);
     rc = allocate_and_upload(ctx, P, st);
+    if (rc == ENM_E_NOMEM && !ctx->cache.empty()) {
+        // Out of device memory with other plans resident: drop them all and
+        // try once more.
+        evict(0);
+        rc = allocate_and_upload(ctx, P, st);
+    }
     if (rc) return rc;
     if (use_graph) {
-        hipGraph_t graph = nullptr;
-        e = hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal);
-        if (e != hipSuccess) return hip_fail(ctx, e, "hipStreamBeginCapture");
-        rc = enqueue(ctx, P, d_arena, d_out, d_eps, d_counts, st, false);
-        hipError_t e2 = hipStreamEndCapture(st, &graph);
-        if (rc) { if (graph) (void)hipGraphDestroy(graph); return rc; }
-        if (e2 != hipSuccess) return hip_fail(ctx, e2, "hipStreamEndCapture");
-        e = hipGraphInstantiate(&P.graph, graph, nullptr, nullptr, 0);
-        (void)hipGraphDestroy(graph);
-        if (e != hipSuccess) return hip_fail(ctx, e, "hipGraphInstantiate");
-        e = hipGraphLaunch(P.graph, st);
-        if (e != hipSuccess) return hip_fail(ctx, e, "hipGraphLaunch");
+        rc = capture_and_launch(P);
+        if (rc) return rc;
     } else {
-        rc = enqueue(ctx, P, d_arena, d_out, d_eps, d_counts, st, (flags & ENM_F_TIMING) != 0);
+        rc = enqueue(ctx, P, d_arena, d_out, d_eps, d_counts, st, timing);
         if (rc) return rc;
     }
     ctx->last = up.get();
@@ -1703,56 +1801,104 @@ int enm_get_stats(enm_ctx* ctx, enm_stats* out) {
     return ENM_OK;
 }
 
+namespace {
+int check_tuning(enm_ctx* ctx, const enm_tuning& t) {
+    if (t.strip_points < 0 || t.strip_points > 65536 || t.strip_points % ENM_TILE != 0)
+        return fail(ctx, ENM_E_ARG, "strip_points must be 0 or a multiple of 64 up to 65536");
+    if (t.waves_per_cu < 1 || t.waves_per_cu > 64) return fail(ctx, ENM_E_ARG, "waves_per_cu must be in 1..64");
+    if (!(t.list_slots == 0 || t.list_slots == 4 || t.list_slots == 8 || t.list_slots == 16 || t.list_slots == 32))
+        return fail(ctx, ENM_E_ARG, "list_slots must be 0, 4, 8, 16 or 32");
+    if (t.items_per_ticket < 0 || t.items_per_ticket > 64)
+        return fail(ctx, ENM_E_ARG, "items_per_ticket must be in 0..64");
+    return ENM_OK;
+}
+}  // namespace
+
 int enm_set_tuning(enm_ctx* ctx, int32_t strip_points, int32_t waves_per_cu) {
     if (!ctx) return ENM_E_ARG;
-    if (strip_points < 0 || strip_points > 65536 || strip_points % ENM_TILE != 0)
-        return fail(ctx, ENM_E_ARG, "strip_points must be 0 or a multiple of 64 up to 65536");
-    if (waves_per_cu < 1 || waves_per_cu > 64) return fail(ctx, ENM_E_ARG, "waves_per_cu must be in 1..64");
-    ctx->strip = strip_points;
-    ctx->waves_per_cu = waves_per_cu;
+    const enm_tuning t{strip_points, waves_per_cu, 0, 0};
+    return enm_set_tuning_ex(ctx, &t);
+}
+
+int enm_set_tuning_ex(enm_ctx* ctx, const enm_tuning* tuning) {
+    if (!ctx || !tuning) return ENM_E_ARG;
+    int rc = check_tuning(ctx, *tuning);
+    if (rc) return rc;
+    ctx->tuning = *tuning;
+    ctx->tuned.clear();
+    ++ctx->tuning_gen;
     return ENM_OK;
 }
 
 int enm_get_tuning(const enm_ctx* ctx, int32_t* strip_points, int32_t* waves_per_cu) {
     if (!ctx) return ENM_E_ARG;
-    if (strip_points) *strip_points = ctx->strip;
-    if (waves_per_cu) *waves_per_cu = ctx->waves_per_cu;
+    if (strip_points) *strip_points = ctx->tuning.strip_points;
+    if (waves_per_cu) *waves_per_cu = ctx->tuning.waves_per_cu;
+    return ENM_OK;
+}
+
+int enm_get_tuning_for(const enm_ctx* ctx, int32_t dims, int32_t k, int64_t n, enm_tuning* out) {
+    if (!ctx || !out || dims < 1 || k < 1 || n < 1) return ENM_E_ARG;
+    *out = tuning_for(ctx, dims, k, n);
     return ENM_OK;
 }
 
 int enm_autotune(enm_ctx* ctx, const enm_instance* inst, int32_t n_inst, const int64_t* seg_len,
                  const double* d_arena, double* d_out, uint32_t flags, void* stream) {
     if (!ctx) return ENM_E_ARG;
+    if (!inst || n_inst <= 0) return fail(ctx, ENM_E_ARG, "no instances");
     hipStream_t st = (hipStream_t)stream;
     hipEvent_t a, b;
     if (hipEventCreate(&a) != hipSuccess) return fail(ctx, ENM_E_HIP, "hipEventCreate");
     if (hipEventCreate(&b) != hipSuccess) { (void)hipEventDestroy(a); return fail(ctx, ENM_E_HIP, "hipEventCreate"); }
+    // The batch's shapes run with the default while it is swept.
+    std::set<std::tuple<int, int, int>> shapes;
+    for (int i = 0; i < n_inst; ++i)
+        shapes.insert(shape_key(inst[i].dx + inst[i].dy + inst[i].dz, inst[i].k, inst[i].n));
+    for (const auto& sk : shapes) ctx->tuned.erase(sk);
+    const enm_tuning saved = ctx->tuning;
+    const uint32_t fl = flags & ~(uint32_t)(ENM_F_GRAPH | ENM_F_TIMING);
+    float best_ms = 1e30f;
+    enm_tuning best = saved;
+    int rc = ENM_OK;
+    auto trial = [&](const enm_tuning& t) -> int {
+        ctx->tuning = t;
+        ++ctx->tuning_gen;
+        // The first call plans and allocates; the second is timed.
+        int r = enm_run_batch(ctx, inst, n_inst, seg_len, d_arena, d_out, nullptr, nullptr, fl, stream);
+        if (r) return r;
+        (void)hipEventRecord(a, st);
+        r = enm_run_batch(ctx, inst, n_inst, seg_len, d_arena, d_out, nullptr, nullptr, fl, stream);
+        (void)hipEventRecord(b, st);
+        if (r) return r;
+        (void)hipEventSynchronize(b);
+        float ms = 0.f;
+        (void)hipEventElapsedTime(&ms, a, b);
+        if (ms < best_ms) { best_ms = ms; best = t; }
+        return ENM_OK;
+    };
+    // Coordinate descent: strip, then waves per CU, then list slots, then
+    // items per ticket, each swept with the best of the ones before.
     const int strips[] = {0, 256, 512, 1024, 2048, 4096};
     const int waves[] = {8, 16, 32};
-    float best = 1e30f;
-    int best_s = ctx->strip, best_w = ctx->waves_per_cu;
-    const uint32_t fl = flags & ~(uint32_t)(ENM_F_GRAPH | ENM_F_TIMING);
-    int rc = ENM_OK;
-    for (int sp : strips) {
-        for (int w : waves) {
-            ctx->strip = sp;
-            ctx->waves_per_cu = w;
-            // First call plans and allocates; the second is timed.
-            rc = enm_run_batch(ctx, inst, n_inst, seg_len, d_arena, d_out, nullptr, nullptr, fl, stream);
-            if (rc) break;
-            (void)hipEventRecord(a, st);
-            rc = enm_run_batch(ctx, inst, n_inst, seg_len, d_arena, d_out, nullptr, nullptr, fl, stream);
-            (void)hipEventRecord(b, st);
-            if (rc) break;
-            (void)hipEventSynchronize(b);
-            float ms = 0.f;
-            (void)hipEventElapsedTime(&ms, a, b);
-            if (ms < best) { best = ms; best_s = sp; best_w = w; }
-        }
-        if (rc) break;
-    }
-    ctx->strip = best_s;
-    ctx->waves_per_cu = best_w;
+    const int slots[] = {0, 8, 16};
+    const int tickets[] = {0, 1, 2, 4};
+    enm_tuning cur = saved;
+    cur.list_slots = 0;
+    cur.items_per_ticket = 0;
+    for (int v : strips) { enm_tuning t = cur; t.strip_points = v; if ((rc = trial(t))) break; }
+    cur = best;
+    if (!rc) for (int v : waves) { enm_tuning t = cur; t.waves_per_cu = v; if ((rc = trial(t))) break; }
+    cur = best;
+    if (!rc) for (int v : slots) { enm_tuning t = cur; t.list_slots = v; if ((rc = trial(t))) break; }
+    cur = best;
+    if (!rc) for (int v : tickets) { enm_tuning t = cur; t.items_per_ticket = v; if ((rc = trial(t))) break; }
+    // The winner becomes the default and is kept for each of the batch's
+    // shapes (later batches of the same shapes use it whatever the default).
+    ctx->tuning = rc ? saved : best;
+    if (!rc)
+        for (const auto& sk : shapes) ctx->tuned[sk] = best;
+    ++ctx->tuning_gen;
     (void)hipEventDestroy(a);
     (void)hipEventDestroy(b);
     return rc;

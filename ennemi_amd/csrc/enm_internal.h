@@ -42,6 +42,9 @@
 #define ENM_LIST_HEAP_GLOBAL -1
 #define ENM_LIST_LDS_MAX 128
 #define ENM_HEAP_GLOBAL_BYTES (256ll << 20)  // most global heap storage of one launch
+// Two-column kNN: strips up to this size are walked a strip per workgroup
+// with the strip and its two neighbours staged in LDS (3 x 1024 x 16 B).
+#define ENM_STRIP_LDS_MAX 1024
 
 // One contiguous key range to sort: keys at arena[src + t], t in [0, len),
 // carried with their instance point index p0 + t.
@@ -115,6 +118,10 @@ struct OrderDev {
     int64_t base_perm;
     int64_t base_c64;
     int64_t base_c32;
+    // >= 0 (even): col64 offset of an interleaved copy of columns 0 and 1,
+    // (c0, c1) of position p at xy + 2p, for the two-column kNN walk (one
+    // 16-byte load per candidate).  -1: none.
+    int64_t xy;
 };
 
 // kNN over one order (its scope ranges are the class segments, or all points).

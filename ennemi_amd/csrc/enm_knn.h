@@ -62,13 +62,14 @@ struct KnnArgs {
     int brute;
     double* heap;       // global heap lists (list class ENM_LIST_HEAP_GLOBAL): per block
     int64_t heap_stride;  // doubles per block in `heap` (64 lanes x the group's largest k+1)
+    int smax;           // strip kernel: the group's largest strip (LDS slot size, points)
 };
 
 // Launch of the kNN kernels of one joint dimension D (one translation unit
 // per dimension class, so the kernels compile in parallel).
 #define ENM_KNN_DIMS(X) X(1) X(2) X(3) X(4) X(5) X(6) X(8) X(12) X(16)
 #define ENM_KNN_DECLARE(D) \
-    hipError_t enm_knn_launch_d##D(const KnnArgs& a, int P, bool f32, bool walk, int grid, \
+    hipError_t enm_knn_launch_d##D(const KnnArgs& a, int P, bool f32, int walk, int grid, \
                                    const double* col64, const float* col32, double* res64, \
                                    unsigned long long* stats, hipStream_t st);
 ENM_KNN_DIMS(ENM_KNN_DECLARE)
@@ -370,47 +371,33 @@ __device__ __forceinline__ int wave_lower_bounds(const double* __restrict__ v, i
 // >= the lane's current radius T (the strip is sorted, so every later one
 // is further still, and T only decreases); the others are evaluated and
 // offered to the list (an insertion at or above T leaves it unchanged).
-// The next candidate's coordinates are loaded before the current one is
-// evaluated.  Adds the lane's evaluated candidates to `visited`.
-template <int D, int P, bool F32>
-__device__ __forceinline__ void walk_strip(KnnTile<D, P, F32>& w, int ks, int ke, int u, int dn, bool live,
-                                           unsigned& visited) {
-    const double* __restrict__ yc = w.c64[1];
-    const double qy = w.q64[1];
+// Candidates are (x, y) pairs of the interleaved copy, one 16-byte load
+// each (the lanes of a wave walk different positions, so every load
+// instruction touches up to 64 cache lines: one per candidate, not two);
+// the next candidate is loaded before the current one is evaluated.  Adds
+// the lane's evaluated candidates to `visited`.
+template <int P, bool F32>
+__device__ __forceinline__ void walk_strip(KnnTile<2, P, F32>& w, const double2* __restrict__ xy, int ks, int ke,
+                                           int u, int dn, bool live, unsigned& visited) {
+    const double qx = w.q64[0], qy = w.q64[1];
     bool up = live && u < ke;
     bool down = live && dn >= ks;
-    double cu[D], cd[D];
-    {
-        const int ju = up ? u : ks, jd = down ? dn : ks;
-#pragma unroll
-        for (int d = 0; d < D; ++d) { cu[d] = w.c64[d][ju]; cd[d] = w.c64[d][jd]; }
-    }
+    double2 cu = xy[up ? u : ks], cd = xy[down ? dn : ks];
     while (wave_any(up || down)) {
-        double nu[D], nd[D];
+        const double2 nu = xy[up && u + 1 < ke ? u + 1 : ks];
+        const double2 nd = xy[down && dn - 1 >= ks ? dn - 1 : ks];
         {
-            const int ju = up && u + 1 < ke ? u + 1 : ks;
-            const int jd = down && dn - 1 >= ks ? dn - 1 : ks;
-#pragma unroll
-            for (int d = 0; d < D; ++d) { nu[d] = w.c64[d][ju]; nd[d] = w.c64[d][jd]; }
-        }
-        {
-            const double dy = cu[1] - qy;
+            const double dy = cu.y - qy;
             up = up && dy < w.T;
-            double m = dy;
-#pragma unroll
-            for (int d = 0; d < D; ++d)
-                if (d != 1) m = fmax(m, fabs(cu[d] - w.q64[d]));
+            const double m = fmax(dy, fabs(cu.x - qx));
             w.L.insert(up ? m : INFINITY);
             w.T = w.L.top();
             visited += up;
         }
         {
-            const double dy = qy - cd[1];
+            const double dy = qy - cd.y;
             down = down && dy < w.T;
-            double m = dy;
-#pragma unroll
-            for (int d = 0; d < D; ++d)
-                if (d != 1) m = fmax(m, fabs(cd[d] - w.q64[d]));
+            const double m = fmax(dy, fabs(cd.x - qx));
             w.L.insert(down ? m : INFINITY);
             w.T = w.L.top();
             visited += down;
@@ -419,8 +406,8 @@ __device__ __forceinline__ void walk_strip(KnnTile<D, P, F32>& w, int ks, int ke
         --dn;
         up = up && u < ke;
         down = down && dn >= ks;
-#pragma unroll
-        for (int d = 0; d < D; ++d) { cu[d] = nu[d]; cd[d] = nd[d]; }
+        cu = nu;
+        cd = nd;
     }
 }
 
@@ -434,9 +421,9 @@ __device__ __forceinline__ void walk_strip(KnnTile<D, P, F32>& w, int ks, int ke
 // evaluates a few dozen candidates instead of the whole tile's window.
 // Returns the lane's number of evaluated candidates (the query itself
 // included).
-template <int D, int P, bool F32>
-__device__ unsigned knn_walk(KnnTile<D, P, F32>& w, const KnnDev& kd, const double* __restrict__ col64,
-                             int lo, int hi, int t0, int i, bool active) {
+template <int P, bool F32>
+__device__ unsigned knn_walk(KnnTile<2, P, F32>& w, const double2* __restrict__ xy, const KnnDev& kd,
+                             const double* __restrict__ col64, int lo, int hi, int t0, int i, bool active) {
     const int S = kd.strip;
     const double* xk = col64 + kd.base_key;  // base order: column 0 sorted over [lo, hi)
     const int s = (t0 - lo) / S;
@@ -448,7 +435,7 @@ __device__ unsigned knn_walk(KnnTile<D, P, F32>& w, const KnnDev& kd, const doub
     }
     {
         const int ss = lo + s * S, se = min(hi, ss + S);
-        walk_strip(w, ss, se, i + 1, i - 1, active, visited);
+        walk_strip(w, xy, ss, se, i + 1, i - 1, active, visited);
     }
     bool ldone = s == 0, rdone = s == nstrip - 1;
     for (int d = 1; !(ldone && rdone); ++d) {
@@ -462,7 +449,7 @@ __device__ unsigned knn_walk(KnnTile<D, P, F32>& w, const KnnDev& kd, const doub
                 ldone = !wave_any(need);
                 if (!ldone) {
                     const int p = wave_lower_bounds(w.c64[1], ks, ke, w.q64[1], need);
-                    walk_strip(w, ks, ke, p, p - 1, need, visited);
+                    walk_strip(w, xy, ks, ke, p, p - 1, need, visited);
                 }
             }
         }
@@ -476,7 +463,7 @@ __device__ unsigned knn_walk(KnnTile<D, P, F32>& w, const KnnDev& kd, const doub
                 rdone = !wave_any(need);
                 if (!rdone) {
                     const int p = wave_lower_bounds(w.c64[1], ks, ke, w.q64[1], need);
-                    walk_strip(w, ks, ke, p, p - 1, need, visited);
+                    walk_strip(w, xy, ks, ke, p, p - 1, need, visited);
                 }
             }
         }
@@ -633,8 +620,8 @@ __device__ void knn_item(const KnnArgs& A, const Item& it, const double* __restr
     if (F32) rechecks += wave_sum_u64(w.rechecks);
 }
 
-// One grid-order item by per-lane walks (knn_walk).
-template <int D, int P>
+// One grid-order item of a two-column order by per-lane walks (knn_walk).
+template <int P>
 __device__ void walk_item(const KnnArgs& A, const Item& it, const double* __restrict__ col64,
                           double* __restrict__ res64, unsigned long long& pairs) {
     const KnnDev kd = A.descs[it.desc];
@@ -642,18 +629,17 @@ __device__ void walk_item(const KnnArgs& A, const Item& it, const double* __rest
     const int64_t n = od.n;
     const int i = it.t0 + (int)threadIdx.x;
     const bool active = i < it.t1;
-    const int ncol = od.ncol;
-    KnnTile<D, P, false> w;
+    KnnTile<2, P, false> w;
 #pragma unroll
-    for (int d = 0; d < D; ++d) {
-        const int c = d < ncol ? d : 0;
-        w.c64[d] = col64 + od.c64 + c * n;
+    for (int d = 0; d < 2; ++d) {
+        w.c64[d] = col64 + od.c64 + d * n;
         w.q64[d] = active ? w.c64[d][i] : 0.0;
     }
+    const double2* xy = reinterpret_cast<const double2*>(col64 + od.xy);
     const int K1 = kd.k + 1;
     w.L.init(K1, heap_base<P>(A));
     w.T = active ? INFINITY : -INFINITY;
-    const unsigned v = knn_walk(w, kd, col64, it.lo, it.hi, it.t0, i, active);
+    const unsigned v = knn_walk(w, xy, kd, col64, it.lo, it.hi, it.t0, i, active);
     const double eps = w.L.top();
     if (active) {
         if (kd.eps >= 0) res64[kd.eps + i] = eps;
@@ -662,7 +648,7 @@ __device__ void walk_item(const KnnArgs& A, const Item& it, const double* __rest
     pairs += wave_sum_u64(v);
 }
 
-template <int D, int P>
+template <int P>
 __global__ __launch_bounds__(ENM_WAVE)
 void knn_walk_kernel(KnnArgs A, const double* __restrict__ col64, double* __restrict__ res64,
                      unsigned long long* __restrict__ stats) {
@@ -671,9 +657,152 @@ void knn_walk_kernel(KnnArgs A, const double* __restrict__ col64, double* __rest
         Item it = A.items[idx];
         it.desc = uni(it.desc); it.tile = uni(it.tile);
         it.lo = uni(it.lo); it.hi = uni(it.hi); it.t0 = uni(it.t0); it.t1 = uni(it.t1);
-        walk_item<D, P>(A, it, col64, res64, pairs);
+        walk_item<P>(A, it, col64, res64, pairs);
     });
     if (threadIdx.x == 0) atomicAdd(stats + ST_KNN_PAIRS, pairs);
+}
+
+// ---------------------------------------------------------------------------
+// kNN of two-column grid orders, a strip per workgroup (knn_strip_kernel).
+//
+// A workgroup takes one strip s of a grid order and stages the (x, y) pairs
+// of strips s-1, s and s+1 in LDS (coalesced loads, three slots of `smax`
+// pairs).  Each thread then finds the k+1 nearest of its queries one after
+// the other: it walks its own strip from its own position in LDS, up and
+// down, a direction stopping at the first candidate whose y distance is at
+// least its current radius T; then the strips outward that T still reaches
+// on the x axis, entering each at its y (binary search) and walking the
+// same way -- in LDS for s-1 and s+1, in global memory further out.  Every
+// candidate that could enter the list is visited, so the radii are those
+// of the per-lane walk and of brute force.  Candidate reads are LDS reads
+// instead of one cache line per lane per load.
+
+constexpr int SK_THREADS_MAX = 512;
+
+// First position in [0, len) of a y-sorted slot whose y is >= qy.
+__device__ __forceinline__ int slot_lower_bound(const double2* __restrict__ w, int len, double qy) {
+    int first = 0, n = len;
+    while (n > 0) {
+        const int half = n >> 1;
+        const bool go = w[first + half].y < qy;
+        first = go ? first + half + 1 : first;
+        n = go ? n - half - 1 : half;
+    }
+    return first;
+}
+
+// The walk of one y-sorted run of candidates w[0, len) from positions u
+// (upward) and dn (downward).
+template <class L>
+__device__ __forceinline__ void run_walk(L& list, double& T, const double2* __restrict__ w, int len, int u, int dn,
+                                         double qx, double qy, unsigned& visited) {
+    bool up = u < len, down = dn >= 0;
+    while (up || down) {
+        if (up) {
+            const double2 c = w[u];
+            const double dy = c.y - qy;
+            if (dy < T) {
+                list.insert(fmax(dy, fabs(c.x - qx)));
+                T = list.top();
+                ++visited;
+                up = ++u < len;
+            } else {
+                up = false;
+            }
+        }
+        if (down) {
+            const double2 c = w[dn];
+            const double dy = qy - c.y;
+            if (dy < T) {
+                list.insert(fmax(dy, fabs(c.x - qx)));
+                T = list.top();
+                ++visited;
+                down = --dn >= 0;
+            } else {
+                down = false;
+            }
+        }
+    }
+}
+
+template <int P>
+__global__ __launch_bounds__(SK_THREADS_MAX)
+void knn_strip_kernel(KnnArgs A, const double* __restrict__ col64, double* __restrict__ res64,
+                      unsigned long long* __restrict__ stats) {
+    extern __shared__ double2 sxy[];   // three slots of A.smax pairs: strips s-1, s, s+1
+    __shared__ unsigned s_ticket;
+    const int NT = (int)blockDim.x;
+    unsigned long long pairs = 0;
+    for_each_block_item(A.qoff, A.counter, &s_ticket, [&](int idx) {
+        const Item it = A.items[idx];
+        const KnnDev kd = A.descs[it.desc];
+        const OrderDev od = A.orders[kd.order];
+        const int S = kd.strip;
+        const double2* __restrict__ xy = reinterpret_cast<const double2*>(col64 + od.xy);
+        const double* __restrict__ xk = col64 + kd.base_key;   // column 0 sorted over the scope range
+        const int lo = it.lo, hi = it.hi;
+        const int s = (it.t0 - lo) / S;
+        const int nstrip = (hi - lo + S - 1) / S;
+        int slo[3], slen[3];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            const int ss = s - 1 + k;
+            slo[k] = lo + ss * S;
+            slen[k] = (ss < 0 || ss >= nstrip) ? 0 : min(hi, slo[k] + S) - slo[k];
+            for (int t = threadIdx.x; t < slen[k]; t += NT) sxy[k * A.smax + t] = xy[slo[k] + t];
+        }
+        __syncthreads();
+        for (int i = it.t0 + (int)threadIdx.x; i < it.t1; i += NT) {
+            const int pos = i - slo[1];
+            const double2 q = sxy[A.smax + pos];
+            RegList<P> list;
+            list.init(kd.k + 1, nullptr);
+            list.insert(0.0);   // the query itself
+            double T = list.top();
+            unsigned visited = 1;
+            run_walk(list, T, sxy + A.smax, slen[1], pos + 1, pos - 1, q.x, q.y, visited);
+            // Strips outward while the radius reaches them on the x axis.
+            bool ldone = s == 0, rdone = s == nstrip - 1;
+            for (int d = 1; !(ldone && rdone); ++d) {
+                if (!ldone) {
+                    const int sl = s - d;
+                    if (sl < 0 || !(q.x - xk[lo + (sl + 1) * S - 1] < T)) {
+                        ldone = true;
+                    } else if (d == 1) {
+                        const double2* w = sxy;
+                        const int p = slot_lower_bound(w, slen[0], q.y);
+                        run_walk(list, T, w, slen[0], p, p - 1, q.x, q.y, visited);
+                    } else {
+                        const double2* w = xy + lo + sl * S;
+                        const int p = slot_lower_bound(w, S, q.y);
+                        run_walk(list, T, w, S, p, p - 1, q.x, q.y, visited);
+                    }
+                }
+                if (!rdone) {
+                    const int sr = s + d;
+                    if (sr >= nstrip || !(xk[lo + sr * S] - q.x < T)) {
+                        rdone = true;
+                    } else if (d == 1) {
+                        const double2* w = sxy + 2 * A.smax;
+                        const int p = slot_lower_bound(w, slen[2], q.y);
+                        run_walk(list, T, w, slen[2], p, p - 1, q.x, q.y, visited);
+                    } else {
+                        const int a = lo + sr * S, len = min(hi, a + S) - a;
+                        const double2* w = xy + a;
+                        const int p = slot_lower_bound(w, len, q.y);
+                        run_walk(list, T, w, len, p, p - 1, q.x, q.y, visited);
+                    }
+                }
+            }
+            const double eps = list.top();
+            if (kd.eps >= 0) res64[kd.eps + i] = eps;
+            if (kd.eps_orig >= 0) res64[kd.eps_orig + A.perm[od.perm + i]] = eps;
+            pairs += visited;
+        }
+        __syncthreads();   // the slots are restaged by the next item
+    });
+    pairs = wave_sum_u64(pairs);
+    if ((threadIdx.x & (ENM_WAVE - 1)) == 0) atomicAdd(stats + ST_KNN_PAIRS, pairs);
 }
 
 template <int D, int P, bool F32>
@@ -705,16 +834,29 @@ void knn_kernel(KnnArgs A, const double* __restrict__ col64, const float* __rest
 }
 
 template <int D, int P>
-hipError_t launch_dp(const KnnArgs& a, bool f32, bool walk, int grid, const double* col64, const float* col32,
+hipError_t launch_dp(const KnnArgs& a, bool f32, int walk, int grid, const double* col64, const float* col32,
                      double* res64, unsigned long long* stats, hipStream_t st) {
     // LDS heaps: 64 lanes x the group's largest K1 doubles per workgroup.
     const unsigned shmem = P == ENM_LIST_HEAP_LDS ? (unsigned)(a.heap_stride * sizeof(double)) : 0u;
-    if constexpr (D >= 2) {
-        if (walk) {
-            hipLaunchKernelGGL((knn_walk_kernel<D, P>), dim3(grid), dim3(ENM_WAVE), shmem, st, a, col64, res64,
-                               stats);
-            return hipGetLastError();
+    // Per-lane walks and strip workgroups exist for two-column grid orders
+    // only; strip workgroups for register lists (their LDS holds strips).
+    if (walk) {
+        if constexpr (D == 2) {
+            if constexpr (P > 0) {
+                if (walk == 2) {
+                    const int nt = a.smax >= SK_THREADS_MAX ? SK_THREADS_MAX : ((a.smax + 63) / 64) * 64;
+                    const unsigned lds = (unsigned)(3 * a.smax * sizeof(double2));
+                    hipLaunchKernelGGL((knn_strip_kernel<P>), dim3(grid), dim3(nt), lds, st, a, col64, res64, stats);
+                    return hipGetLastError();
+                }
+            }
+            if (walk == 1) {
+                hipLaunchKernelGGL((knn_walk_kernel<P>), dim3(grid), dim3(ENM_WAVE), shmem, st, a, col64, res64,
+                                   stats);
+                return hipGetLastError();
+            }
         }
+        return hipErrorInvalidValue;
     }
     if (f32)
         hipLaunchKernelGGL((knn_kernel<D, P, true>), dim3(grid), dim3(ENM_WAVE), shmem, st,
@@ -726,7 +868,7 @@ hipError_t launch_dp(const KnnArgs& a, bool f32, bool walk, int grid, const doub
 }
 
 template <int D>
-hipError_t launch_d(const KnnArgs& a, int P, bool f32, bool walk, int grid, const double* col64,
+hipError_t launch_d(const KnnArgs& a, int P, bool f32, int walk, int grid, const double* col64,
                     const float* col32, double* res64, unsigned long long* stats, hipStream_t st) {
     switch (P) {
         case 4: return launch_dp<D, 4>(a, f32, walk, grid, col64, col32, res64, stats, st);
@@ -744,7 +886,7 @@ hipError_t launch_d(const KnnArgs& a, int P, bool f32, bool walk, int grid, cons
 }  // namespace
 
 #define ENM_KNN_DEFINE(D) \
-    hipError_t enm_knn_launch_d##D(const KnnArgs& a, int P, bool f32, bool walk, int grid, \
+    hipError_t enm_knn_launch_d##D(const KnnArgs& a, int P, bool f32, int walk, int grid, \
                                    const double* col64, const float* col32, double* res64, \
                                    unsigned long long* stats, hipStream_t st) { \
         return launch_d<D>(a, P, f32, walk, grid, col64, col32, res64, stats, st); \

@@ -25,13 +25,13 @@ hipError_t enm_launch_knn(const Item* items, const int* qoff, int n_items, unsig
                           const unsigned long long* maxabs, const int* perm,
                           const double* col64, const float* col32, double* res64,
                           unsigned long long* stats, int dclass, int pclass, bool f32, bool brute,
-                          bool walk, int grid, int k1max, double* heap, hipStream_t st) {
+                          int walk, int grid, int k1max, double* heap, int ticket, int smax, hipStream_t st) {
     if (n_items <= 0) return hipSuccess;
     // Tickets of up to ENM_KCHUNK items, but at least four per wave of the
     // grid, so small launches (one instance) still fill the machine.
-    const int chunk = std::max(1, std::min(ENM_KCHUNK, n_items / (4 * std::max(1, grid))));
+    const int chunk = ticket > 0 ? ticket : std::max(1, std::min(ENM_KCHUNK, n_items / (4 * std::max(1, grid))));
     KnnArgs a{items, qoff, counter, chunk, descs, orders, maxabs, perm, brute ? 1 : 0,
-              heap, (int64_t)ENM_WAVE * k1max};
+              heap, (int64_t)ENM_WAVE * k1max, smax};
     if (pclass == ENM_LIST_HEAP_GLOBAL && heap == nullptr) return hipErrorInvalidValue;
     switch (dclass) {
         case 1: return enm_knn_launch_d1(a, pclass, f32, walk, grid, col64, col32, res64, stats, st);

@@ -58,7 +58,7 @@ hipError_t enm_launch_knn(const Item* items, const int* qoff, int n_items, unsig
                           const unsigned long long* maxabs, const int* perm,
                           const double* col64, const float* col32, double* res64,
                           unsigned long long* stats, int dclass, int pclass, bool f32, bool brute,
-                          bool walk, int grid, int k1max, double* heap, hipStream_t st);
+                          int walk, int grid, int k1max, double* heap, int ticket, int smax, hipStream_t st);
 // Grid of a heap-list kNN launch (ENM_LIST_HEAP_GLOBAL) whose heaps fit
 // ENM_HEAP_GLOBAL_BYTES: at most `grid` blocks of 64 x k1max doubles.
 int enm_knn_heap_grid(int grid, int k1max);
@@ -70,7 +70,7 @@ hipError_t enm_launch_count(const Item* items, const int* qoff, int n_items, uns
                             const double* col64, const float* col32, double* res64, int* res32,
                             unsigned long long* stats, int dclass, bool f32, bool brute,
                             int grid, unsigned* deferred, unsigned* n_deferred, unsigned seg_cap,
-                            hipStream_t st);
+                            int ticket, hipStream_t st);
 hipError_t enm_launch_psi_table(double* tab, int64_t n, hipStream_t st);
 hipError_t enm_launch_psi_tiles(const Item* items, int n_items, const CountDev* descs, const double* psi_tab,
                                 const int* counts, double* res64, int* res32, hipStream_t st);

@@ -287,6 +287,12 @@ void strip_sort_kernel(const GatherItem* __restrict__ items, const SortSeg* __re
             col64[od.c64 + c * n + p] = col64[od.base_c64 + c * n + idx];
             if (with32) col32[od.c32 + c * n + p] = col32[od.base_c32 + c * n + idx];
         }
+        if (od.xy >= 0) {
+            double2 v;
+            v.x = col64[od.base_c64 + idx];
+            v.y = col64[od.base_c64 + n + idx];
+            reinterpret_cast<double2*>(col64 + od.xy)[p] = v;
+        }
     }
 }
 
@@ -401,6 +407,12 @@ void gather_kernel(const GatherItem* __restrict__ items,
             for (int c = 0; c < od.ncol; ++c) {
                 col64[od.c64 + c * n + p] = col64[od.base_c64 + c * n + idx];
                 if (with32) col32[od.c32 + c * n + p] = col32[od.base_c32 + c * n + idx];
+            }
+            if (od.xy >= 0) {
+                double2 v;
+                v.x = col64[od.base_c64 + idx];
+                v.y = col64[od.base_c64 + n + idx];
+                reinterpret_cast<double2*>(col64 + od.xy)[p] = v;
             }
             continue;
         }

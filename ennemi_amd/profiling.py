@@ -2,6 +2,7 @@
 
     python -m ennemi_amd.profiling stats  <kernel_stats.csv>  [title]
     python -m ennemi_amd.profiling traffic <fetch counter_collection.csv> <write counter_collection.csv>
+    python -m ennemi_amd.profiling counters <counter_collection.csv> [kernel substring ...]
 
 `stats` prints a Markdown table of per-kernel calls and durations from a
 `rocprofv3 --kernel-trace --stats --output-format csv` run.  `traffic`
@@ -9,7 +10,11 @@ prints JSON: per kernel, the average FETCH_SIZE and WRITE_SIZE per dispatch
 (KiB as the counters report them), each scaled by what the same counter
 reads for a plain device-to-device copy of known size in the same run (the
 `--copy-calib` step of bench.py: 1 GiB read and 1 GiB written), and their
-sum in bytes per launch.
+sum in bytes per launch.  `counters` prints a Markdown table of the average
+value per dispatch of every counter of a `--pmc` run, per kernel, and for the
+SQ counters the shares of the waves' resident cycles (SQ_WAVE_CYCLES) spent
+waiting for an instruction to issue (SQ_WAIT_INST_ANY) or issuing VALU work
+(SQ_ACTIVE_INST_VALU); TCC_HIT / TCC_MISS give the L2 hit rate.
 """
 
 from __future__ import annotations
@@ -89,12 +94,46 @@ def traffic(fetch_path: str, write_path: str) -> dict:
     return out
 
 
+def counters(path: str, only=()) -> str:
+    per = collections.defaultdict(lambda: collections.defaultdict(float))
+    cnt = collections.defaultdict(collections.Counter)
+    names = []
+    for r in csv.DictReader(open(path)):
+        k = _short(r["Kernel_Name"])
+        if only and not any(o in k for o in only):
+            continue
+        c = r["Counter_Name"]
+        if c not in names:
+            names.append(c)
+        per[k][c] += float(r["Counter_Value"])
+        cnt[k][c] += 1
+    lines = ["| kernel | " + " | ".join(names) + " | derived |", "|---|" + "---:|" * (len(names) + 1)]
+    for k in sorted(per):
+        avg = {c: per[k][c] / max(cnt[k][c], 1) for c in names}
+        derived = []
+        wave = avg.get("SQ_WAVE_CYCLES", 0.0)
+        if wave > 0:
+            if "SQ_WAIT_INST_ANY" in avg:
+                derived.append(f"wait {avg['SQ_WAIT_INST_ANY'] / wave:.2f}")
+            if "SQ_ACTIVE_INST_VALU" in avg:
+                derived.append(f"VALU {avg['SQ_ACTIVE_INST_VALU'] / wave:.2f}")
+        hit, miss = avg.get("TCC_HIT_sum", avg.get("TCC_HIT")), avg.get("TCC_MISS_sum", avg.get("TCC_MISS"))
+        if hit is not None and miss is not None and hit + miss > 0:
+            derived.append(f"L2 hit {hit / (hit + miss):.2f}")
+        lines.append(f"| `{k}` | " + " | ".join(f"{avg[c]:.4g}" for c in names) + " | " +
+                     ", ".join(derived) + " |")
+    return "\n".join(lines)
+
+
 def main(argv) -> int:
     if len(argv) >= 2 and argv[0] == "stats":
         print(kernel_stats(argv[1], argv[2] if len(argv) > 2 else ""))
         return 0
     if len(argv) == 3 and argv[0] == "traffic":
         print(json.dumps(traffic(argv[1], argv[2]), indent=1))
+        return 0
+    if len(argv) >= 2 and argv[0] == "counters":
+        print(counters(argv[1], tuple(argv[2:])))
         return 0
     print(__doc__, file=sys.stderr)
     return 2

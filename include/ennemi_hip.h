@@ -269,20 +269,43 @@ int enm_unique_rows(enm_ctx* ctx, const enm_rowset* sets, int32_t n_sets,
 int enm_get_stats(enm_ctx* ctx, enm_stats* out);
 
 /*
- * Autotuning.  The kNN runs on a grid order: the key-sorted points cut into
- * strips of `strip_points` consecutive points, each strip sorted by the
- * second column (0 = plain key-sorted order); `waves_per_cu` sizes the
- * persistent grids.  enm_autotune times strip sizes {0, 256, ..., 4096} x
- * waves {8, 16, 32} on the given batch (two enm_run_batch calls per shape)
- * and keeps the fastest in the context; enm_set_tuning sets one explicitly
- * (strip a multiple of 64 up to 65536, waves in 1..64).
+ * Tile shapes.  They change how the work is cut, never a result: every
+ * radius, count and estimate is the same bits under every shape.
+ *   strip_points      the kNN runs on a grid order: the key-sorted points
+ *                     cut into strips of this many consecutive points, each
+ *                     strip sorted by the second column (0 = plain
+ *                     key-sorted order); a multiple of 64 up to 65536;
+ *   waves_per_cu      sizes the persistent grids (1..64);
+ *   list_slots        register-list slots of the kNN (4, 8, 16 or 32: at
+ *                     least k+1 are used; 0 = the fewest that hold k+1);
+ *                     lists of k+1 > 32 are heaps whatever this says;
+ *   items_per_ticket  work items a wave takes per pop of its queue (1..64;
+ *                     0 = chosen by the launch size).
+ * The planner takes each instance's shape from the context: the shape
+ * enm_autotune kept for its (joint dimension class, k+1 class,
+ * floor(log2 n)) if there is one, else the default.  enm_autotune sweeps
+ * strips {0, 256, ..., 4096}, then waves {8, 16, 32}, then list slots
+ * {0, 8, 16}, then items per ticket {0, 1, 2, 4}, each with the best of the
+ * ones before (two enm_run_batch calls per shape, the second timed), and
+ * keeps the winner as the default and for each shape of the batch.
+ * enm_set_tuning(_ex) sets the default and forgets the kept shapes.
  */
+typedef struct enm_tuning {
+    int32_t strip_points;
+    int32_t waves_per_cu;
+    int32_t list_slots;
+    int32_t items_per_ticket;
+} enm_tuning;
+
 int enm_autotune(enm_ctx* ctx,
                  const enm_instance* inst, int32_t n_inst,
                  const int64_t* seg_len, const double* d_arena,
                  double* d_out, uint32_t flags, void* stream);
 int enm_set_tuning(enm_ctx* ctx, int32_t strip_points, int32_t waves_per_cu);
+int enm_set_tuning_ex(enm_ctx* ctx, const enm_tuning* tuning);
 int enm_get_tuning(const enm_ctx* ctx, int32_t* strip_points, int32_t* waves_per_cu);
+/* The shape an instance of `dims` joint columns, k and n would run with. */
+int enm_get_tuning_for(const enm_ctx* ctx, int32_t dims, int32_t k, int64_t n, enm_tuning* out);
 
 #ifdef __cplusplus
 }

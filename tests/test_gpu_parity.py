@@ -421,6 +421,21 @@ def _mixed_tuning_batch():
     return b
 
 
+AUTOTUNE_SLOTS = [0, 8, 16, 32]
+AUTOTUNE_TICKETS = [0, 1, 2, 4, 64]
+
+
+def _shapes():
+    """Every strip x waves pair the autotuner sweeps, and list slots and
+    items per ticket on top of the default."""
+    for strip in AUTOTUNE_STRIPS:
+        for waves in AUTOTUNE_WAVES:
+            yield (strip, waves, 0, 0)
+    for slots in AUTOTUNE_SLOTS:
+        for ticket in AUTOTUNE_TICKETS:
+            yield (512, 16, slots, ticket)
+
+
 @pytest.mark.parametrize("flags_name", ["f32", "fp64"])
 def test_every_autotuner_shape_gives_identical_results(flags_name):
     nat = _nat()
@@ -431,13 +446,76 @@ def test_every_autotuner_shape_gives_identical_results(flags_name):
     try:
         ctx.set_tuning(1024, 32)
         ref_vals, ref_eps, ref_cnt = b.run(flags=flags, export=True)
-        for strip in AUTOTUNE_STRIPS:
-            for waves in AUTOTUNE_WAVES:
-                ctx.set_tuning(strip, waves)
-                vals, eps, cnt = b.run(flags=flags, export=True)
-                np.testing.assert_array_equal(vals, ref_vals, err_msg=f"strip={strip} waves={waves}")
-                for i in range(len(vals)):
-                    np.testing.assert_array_equal(eps[i], ref_eps[i], err_msg=f"eps {i} strip={strip}")
-                    np.testing.assert_array_equal(cnt[i], ref_cnt[i], err_msg=f"counts {i} strip={strip}")
+        for shape in _shapes():
+            ctx.set_tuning_ex(*shape)
+            vals, eps, cnt = b.run(flags=flags, export=True)
+            np.testing.assert_array_equal(vals, ref_vals, err_msg=f"shape {shape}")
+            for i in range(len(vals)):
+                np.testing.assert_array_equal(eps[i], ref_eps[i], err_msg=f"eps {i} shape {shape}")
+                np.testing.assert_array_equal(cnt[i], ref_cnt[i], err_msg=f"counts {i} shape {shape}")
     finally:
         ctx.set_tuning(*old)
+
+
+def test_autotune_keeps_the_winner_per_shape():
+    """enm_autotune keeps its winner for the batch's shapes (joint dimension
+    class, k+1 class, log2 n) and the results do not change."""
+    nat = _nat()
+    ctx = nat.context()
+    old = ctx.get_tuning()
+    b = _mixed_tuning_batch()
+    try:
+        ctx.set_tuning(1024, 32)
+        ref = b.run(export=True)
+        b.device(0).autotune()
+        win_a = ctx.tuning_for(2, 3, 9000)             # the first KSG instance's shape
+        assert win_a == ctx.tuning_for(11, 11, 3)      # the winner is also the new default
+        # A second batch of another shape: its winner becomes the default,
+        # the first batch's shapes keep theirs.
+        from ennemi_amd._batch import Batch
+        other = Batch()
+        other.add(nat.KIND_KL, 3, _cols(_data(77, 40_000, 3)))
+        other.device(0).autotune()
+        win_b = ctx.tuning_for(3, 3, 40_000)
+        assert ctx.tuning_for(11, 11, 3) == win_b
+        assert ctx.tuning_for(2, 3, 9000) == win_a
+        ctx.set_tuning(256, 8)
+        assert ctx.tuning_for(2, 3, 9000) == (256, 8, 0, 0)   # set_tuning forgets the kept shapes
+        got = b.run(export=True)
+        np.testing.assert_array_equal(got[0], ref[0])
+        for i in range(len(got[0])):
+            np.testing.assert_array_equal(got[1][i], ref[1][i])
+            np.testing.assert_array_equal(got[2][i], ref[2][i])
+    finally:
+        ctx.set_tuning(*old)
+
+
+def test_plans_are_reused_across_arenas_of_one_shape():
+    """Two batches of the same shape (same sizes and column offsets) but
+    different data and device buffers share one cached plan; each must get
+    its own results, eagerly and as graph replays (a graph captured with the
+    first batch's pointers is captured again for the second)."""
+    from ennemi_amd._batch import Batch
+    nat = _nat()
+
+    def make(seed):
+        a = _data(seed, 5000, 3)
+        b = Batch()
+        b.add(nat.KIND_KSG, 3, _cols(a[:, :1]), _cols(a[:, 1:2]))
+        b.add(nat.KIND_CMI, 4, _cols(a[:, :1]), _cols(a[:, 1:2]), _cols(a[:, 2:]))
+        return b, a
+
+    (b1, a1), (b2, a2) = make(501), make(502)
+    w1 = [orc.estimate_single_mi(a1[:, :1], a1[:, 1:2], 3), orc.estimate_conditional_mi(a1[:, :1], a1[:, 1:2],
+                                                                                          a1[:, 2:], 4)]
+    w2 = [orc.estimate_single_mi(a2[:, :1], a2[:, 1:2], 3), orc.estimate_conditional_mi(a2[:, :1], a2[:, 1:2],
+                                                                                          a2[:, 2:], 4)]
+    for flags in (0, nat.F_GRAPH):
+        d1, d2 = b1.device(flags), b2.device(flags)
+        for _ in range(2):
+            d1.launch()
+            v1 = d1.results()
+            d2.launch()
+            v2 = d2.results()
+            assert all(_close(v, w) for v, w in zip(v1, w1)), (flags, v1, w1)
+            assert all(_close(v, w) for v, w in zip(v2, w2)), (flags, v2, w2)
