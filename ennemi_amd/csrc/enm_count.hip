@@ -562,7 +562,9 @@ hipError_t enm_launch_count(const Item* items, const int* qoff, int n_items, uns
                             int ticket, hipStream_t st) {
     if (n_items <= 0) return hipSuccess;
     // Tickets of up to ENM_KCHUNK items, at least four per wave of the grid.
-    const int chunk = ticket > 0 ? ticket : std::max(1, std::min(ENM_KCHUNK, n_items / (4 * std::max(1, grid))));
+    // Tickets of up to ENM_KCHUNK items (or the tuned items per ticket), but
+    // at least four per wave of the grid, so small launches still fill it.
+    const int chunk = std::max(1, std::min(ticket > 0 ? ticket : ENM_KCHUNK, n_items / (4 * std::max(1, grid))));
     CountArgs a{items, qoff, counter, chunk, descs, orders, maxabs, perm, psi_tab, brute ? 1 : 0};
     if (dclass == 1) {
         // `grid` counts waves; a 1-D workgroup has four.

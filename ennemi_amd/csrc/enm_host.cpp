@@ -300,6 +300,7 @@ struct Planner {
         od.base_c64 = -1;
         od.base_c32 = -1;
         od.xy = -1;
+        od.xy_only = 0;
         for (size_t c = 0; c < cols_src.size(); ++c) od.src[c] = cols_src[c];
         P.n_col64 += od.ncol * I.n;
         P.n_col32 += od.ncol * I.n;
@@ -333,6 +334,7 @@ struct Planner {
         const OrderDev bo = P.orders[base];
         OrderDev od = bo;
         od.xy = -1;
+        od.xy_only = 0;
         od.c64 = P.n_col64;
         od.c32 = P.n_col32;
         od.perm = P.n_perm;
@@ -434,6 +436,7 @@ int plan_batch(enm_ctx* ctx, Plan& P, const enm_instance* inst, int n_inst, cons
     // by one block each, with the smallest block tile that holds the
     // largest strip.
     std::vector<enm_tuning> tunes(n_inst);
+    std::set<int> cols_read;   // orders whose column-major columns a kernel reads
     int max_strip = 0;
     for (int ii = 0; ii < n_inst; ++ii) {
         const int cols = std::max(0, inst[ii].dx) + std::max(0, inst[ii].dy) + std::max(0, inst[ii].dz);
@@ -639,6 +642,7 @@ int plan_batch(enm_ctx* ctx, Plan& P, const enm_instance* inst, int n_inst, cons
             id.ntile[0] = chunks;
         }
         P.kdescs.push_back(kd);
+        if (!strip_items) cols_read.insert(K);
         kpend.insert(kpend.end(), mine.begin(), mine.end());
         ex.eps = kd.eps;
         ex.permA = P.orders[K].perm;
@@ -721,6 +725,12 @@ int plan_batch(enm_ctx* ctx, Plan& P, const enm_instance* inst, int n_inst, cons
         P.insts.push_back(id);
         P.exports.push_back(ex);
     }
+
+    // Grid orders read through their interleaved copy only (by the strip
+    // kNN) skip the column-major layout.
+    for (const CountDev& cd : P.cdescs) cols_read.insert(cd.order);
+    for (size_t o = 0; o < P.orders.size(); ++o)
+        if (P.orders[o].xy >= 0 && !cols_read.count((int)o)) P.orders[o].xy_only = 1;
 
     // Work lists: grouped by kernel class (or per instance in per-instance
     // mode).  Inside a group the items go to ENM_NQ queues, one per XCD: each
@@ -1893,9 +1903,9 @@ int enm_autotune(enm_ctx* ctx, const enm_instance* inst, int32_t n_inst, const i
     if (!rc) for (int v : slots) { enm_tuning t = cur; t.list_slots = v; if ((rc = trial(t))) break; }
     cur = best;
     if (!rc) for (int v : tickets) { enm_tuning t = cur; t.items_per_ticket = v; if ((rc = trial(t))) break; }
-    // The winner becomes the default and is kept for each of the batch's
-    // shapes (later batches of the same shapes use it whatever the default).
-    ctx->tuning = rc ? saved : best;
+    // The winner is kept for each of the batch's shapes; other shapes keep
+    // the default.
+    ctx->tuning = saved;
     if (!rc)
         for (const auto& sk : shapes) ctx->tuned[sk] = best;
     ++ctx->tuning_gen;

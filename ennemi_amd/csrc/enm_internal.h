@@ -122,6 +122,10 @@ struct OrderDev {
     // (c0, c1) of position p at xy + 2p, for the two-column kNN walk (one
     // 16-byte load per candidate).  -1: none.
     int64_t xy;
+    // 1: nothing reads the column-major columns (only the strip kNN, which
+    // reads xy), so the layout writes xy and the permutation only.
+    int32_t xy_only;
+    int32_t pad2;
 };
 
 // kNN over one order (its scope ranges are the class segments, or all points).

@@ -29,7 +29,9 @@ hipError_t enm_launch_knn(const Item* items, const int* qoff, int n_items, unsig
     if (n_items <= 0) return hipSuccess;
     // Tickets of up to ENM_KCHUNK items, but at least four per wave of the
     // grid, so small launches (one instance) still fill the machine.
-    const int chunk = ticket > 0 ? ticket : std::max(1, std::min(ENM_KCHUNK, n_items / (4 * std::max(1, grid))));
+    // Tickets of up to ENM_KCHUNK items (or the tuned items per ticket), but
+    // at least four per wave of the grid, so small launches still fill it.
+    const int chunk = std::max(1, std::min(ticket > 0 ? ticket : ENM_KCHUNK, n_items / (4 * std::max(1, grid))));
     KnnArgs a{items, qoff, counter, chunk, descs, orders, maxabs, perm, brute ? 1 : 0,
               heap, (int64_t)ENM_WAVE * k1max, smax};
     if (pclass == ENM_LIST_HEAP_GLOBAL && heap == nullptr) return hipErrorInvalidValue;

@@ -283,7 +283,7 @@ void strip_sort_kernel(const GatherItem* __restrict__ items, const SortSeg* __re
         const int idx = si[lp(t)];
         const int64_t p = it.pos0 + t;
         perm[od.perm + p] = perm[od.base_perm + idx];
-        for (int c = 0; c < ncol; ++c) {
+        for (int c = 0; c < (od.xy_only ? 0 : ncol); ++c) {
             col64[od.c64 + c * n + p] = col64[od.base_c64 + c * n + idx];
             if (with32) col32[od.c32 + c * n + p] = col32[od.base_c32 + c * n + idx];
         }
@@ -404,7 +404,7 @@ void gather_kernel(const GatherItem* __restrict__ items,
         if (indirect) {
             // idx is a base-order position in the same strip: read locally.
             perm[od.perm + p] = perm[od.base_perm + idx];
-            for (int c = 0; c < od.ncol; ++c) {
+            for (int c = 0; c < (od.xy_only ? 0 : od.ncol); ++c) {
                 col64[od.c64 + c * n + p] = col64[od.base_c64 + c * n + idx];
                 if (with32) col32[od.c32 + c * n + p] = col32[od.base_c32 + c * n + idx];
             }

@@ -13,7 +13,8 @@ reads for a plain device-to-device copy of known size in the same run (the
 sum in bytes per launch.  `counters` prints a Markdown table of the average
 value per dispatch of every counter of a `--pmc` run, per kernel, and for the
 SQ counters the shares of the waves' resident cycles (SQ_WAVE_CYCLES) spent
-waiting for an instruction to issue (SQ_WAIT_INST_ANY) or issuing VALU work
+waiting on anything, memory included (SQ_WAIT_ANY), waiting for an
+instruction to issue (SQ_WAIT_INST_ANY) or issuing VALU work
 (SQ_ACTIVE_INST_VALU); TCC_HIT / TCC_MISS give the L2 hit rate.
 """
 
@@ -113,6 +114,8 @@ def counters(path: str, only=()) -> str:
         derived = []
         wave = avg.get("SQ_WAVE_CYCLES", 0.0)
         if wave > 0:
+            if "SQ_WAIT_ANY" in avg:
+                derived.append(f"waiting {avg['SQ_WAIT_ANY'] / wave:.2f}")
             if "SQ_WAIT_INST_ANY" in avg:
                 derived.append(f"wait {avg['SQ_WAIT_INST_ANY'] / wave:.2f}")
             if "SQ_ACTIVE_INST_VALU" in avg:

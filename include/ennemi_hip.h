@@ -279,15 +279,16 @@ int enm_get_stats(enm_ctx* ctx, enm_stats* out);
  *   list_slots        register-list slots of the kNN (4, 8, 16 or 32: at
  *                     least k+1 are used; 0 = the fewest that hold k+1);
  *                     lists of k+1 > 32 are heaps whatever this says;
- *   items_per_ticket  work items a wave takes per pop of its queue (1..64;
- *                     0 = chosen by the launch size).
+ *   items_per_ticket  most work items a wave takes per pop of its queue
+ *                     (1..64; 0 = 8); fewer when the launch is small, so
+ *                     that every wave of the grid gets work.
  * The planner takes each instance's shape from the context: the shape
  * enm_autotune kept for its (joint dimension class, k+1 class,
  * floor(log2 n)) if there is one, else the default.  enm_autotune sweeps
  * strips {0, 256, ..., 4096}, then waves {8, 16, 32}, then list slots
  * {0, 8, 16}, then items per ticket {0, 1, 2, 4}, each with the best of the
  * ones before (two enm_run_batch calls per shape, the second timed), and
- * keeps the winner as the default and for each shape of the batch.
+ * keeps the winner for each shape of the batch (the default is unchanged).
  * enm_set_tuning(_ex) sets the default and forgets the kept shapes.
  */
 typedef struct enm_tuning {
