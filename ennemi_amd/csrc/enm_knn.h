@@ -679,7 +679,7 @@ void knn_walk_kernel(KnnArgs A, const double* __restrict__ col64, double* __rest
 
 constexpr int SK_THREADS_MAX = 512;
 
-// First position in [0, len) of a y-sorted slot whose y is >= qy.
+// First position in [0, len) of a y-sorted run whose y is >= qy.
 __device__ __forceinline__ int slot_lower_bound(const double2* __restrict__ w, int len, double qy) {
     int first = 0, n = len;
     while (n > 0) {
@@ -691,8 +691,25 @@ __device__ __forceinline__ int slot_lower_bound(const double2* __restrict__ w, i
     return first;
 }
 
+// The same for two queries at once (fixed trip count, the two chains of
+// LDS reads interleaved): the number of entries of w[0, len) below each y.
+__device__ __forceinline__ void slot_lower_bound2(const double2* __restrict__ w, int len, double y0, double y1,
+                                                  int& p0, int& p1) {
+    int top = 1;
+    while (top < len) top <<= 1;
+    p0 = 0;
+    p1 = 0;
+    for (int step = top; step > 0; step >>= 1) {
+        const int j0 = p0 + step, j1 = p1 + step;
+        const double v0 = w[min(j0, len) - 1].y, v1 = w[min(j1, len) - 1].y;
+        if (j0 <= len && v0 < y0) p0 = j0;
+        if (j1 <= len && v1 < y1) p1 = j1;
+    }
+}
+
 // The walk of one y-sorted run of candidates w[0, len) from positions u
-// (upward) and dn (downward).
+// (upward) and dn (downward): a direction stops at the first candidate
+// whose y distance is at least the current radius T.
 template <class L>
 __device__ __forceinline__ void run_walk(L& list, double& T, const double2* __restrict__ w, int len, int u, int dn,
                                          double qx, double qy, unsigned& visited) {
@@ -725,6 +742,105 @@ __device__ __forceinline__ void run_walk(L& list, double& T, const double2* __re
     }
 }
 
+// One query's walk state in run_walk2.
+struct WalkQ {
+    double2 q;
+    double T;
+    int u, dn;
+    bool up, down;
+};
+
+// Two queries walking the same run in lockstep: the four next candidates
+// (up and down of each) are read together, so their LDS latencies overlap;
+// each query still takes its candidates in its own order, up then down.
+template <class L>
+__device__ __forceinline__ void run_walk2(L& l0, WalkQ& a, L& l1, WalkQ& b, const double2* __restrict__ w, int len,
+                                          unsigned& visited) {
+    a.up = a.up && a.u < len;
+    a.down = a.down && a.dn >= 0;
+    b.up = b.up && b.u < len;
+    b.down = b.down && b.dn >= 0;
+    while (a.up || a.down || b.up || b.down) {
+        const double2 ca = w[a.up ? a.u : 0], da = w[a.down ? a.dn : 0];
+        const double2 cb = w[b.up ? b.u : 0], db = w[b.down ? b.dn : 0];
+        if (a.up) {
+            const double dy = ca.y - a.q.y;
+            if (dy < a.T) {
+                l0.insert(fmax(dy, fabs(ca.x - a.q.x)));
+                a.T = l0.top();
+                ++visited;
+                a.up = ++a.u < len;
+            } else {
+                a.up = false;
+            }
+        }
+        if (a.down) {
+            const double dy = a.q.y - da.y;
+            if (dy < a.T) {
+                l0.insert(fmax(dy, fabs(da.x - a.q.x)));
+                a.T = l0.top();
+                ++visited;
+                a.down = --a.dn >= 0;
+            } else {
+                a.down = false;
+            }
+        }
+        if (b.up) {
+            const double dy = cb.y - b.q.y;
+            if (dy < b.T) {
+                l1.insert(fmax(dy, fabs(cb.x - b.q.x)));
+                b.T = l1.top();
+                ++visited;
+                b.up = ++b.u < len;
+            } else {
+                b.up = false;
+            }
+        }
+        if (b.down) {
+            const double dy = b.q.y - db.y;
+            if (dy < b.T) {
+                l1.insert(fmax(dy, fabs(db.x - b.q.x)));
+                b.T = l1.top();
+                ++visited;
+                b.down = --b.dn >= 0;
+            } else {
+                b.down = false;
+            }
+        }
+    }
+}
+
+// Strips further out than the staged neighbours, for one query: in global
+// memory, from distance d = 2 on each side that is not done.
+template <class L>
+__device__ __forceinline__ void far_strips(L& list, double& T, const double2* __restrict__ xy,
+                                           const double* __restrict__ xk, int lo, int hi, int S, int s, int nstrip,
+                                           double2 q, bool ldone, bool rdone, unsigned& visited) {
+    for (int d = 2; !(ldone && rdone); ++d) {
+        if (!ldone) {
+            const int sl = s - d;
+            if (sl < 0 || !(q.x - xk[lo + (sl + 1) * S - 1] < T)) {
+                ldone = true;
+            } else {
+                const double2* w = xy + lo + sl * S;
+                const int p = slot_lower_bound(w, S, q.y);
+                run_walk(list, T, w, S, p, p - 1, q.x, q.y, visited);
+            }
+        }
+        if (!rdone) {
+            const int sr = s + d;
+            if (sr >= nstrip || !(xk[lo + sr * S] - q.x < T)) {
+                rdone = true;
+            } else {
+                const int a = lo + sr * S, len = min(hi, a + S) - a;
+                const double2* w = xy + a;
+                const int p = slot_lower_bound(w, len, q.y);
+                run_walk(list, T, w, len, p, p - 1, q.x, q.y, visited);
+            }
+        }
+    }
+}
+
 template <int P>
 __global__ __launch_bounds__(SK_THREADS_MAX)
 void knn_strip_kernel(KnnArgs A, const double* __restrict__ col64, double* __restrict__ res64,
@@ -751,52 +867,61 @@ void knn_strip_kernel(KnnArgs A, const double* __restrict__ col64, double* __res
             slen[k] = (ss < 0 || ss >= nstrip) ? 0 : min(hi, slo[k] + S) - slo[k];
             for (int t = threadIdx.x; t < slen[k]; t += NT) sxy[k * A.smax + t] = xy[slo[k] + t];
         }
+        // The x bounds of the neighbours: largest x left, smallest x right.
+        const double xl = slen[0] > 0 ? xk[slo[1] - 1] : 0.0;
+        const double xr = slen[2] > 0 ? xk[slo[2]] : 0.0;
         __syncthreads();
-        for (int i = it.t0 + (int)threadIdx.x; i < it.t1; i += NT) {
-            const int pos = i - slo[1];
-            const double2 q = sxy[A.smax + pos];
-            RegList<P> list;
-            list.init(kd.k + 1, nullptr);
-            list.insert(0.0);   // the query itself
-            double T = list.top();
-            unsigned visited = 1;
-            run_walk(list, T, sxy + A.smax, slen[1], pos + 1, pos - 1, q.x, q.y, visited);
-            // Strips outward while the radius reaches them on the x axis.
-            bool ldone = s == 0, rdone = s == nstrip - 1;
-            for (int d = 1; !(ldone && rdone); ++d) {
-                if (!ldone) {
-                    const int sl = s - d;
-                    if (sl < 0 || !(q.x - xk[lo + (sl + 1) * S - 1] < T)) {
-                        ldone = true;
-                    } else if (d == 1) {
-                        const double2* w = sxy;
-                        const int p = slot_lower_bound(w, slen[0], q.y);
-                        run_walk(list, T, w, slen[0], p, p - 1, q.x, q.y, visited);
-                    } else {
-                        const double2* w = xy + lo + sl * S;
-                        const int p = slot_lower_bound(w, S, q.y);
-                        run_walk(list, T, w, S, p, p - 1, q.x, q.y, visited);
-                    }
-                }
-                if (!rdone) {
-                    const int sr = s + d;
-                    if (sr >= nstrip || !(xk[lo + sr * S] - q.x < T)) {
-                        rdone = true;
-                    } else if (d == 1) {
-                        const double2* w = sxy + 2 * A.smax;
-                        const int p = slot_lower_bound(w, slen[2], q.y);
-                        run_walk(list, T, w, slen[2], p, p - 1, q.x, q.y, visited);
-                    } else {
-                        const int a = lo + sr * S, len = min(hi, a + S) - a;
-                        const double2* w = xy + a;
-                        const int p = slot_lower_bound(w, len, q.y);
-                        run_walk(list, T, w, len, p, p - 1, q.x, q.y, visited);
-                    }
-                }
+        const double2* own = sxy + A.smax;
+        // Two queries per thread, NT apart, walked in lockstep.
+        for (int i0 = it.t0 + (int)threadIdx.x; i0 < it.t1; i0 += 2 * NT) {
+            const int i1 = i0 + NT;
+            const bool act1 = i1 < it.t1;
+            RegList<P> l0, l1;
+            l0.init(kd.k + 1, nullptr);
+            l1.init(kd.k + 1, nullptr);
+            WalkQ a, b;
+            const int p0 = i0 - slo[1], p1 = act1 ? i1 - slo[1] : p0;
+            a.q = own[p0];
+            b.q = own[p1];
+            l0.insert(0.0);   // the queries themselves
+            if (act1) l1.insert(0.0);
+            a.T = l0.top();
+            b.T = l1.top();
+            unsigned visited = 1 + (act1 ? 1u : 0u);
+            a.u = p0 + 1; a.dn = p0 - 1; a.up = a.down = true;
+            b.u = p1 + 1; b.dn = p1 - 1; b.up = b.down = act1;
+            run_walk2(l0, a, l1, b, own, slen[1], visited);
+            // The staged neighbours, where the radius reaches them on x.
+            const bool la = slen[0] > 0 && a.q.x - xl < a.T, lb = act1 && slen[0] > 0 && b.q.x - xl < b.T;
+            if (la || lb) {
+                int e0, e1;
+                slot_lower_bound2(sxy, slen[0], a.q.y, b.q.y, e0, e1);
+                a.u = e0; a.dn = e0 - 1; a.up = a.down = la;
+                b.u = e1; b.dn = e1 - 1; b.up = b.down = lb;
+                run_walk2(l0, a, l1, b, sxy, slen[0], visited);
             }
-            const double eps = list.top();
-            if (kd.eps >= 0) res64[kd.eps + i] = eps;
-            if (kd.eps_orig >= 0) res64[kd.eps_orig + A.perm[od.perm + i]] = eps;
+            const bool ra = slen[2] > 0 && xr - a.q.x < a.T, rb = act1 && slen[2] > 0 && xr - b.q.x < b.T;
+            if (ra || rb) {
+                const double2* w = sxy + 2 * A.smax;
+                int e0, e1;
+                slot_lower_bound2(w, slen[2], a.q.y, b.q.y, e0, e1);
+                a.u = e0; a.dn = e0 - 1; a.up = a.down = ra;
+                b.u = e1; b.dn = e1 - 1; b.up = b.down = rb;
+                run_walk2(l0, a, l1, b, w, slen[2], visited);
+            }
+            // Further strips in global memory (a side is done once its
+            // nearer strip was out of reach).
+            far_strips(l0, a.T, xy, xk, lo, hi, S, s, nstrip, a.q, !la, !ra, visited);
+            if (act1) far_strips(l1, b.T, xy, xk, lo, hi, S, s, nstrip, b.q, !lb, !rb, visited);
+            const double e0 = l0.top(), e1 = l1.top();
+            if (kd.eps >= 0) {
+                res64[kd.eps + i0] = e0;
+                if (act1) res64[kd.eps + i1] = e1;
+            }
+            if (kd.eps_orig >= 0) {
+                res64[kd.eps_orig + A.perm[od.perm + i0]] = e0;
+                if (act1) res64[kd.eps_orig + A.perm[od.perm + i1]] = e1;
+            }
             pairs += visited;
         }
         __syncthreads();   // the slots are restaged by the next item
