@@ -402,9 +402,10 @@ def main() -> int:
         import concurrent.futures
         from oracle import ennemi_oracle as orc
         threads = os.cpu_count() or 1
-        n_cpu = args.cpu_lags * x.shape[1]
-        params = [(x[:, v], y, int(lags[li]), int(lags.max()), int(lags.min()), 3, None, None,
-                   np.zeros(1, dtype=int), False, False, True, True) for (li, v) in indices[:n_cpu]]
+        sel = [i for i, (li, v) in enumerate(indices) if li < args.cpu_lags]
+        n_cpu = len(sel)
+        params = [(x[:, indices[i][1]], y, int(lags[indices[i][0]]), int(lags.max()), int(lags.min()), 3,
+                   None, None, np.zeros(1, dtype=int), False, False, True, True) for i in sel]
         t0 = time.perf_counter()
         with concurrent.futures.ThreadPoolExecutor(threads) as ex:
             cvals = np.array(list(ex.map(lambda p: orc.lagged_task(p, backend="kdtree")[0], params)))
@@ -413,7 +414,7 @@ def main() -> int:
                "sample": f"the first {n_cpu} of the 776 tasks (lags {int(lags[0])}.."
                          f"{int(lags[args.cpu_lags - 1])} x 8 columns), end to end per task "
                          f"(window, NaN drop, rescaling, cKDTree), {cpu_s:.1f} s",
-               "max_abs_diff_vs_gpu": float(np.max(np.abs(values[:n_cpu] - cvals))),
+               "max_abs_diff_vs_gpu": float(np.max(np.abs(values[sel] - cvals))),
                "host": cpu_info()}
         log("cpu", cpu)
 

@@ -101,9 +101,11 @@ class Batch:
 class DeviceBatch:
     """A packed batch resident in HBM, launchable many times (bench, graphs)."""
 
-    def __init__(self, batch: Batch, flags: int, export: bool):
+    def __init__(self, batch: Batch, flags: int, export: bool, hints: bool = True):
         import torch
         self.batch = batch
+        # Sort hints of a device-prepared arena (enm_run_batch_hinted).
+        self.hint = getattr(batch, "device_hint", None) if hints else None
         self.ctx = nat.context()
         self.flags = flags
         arena, self.inst_arr, self.seg_arr = batch.pack()
@@ -126,7 +128,8 @@ class DeviceBatch:
             self.ctx.run(self.inst_arr, self.n_inst, self.seg_arr, self.arena.data_ptr(),
                          self.out.data_ptr(), self.eps.data_ptr() if self.eps is not None else 0,
                          self.counts.data_ptr() if self.counts is not None else 0,
-                         self.flags, self.stream.cuda_stream)
+                         self.flags, self.stream.cuda_stream,
+                         self.hint.data_ptr() if self.hint is not None else 0)
 
     def stats(self) -> dict:
         with self.ctx.lock:
@@ -159,11 +162,13 @@ class DeviceBatch:
 
 class ArenaBatch:
     """A batch whose float64 arena was written on the device (the columns
-    of device-prepared estimate_mi tasks, enm_prep_write); instances name
-    their columns by arena offset.  Same interface as Batch."""
+    of device-prepared estimate_mi tasks, enm_prep_write), with the sort
+    hints of its columns when they were made (enm_prep_hints); instances
+    name their columns by arena offset.  Same interface as Batch."""
 
-    def __init__(self, device_arena) -> None:
+    def __init__(self, device_arena, device_hint=None) -> None:
         self.device_arena = device_arena
+        self.device_hint = device_hint
         self._inst: List[tuple] = []
 
     def __len__(self) -> int:
@@ -189,11 +194,11 @@ class ArenaBatch:
         n = np.array([t[2] for t in self._inst], dtype=np.int64)
         return np.concatenate([[0], np.cumsum(n)])
 
-    def device(self, flags: int = 0, export: bool = False):
-        return DeviceBatch(self, flags, export)
+    def device(self, flags: int = 0, export: bool = False, hints: bool = True):
+        return DeviceBatch(self, flags, export, hints)
 
-    def run(self, flags: int = 0, export: bool = False):
-        db = self.device(flags, export)
+    def run(self, flags: int = 0, export: bool = False, hints: bool = True):
+        db = self.device(flags, export, hints)
         db.launch()
         return db.results()
 

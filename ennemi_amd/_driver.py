@@ -365,7 +365,8 @@ def plan_mi(y, x, lag=0, *, k: int = 3, cond=None, cond_lag=0, mask=None,
             drop_nan: bool = False, prep: str = "auto"):
     """Validate and rescale every task of an ``estimate_mi`` call and return
     its batch of GPU instances without running it (for timing the device
-    work alone).  Returns (batch, task indices [(lag index, x column)]).
+    work alone).  Returns (batch, task indices [(lag index, x column)]),
+    the indices in the batch's instance order.
     prep="host" prepares the tasks with NumPy on the host even where the
     device could (for comparing the two)."""
     x_arr = np.asarray(x)
@@ -380,7 +381,10 @@ def plan_mi(y, x, lag=0, *, k: int = 3, cond=None, cond_lag=0, mask=None,
                                  discrete_x, discrete_y, preprocess, drop_nan, prep)
     if tasks.host_values:
         raise ValueError("plan_mi covers the GPU estimators only (not discrete-discrete tasks)")
-    return tasks.batch, indices
+    in_batch = [None] * len(indices)
+    for index, (_, inst) in zip(indices, tasks.slots):
+        in_batch[inst] = index
+    return tasks.batch, in_batch
 
 
 def _plan_mi(y, x, lag, k, cond, cond_lag, mask, discrete_x, discrete_y, preprocess, drop_nan,
@@ -427,78 +431,138 @@ def _device_prep_eligible(x, y, cond, discrete_x: bool, discrete_y: bool) -> boo
     return True
 
 
+class DevicePrep:
+    """The device preparation of one estimate_mi call's continuous tasks
+    (windows, mask and NaN drop, rescaling with the reference's noise, and
+    the sort hints of the prepared columns): the raw series live in HBM, and
+    count() then write() rebuild the batch arena from them
+    (enm_prep_count, enm_prep_write, enm_prep_hints).  Tasks are kept in
+    device order (by x column, then lag); place[t] is task t's slot."""
+
+    def __init__(self, y, x, lag, cond, cond_lag, mask, preprocess, drop_nan, min_lag, max_lag, indices,
+                 hints: bool = True):
+        import torch
+        from ._batch import _stream
+        self.ctx = nat.context()
+        self.dev = torch.device("cuda", self.ctx.device)
+        self.st = _stream()
+        n_rows = len(y)
+        nvar = 1 if x.ndim == 1 else x.shape[1]
+        self.dz = dz = 0 if cond is None else cond.shape[1]
+        series = [y] + ([x] if x.ndim == 1 else [x[:, v] for v in range(nvar)]) + \
+            ([] if cond is None else [cond[:, c] for c in range(dz)])
+        raw = np.empty(n_rows * len(series), dtype=np.float64)
+        for i, col in enumerate(series):
+            raw[i * n_rows:(i + 1) * n_rows] = col
+        min_lag = min(min_lag, 0)
+        max_lag = max(max_lag, 0)
+        w = n_rows - max_lag + min_lag
+        self.n_tasks = n_tasks = len(indices)
+        # Device order: by x column, then lag, so a kernel's consecutive
+        # blocks (one XCD's share) read one x series.
+        order = sorted(range(n_tasks), key=lambda t: (indices[t][1], indices[t][0]))
+        self.place = np.empty(n_tasks, dtype=np.int64)   # task -> its device slot
+        self.place[order] = np.arange(n_tasks)
+        self.tasks = tasks = (nat.EnmPrepTask * n_tasks)()
+        for t, (li, v) in enumerate(indices[u] for u in order):
+            T = tasks[t]
+            T.x_col, T.y_col = (1 + v) * n_rows, 0
+            T.x0, T.y0, T.w, T.dz = max_lag - int(lag[li]), max_lag, w, dz
+            for c in range(dz):
+                T.z_col[c] = (1 + nvar + c) * n_rows
+                T.z0[c] = max_lag - int(cond_lag[li][c])
+        self.preprocess = preprocess
+        self.hints = hints
+        self.flags = (nat.P_DROP_NAN if drop_nan else 0) | (nat.P_RESCALE if preprocess else 0) | \
+            (nat.P_HINTS if hints else 0)
+        self.h_n = np.zeros(n_tasks, dtype=np.int64)
+        self.h_nan = np.zeros(n_tasks, dtype=np.uint32)
+        with torch.cuda.stream(self.st):
+            self.d_raw = torch.from_numpy(raw).to(self.dev, non_blocking=False)
+            self.d_mask = torch.from_numpy(np.ascontiguousarray(mask, dtype=np.uint8)).to(self.dev) \
+                if mask is not None else None
+        self.arena = self.hint = self.d_const = self.d_noise = None
+        self.noise_len = 0
+
+    def _ptrs(self):
+        return self.d_raw.data_ptr(), self.d_mask.data_ptr() if self.d_mask is not None else 0
+
+    def count(self, k: int) -> None:
+        """Kept rows of every task; validation in task order with the
+        reference's messages."""
+        raw, mask = self._ptrs()
+        with self.ctx.lock:
+            self.ctx.prep_count(self.tasks, self.n_tasks, raw, mask, self.flags & ~nat.P_HINTS,
+                                self.h_n, self.h_nan, self.st.cuda_stream)
+        for t in range(self.n_tasks):
+            if self.h_n[self.place[t]] <= k:
+                raise ValueError(K_TOO_LARGE_MSG)
+            if self.h_nan[self.place[t]]:
+                raise ValueError(NANS_LEFT_MSG)
+
+    def write(self) -> None:
+        """The arena (and hints) from the counts of the last count(); the
+        buffers are kept while the counts stay the same."""
+        import torch
+        per = self.h_n * (2 + self.dz)
+        self.arena_off = np.concatenate([[0], np.cumsum(per)[:-1]]).astype(np.int64)
+        total = int(per.sum())
+        noise_len = int(per.max()) if self.preprocess else 0
+        with torch.cuda.stream(self.st):
+            if self.arena is None or self.arena.numel() != total:
+                self.arena = torch.empty(total, dtype=torch.float64, device=self.dev)
+                self.hint = torch.empty(total, dtype=torch.int32, device=self.dev) if self.hints else None
+                self.d_const = torch.zeros(self.n_tasks, dtype=torch.int32, device=self.dev)
+            if self.preprocess and noise_len > self.noise_len:
+                self.d_noise = torch.from_numpy(
+                    np.random.default_rng(NOISE_SEED).normal(0.0, 1e-10, noise_len)).to(self.dev)
+                self.noise_len = noise_len
+        raw, mask = self._ptrs()
+        noise = self.d_noise.data_ptr() if self.preprocess else 0
+        nlen = self.noise_len if self.preprocess else 0
+        with self.ctx.lock:
+            self.ctx.prep_write(self.tasks, self.n_tasks, raw, mask, noise, nlen, self.flags, self.arena_off,
+                                self.arena.data_ptr(), self.d_const.data_ptr(), self.st.cuda_stream)
+            if self.hints:
+                self.ctx.prep_hints(self.tasks, self.n_tasks, raw, mask, noise, nlen, self.flags, self.arena_off,
+                                    self.arena.data_ptr(), self.hint.data_ptr(), self.st.cuda_stream)
+
+    def constant_flags(self) -> np.ndarray:
+        """Per task (task order): bit 0/1/2 when x/y/the condition was
+        constant (left unscaled).  Synchronises the preparation stream."""
+        self.st.synchronize()
+        if not self.preprocess:
+            return np.zeros(self.n_tasks, dtype=np.int32)
+        return self.d_const.cpu().numpy()[self.place]
+
+
 def _plan_mi_device(y, x, lag, k, cond, cond_lag, mask, preprocess, drop_nan, min_lag, max_lag,
                     indices):
     """_plan_mi with each task prepared on the device: the raw series go up
     once, the kept rows of every task are counted there (the host validates
     with those counts, in task order, with the reference's messages), and
     the compacted, rescaled columns are written straight into the batch's
-    arena."""
-    import torch
-    from ._batch import ArenaBatch, _stream
-    ctx = nat.context()
-    dev = torch.device("cuda", ctx.device)
-    st = _stream()
-    n_rows = len(y)
-    nvar = 1 if x.ndim == 1 else x.shape[1]
-    dz = 0 if cond is None else cond.shape[1]
-    series = [y] + ([x] if x.ndim == 1 else [x[:, v] for v in range(nvar)]) + \
-        ([] if cond is None else [cond[:, c] for c in range(dz)])
-    raw = np.empty(n_rows * len(series), dtype=np.float64)
-    for i, col in enumerate(series):
-        raw[i * n_rows:(i + 1) * n_rows] = col
-    min_lag = min(min_lag, 0)
-    max_lag = max(max_lag, 0)
-    w = n_rows - max_lag + min_lag
-    n_tasks = len(indices)
-    tasks = (nat.EnmPrepTask * n_tasks)()
-    for t, (li, v) in enumerate(indices):
-        T = tasks[t]
-        T.x_col, T.y_col = (1 + v) * n_rows, 0
-        T.x0, T.y0, T.w, T.dz = max_lag - int(lag[li]), max_lag, w, dz
-        for c in range(dz):
-            T.z_col[c] = (1 + nvar + c) * n_rows
-            T.z0[c] = max_lag - int(cond_lag[li][c])
-    flags = (nat.P_DROP_NAN if drop_nan else 0) | (nat.P_RESCALE if preprocess else 0)
-    h_n = np.zeros(n_tasks, dtype=np.int64)
-    h_nan = np.zeros(n_tasks, dtype=np.uint32)
-    with torch.cuda.stream(st):
-        d_raw = torch.from_numpy(raw).to(dev, non_blocking=False)
-        d_mask = torch.from_numpy(np.ascontiguousarray(mask, dtype=np.uint8)).to(dev) \
-            if mask is not None else None
-    with ctx.lock:
-        ctx.prep_count(tasks, n_tasks, d_raw.data_ptr(), d_mask.data_ptr() if d_mask is not None else 0,
-                       flags, h_n, h_nan, st.cuda_stream)
-    for t in range(n_tasks):
-        if h_n[t] <= k:
-            raise ValueError(K_TOO_LARGE_MSG)
-        if h_nan[t]:
-            raise ValueError(NANS_LEFT_MSG)
-    per = h_n * (2 + dz)
-    arena_off = np.concatenate([[0], np.cumsum(per)[:-1]]).astype(np.int64)
-    noise_len = int(per.max()) if preprocess else 0
-    with torch.cuda.stream(st):
-        arena = torch.empty(int(per.sum()), dtype=torch.float64, device=dev)
-        d_const = torch.zeros(n_tasks, dtype=torch.int32, device=dev)
-        d_noise = torch.from_numpy(np.random.default_rng(NOISE_SEED).normal(0.0, 1e-10, noise_len)).to(dev) \
-            if preprocess else None
-    with ctx.lock:
-        ctx.prep_write(tasks, n_tasks, d_raw.data_ptr(), d_mask.data_ptr() if d_mask is not None else 0,
-                       d_noise.data_ptr() if d_noise is not None else 0, noise_len, flags, arena_off,
-                       arena.data_ptr(), d_const.data_ptr(), st.cuda_stream)
-    st.synchronize()
-    const = d_const.cpu().numpy() if preprocess else np.zeros(n_tasks, dtype=np.int32)
-    for t in range(n_tasks):
+    arena, with the sort hints of every column."""
+    from ._batch import ArenaBatch
+    prep = DevicePrep(y, x, lag, cond, cond_lag, mask, preprocess, drop_nan, min_lag, max_lag, indices)
+    prep.count(k)
+    prep.write()
+    const = prep.constant_flags()
+    dz = prep.dz
+    for t in range(prep.n_tasks):
         for bit in (1, 2, 4):
             if const[t] & bit and (bit != 4 or dz > 0):
                 warn(PREPROCESS_CONSTANT_DATA_WARNING)
     tasks_out = _Tasks()
-    tasks_out.batch = ArenaBatch(arena)
+    tasks_out.batch = ArenaBatch(prep.arena, prep.hint)
+    tasks_out.batch.prep = prep
     kind = nat.KIND_KSG if dz == 0 else nat.KIND_CMI
-    for t in range(n_tasks):
-        n = int(h_n[t])
-        offs = [int(arena_off[t]) + c * n for c in range(2 + dz)]
-        tasks_out.gpu(tasks_out.batch.add(kind, k, n, offs, 1, 1, dz))
+    for t in range(prep.n_tasks):
+        n = int(prep.h_n[t])
+        offs = [int(prep.arena_off[t]) + c * n for c in range(2 + dz)]
+        tasks_out.batch.add(kind, k, n, offs, 1, 1, dz)
+    for t in range(prep.n_tasks):
+        tasks_out.gpu(int(prep.place[t]))
     return tasks_out
 
 

@@ -28,8 +28,9 @@ F_PER_INSTANCE = 0x8
 F_TIMING = 0x10
 STAGES = ("fill", "sort", "gather", "knn", "count", "combine")
 
-EXPORTED_SYMBOLS = ("enm_init", "enm_shutdown", "enm_last_error", "enm_run_batch",
-                    "enm_run_batch_static", "enm_prep_count", "enm_prep_write", "enm_unique_rows", "enm_get_stats",
+EXPORTED_SYMBOLS = ("enm_init", "enm_shutdown", "enm_last_error", "enm_run_batch", "enm_run_batch_hinted",
+                    "enm_run_batch_static", "enm_prep_count", "enm_prep_write", "enm_prep_hints",
+                    "enm_unique_rows", "enm_get_stats",
                     "enm_autotune", "enm_set_tuning", "enm_set_tuning_ex", "enm_get_tuning",
                     "enm_get_tuning_for")
 # Workspace of enm_run_batch_static, bytes (ENM_STATIC_WORKSPACE_BYTES).
@@ -50,6 +51,7 @@ class EnmInstance(ctypes.Structure):
 PREP_MAXZ = MAX_COLS - 2
 P_DROP_NAN = 0x1
 P_RESCALE = 0x2
+P_HINTS = 0x4
 
 
 class EnmPrepTask(ctypes.Structure):
@@ -107,6 +109,10 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
                                   ctypes.POINTER(ctypes.c_int64), vp, vp, vp, vp,
                                   ctypes.c_uint32, vp]
     lib.enm_run_batch.restype = ctypes.c_int
+    lib.enm_run_batch_hinted.argtypes = [vp, ctypes.POINTER(EnmInstance), ctypes.c_int32,
+                                         ctypes.POINTER(ctypes.c_int64), vp, vp, vp, vp, vp,
+                                         ctypes.c_uint32, vp]
+    lib.enm_run_batch_hinted.restype = ctypes.c_int
     lib.enm_run_batch_static.argtypes = lib.enm_run_batch.argtypes
     lib.enm_run_batch_static.restype = ctypes.c_int
     lib.enm_prep_count.argtypes = [vp, ctypes.POINTER(EnmPrepTask), ctypes.c_int32, vp, vp,
@@ -115,6 +121,9 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
     lib.enm_prep_write.argtypes = [vp, ctypes.POINTER(EnmPrepTask), ctypes.c_int32, vp, vp, vp,
                                    ctypes.c_int64, ctypes.c_uint32, vp, vp, vp, vp]
     lib.enm_prep_write.restype = ctypes.c_int
+    lib.enm_prep_hints.argtypes = [vp, ctypes.POINTER(EnmPrepTask), ctypes.c_int32, vp, vp, vp,
+                                   ctypes.c_int64, ctypes.c_uint32, vp, vp, vp, vp]
+    lib.enm_prep_hints.restype = ctypes.c_int
     lib.enm_unique_rows.argtypes = [vp, ctypes.POINTER(EnmRowset), ctypes.c_int32, vp, vp, vp, vp, vp]
     lib.enm_unique_rows.restype = ctypes.c_int
     lib.enm_get_stats.argtypes = [vp, ctypes.POINTER(EnmStats)]
@@ -160,7 +169,14 @@ class Context:
             raise RuntimeError(f"{what} failed ({rc}): {self.error()}")
 
     def run(self, inst_arr, n_inst: int, seg_arr, arena_ptr: int, out_ptr: int,
-            eps_ptr: int = 0, cnt_ptr: int = 0, flags: int = 0, stream_ptr: int = 0) -> None:
+            eps_ptr: int = 0, cnt_ptr: int = 0, flags: int = 0, stream_ptr: int = 0,
+            hint_ptr: int = 0) -> None:
+        """enm_run_batch, or enm_run_batch_hinted when hint_ptr is given."""
+        if hint_ptr:
+            rc = self.lib.enm_run_batch_hinted(self.ctx, inst_arr, n_inst, seg_arr, arena_ptr, hint_ptr, out_ptr,
+                                               eps_ptr or None, cnt_ptr or None, flags, stream_ptr or None)
+            self.check(rc, "enm_run_batch_hinted")
+            return
         rc = self.lib.enm_run_batch(self.ctx, inst_arr, n_inst, seg_arr, arena_ptr, out_ptr,
                                     eps_ptr or None, cnt_ptr or None, flags, stream_ptr or None)
         self.check(rc, "enm_run_batch")
@@ -186,6 +202,14 @@ class Context:
                                      noise_ptr or None, noise_len, flags, arena_off.ctypes.data,
                                      arena_ptr, const_ptr, stream_ptr or None)
         self.check(rc, "enm_prep_write")
+
+    def prep_hints(self, tasks, n_tasks: int, raw_ptr: int, mask_ptr: int, noise_ptr: int,
+                   noise_len: int, flags: int, arena_off: np.ndarray, arena_ptr: int,
+                   hint_ptr: int, stream_ptr: int = 0) -> None:
+        rc = self.lib.enm_prep_hints(self.ctx, tasks, n_tasks, raw_ptr, mask_ptr or None,
+                                     noise_ptr or None, noise_len, flags, arena_off.ctypes.data,
+                                     arena_ptr, hint_ptr, stream_ptr or None)
+        self.check(rc, "enm_prep_hints")
 
     def unique_rows(self, sets, n_sets: int, values_ptr: int, codes_ptr: int, counts_ptr: int,
                     h_distinct: np.ndarray, stream_ptr: int = 0) -> int:

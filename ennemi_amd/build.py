@@ -25,7 +25,7 @@ LIB = os.path.join(HERE, "libennemi_hip.so")
 # (largest first, so the parallel build starts with the slowest).
 KNN_DIMS = (2, 3, 4, 5, 6, 8, 12, 16, 1)
 SOURCES = ([f"enm_knn_d{d}.hip" for d in KNN_DIMS] +
-           ["enm_count.hip", "enm_sort.hip", "enm_knn.hip", "enm_prep.hip", "enm_unique.hip", "enm_reduce.hip", "enm_host.cpp"])
+           ["enm_count.hip", "enm_sort.hip", "enm_knn.hip", "enm_prep.hip", "enm_hint.hip", "enm_unique.hip", "enm_reduce.hip", "enm_host.cpp"])
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950",

@@ -220,6 +220,16 @@ __device__ __forceinline__ void strip_window(const double* __restrict__ yc, int 
     wave_bounds(yc, s0, s1, lb, ub, a, b);
 }
 
+// The block number that block b of an nb-block launch works on when
+// consecutive numbers should share an XCD's L2: the hardware deals
+// workgroups to the ENM_NQ XCDs round-robin, so this gives XCD x the x-th of
+// ENM_NQ nearly equal contiguous ranges.  A bijection of [0, nb).
+__device__ __forceinline__ int64_t xcd_block(int64_t b, int64_t nb) {
+    const int64_t q = nb / ENM_NQ, r = nb % ENM_NQ;
+    const int64_t x = b % ENM_NQ, k = b / ENM_NQ;
+    return x < r ? x * (q + 1) + k : r * (q + 1) + (x - r) * q + k;
+}
+
 // Persistent work loop of a wave64 workgroup over ENM_NQ queues.  The host
 // places each instance's items in one queue (large instances split into a
 // few contiguous blocks), balancing the queues by work; a workgroup starts on

@@ -1,0 +1,258 @@
+// Sort hints for batches prepared on the device (enm_prep_hints): the
+// ascending order of every task column, derived from one sort per series
+// instead of one sort per column.  They stand in for the per-task kd-tree
+// builds of the reference (ennemi/_entropy_estimators.py:36,100-102,135-139),
+// which enm_run_batch otherwise replaces by a sort of every column.
+//
+// A task column is a window of a raw series with some rows dropped (mask,
+// NaN), standardised as a = (v - mean) / std -- monotone in v, rounding
+// included -- plus the reference's noise N (|N| <= M, the largest noise
+// magnitude drawn): s = a + N.  So:
+//   1. each series' rows in the windows' union are sorted once by (value,
+//      row), NaN last (a NaN row is never kept);
+//   2. a column's sorted series rows, filtered to its task's kept rows (the
+//      kept-row map enm_prep_write made) and renumbered to positions in the
+//      task, are its values' order before the noise; a is recomputed there
+//      from the sorted raw value, bit for bit as the preparation did;
+//   3. the noise can only swap values whose a differ by about 2M or less:
+//      the column cuts into runs wherever neighbours differ by
+//      a - a' > 2M + 2^-48 (|a| + |a'| + 2M + 1), more than the noise and
+//      the rounding of a + N on both sides, so no value crosses a cut, and
+//      each run of two to ENM_HINT_RUN values is re-sorted by (s, position)
+//      in place.  A longer run (ties or near-ties in the raw data) leaves the
+//      column without a hint (hint[out] = -1) and the batch sorts it.
+// The batch checks every hint in full before it trusts it, so a hint is an
+// accelerator, never a source of results.
+#include "enm_device.h"
+#include "enm_launch.h"
+
+#include <algorithm>
+
+namespace {
+
+constexpr int HT = 256;                           // threads per block
+constexpr int HPER = ENM_HINT_BLOCK / HT;         // sorted rows per thread
+constexpr int PPER = ENM_HINT_POS / HT;           // column positions per thread
+
+// Keys of every series range: the raw values, NaN as +inf.
+__global__ __launch_bounds__(HT)
+void hint_keys_kernel(const HintSeries* __restrict__ series, const double* __restrict__ raw,
+                      double* __restrict__ keys) {
+    const HintSeries S = series[blockIdx.y];
+    const int64_t i = (int64_t)blockIdx.x * HT + threadIdx.x;
+    if (i >= S.len) return;
+    const double v = raw[S.col + S.lo + i];
+    keys[S.kbase + i] = isnan(v) ? INFINITY : v;
+}
+
+// The task position of sorted series row e of column C, or -1 when the row
+// is outside the window or dropped.  The task's kept-row map is either in
+// LDS, as its words' bits and the prefix of every other word (6 bytes per
+// 32 rows, so eight blocks fit a CU), or read in place.
+__device__ __forceinline__ int kept_pos(const HintCol& C, const int* __restrict__ sidx, bool in_lds,
+                                        const uint32_t* lbits, const int* lpre, const KeptWord* __restrict__ gmap,
+                                        int64_t e) {
+    if (e >= C.len) return -1;
+    const int64_t r = (int64_t)sidx[C.sb + e] - C.ws;
+    if (r < 0 || r >= C.w) return -1;
+    const int64_t w = r >> 5;
+    const uint32_t bit = 1u << (r & 31);
+    if (in_lds) {
+        const uint32_t bits = lbits[w];
+        if (!(bits & bit)) return -1;
+        return lpre[w >> 1] + ((w & 1) ? __popc(lbits[w - 1]) : 0) + __popc(bits & (bit - 1u));
+    }
+    const KeptWord kw = gmap[w];
+    return (kw.bits & bit) ? kw.prefix + __popc(kw.bits & (bit - 1u)) : -1;
+}
+
+// One block per column: the task's kept-row map into LDS (when it fits),
+// then the series' sorted rows in chunks of ENM_HINT_BLOCK, each compacted
+// in order (ballots, wave offsets) with the running offset carried from
+// chunk to chunk.  Writes the kept rows' positions, in sorted order, to the
+// hint and each one's value before the noise to pre[] (same offsets); a
+// column whose kept rows do not add up to n is marked.  Columns of one
+// series run on one XCD.
+__global__ __launch_bounds__(HT)
+void hint_filter_kernel(const HintCol* __restrict__ cols, const int* __restrict__ sidx,
+                        const double* __restrict__ skey, const KeptWord* __restrict__ kept,
+                        const PrepTaskDev* __restrict__ tasks, const double* __restrict__ stats, int rescale,
+                        int lds_words, int* __restrict__ hint, double* __restrict__ pre, int* __restrict__ bad) {
+    constexpr int NW = HT / ENM_WAVE;
+    extern __shared__ uint32_t smap[];   // lds_words bits, then (lds_words + 1) / 2 prefixes
+    __shared__ int s_wcnt[HPER][NW];
+    const int ci = (int)xcd_block(blockIdx.x, gridDim.x);
+    const HintCol C = cols[ci];
+    const int64_t words = (C.w + 31) / 32;
+    const bool in_lds = words <= lds_words;
+    uint32_t* lbits = smap;
+    int* lpre = reinterpret_cast<int*>(smap + lds_words);
+    if (in_lds)
+        for (int64_t t = threadIdx.x; t < words; t += HT) {
+            const KeptWord kw = kept[C.pm + t];
+            lbits[t] = kw.bits;
+            if (!(t & 1)) lpre[t >> 1] = kw.prefix;
+        }
+    const KeptWord* gmap = kept + C.pm;
+    // The column's standardisation, as the preparation applied it (a
+    // constant variable is left as it is).
+    const PrepTaskDev& T = tasks[C.task];
+    const unsigned cbit = C.slot == 0 ? 1u : C.slot == 1 ? 2u : 4u;
+    const bool scaled = rescale && !(T.flags & cbit);
+    const double mean = scaled ? stats[T.stat + 2 * C.slot] : 0.0;
+    const double sd = scaled ? stats[T.stat + 2 * C.slot + 1] : 1.0;
+    const int lane = threadIdx.x & (ENM_WAVE - 1), wv = threadIdx.x / ENM_WAVE;
+    __syncthreads();
+    int64_t at = 0;
+    for (int64_t e0 = 0; e0 < C.len; e0 += ENM_HINT_BLOCK) {
+        int pos[HPER], below[HPER];
+#pragma unroll
+        for (int j = 0; j < HPER; ++j)
+            pos[j] = kept_pos(C, sidx, in_lds, lbits, lpre, gmap, e0 + j * HT + threadIdx.x);
+#pragma unroll
+        for (int j = 0; j < HPER; ++j) {
+            const unsigned long long m = __builtin_amdgcn_ballot_w64(pos[j] >= 0);
+            below[j] = __popcll(m & ((1ull << lane) - 1ull));
+            if (lane == 0) s_wcnt[j][wv] = __popcll(m);
+        }
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < HPER; ++j) {
+            int before = 0, total = 0;
+#pragma unroll
+            for (int w = 0; w < NW; ++w) {
+                before += w < wv ? s_wcnt[j][w] : 0;
+                total += s_wcnt[j][w];
+            }
+            const int64_t o = at + before + below[j];
+            if (pos[j] >= 0 && o < C.n) {
+                const double v = skey[C.sb + e0 + j * HT + threadIdx.x];
+                hint[C.out + o] = pos[j];
+                pre[C.out + o] = scaled ? (v - mean) / sd : v;
+            }
+            at += total;
+        }
+        __syncthreads();   // s_wcnt is rewritten by the next chunk
+    }
+    if (threadIdx.x == 0) bad[ci] = at != C.n ? 1 : 0;
+}
+
+// M: the largest noise magnitude (0 without noise), folded into *out as
+// IEEE bits (|v| >= 0, so the bit patterns order like the values); *out is
+// zeroed before.
+__global__ __launch_bounds__(HT)
+void hint_noise_max_kernel(const double* __restrict__ noise, int64_t len, unsigned long long* __restrict__ out) {
+    __shared__ double red[HT / ENM_WAVE];
+    double m = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * HT + threadIdx.x; i < len; i += (int64_t)gridDim.x * HT)
+        m = fmax(m, fabs(noise[i]));
+    m = wave_max(m);
+    if ((threadIdx.x & (ENM_WAVE - 1)) == 0) red[threadIdx.x / ENM_WAVE] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double mm = red[0];
+        for (int w = 1; w < HT / ENM_WAVE; ++w) mm = fmax(mm, red[w]);
+        atomicMax(out, (unsigned long long)__double_as_longlong(mm));
+    }
+}
+
+// Whether neighbours with values a' <= a before the noise are far enough
+// apart that the noise cannot swap anything across them (m2 = 2M).
+__device__ __forceinline__ bool is_cut(double ap, double a, double m2) {
+    return a - ap > m2 + 0x1p-48 * (fabs(a) + fabs(ap) + m2 + 1.0);
+}
+
+// Re-sort each run of two to ENM_HINT_RUN positions (positions i .. j-1 with
+// no cut between them, cuts before i and at j) by (value, position), in
+// place, by the thread at its start; a longer run marks the column.  Only
+// run members' values are gathered.
+__global__ __launch_bounds__(HT)
+void hint_run_kernel(const HintCol* __restrict__ cols, int64_t pstride,
+                     const unsigned long long* __restrict__ noise_max, const double* __restrict__ arena,
+                     const double* __restrict__ pre, int* __restrict__ hint, int* __restrict__ bad) {
+    const int64_t b = xcd_block(blockIdx.x, gridDim.x);   // a column's blocks on one XCD
+    const int ci = (int)(b / pstride);
+    const HintCol C = cols[ci];
+    const int64_t p0 = (b - ci * pstride) * ENM_HINT_POS;
+    if (p0 >= C.n || bad[ci]) return;
+    const double m2 = 2.0 * __longlong_as_double((long long)*noise_max);
+    const double* a = pre + C.out;
+    const double* v = arena + C.out;
+    int* h = hint + C.out;
+#pragma unroll
+    for (int jj = 0; jj < PPER; ++jj) {
+        const int64_t i = p0 + jj * HT + threadIdx.x;
+        if (i + 1 >= C.n) continue;
+        const double ai = a[i];
+        if ((i > 0 && !is_cut(a[i - 1], ai, m2)) || is_cut(ai, a[i + 1], m2)) continue;   // no run starts here
+        int64_t j = i + 1;
+        while (j < C.n && j - i <= ENM_HINT_RUN && !is_cut(a[j - 1], a[j], m2)) ++j;
+        if (j - i > ENM_HINT_RUN) {
+            bad[ci] = 1;
+            continue;
+        }
+        for (int64_t u = i + 1; u < j; ++u) {
+            const int hu = h[u];
+            const double su = v[hu];
+            int64_t q = u - 1;
+            for (; q >= i; --q) {
+                const int hq = h[q];
+                const double sq = v[hq];
+                if (sq < su || (sq == su && hq < hu)) break;
+                h[q + 1] = hq;
+            }
+            h[q + 1] = hu;
+        }
+    }
+}
+
+// Columns without a hint: hint[out] = -1.
+__global__ void hint_mark_kernel(const HintCol* __restrict__ cols, int n_cols, const int* __restrict__ bad,
+                                 int* __restrict__ hint) {
+    const int ci = blockIdx.x * blockDim.x + threadIdx.x;
+    if (ci < n_cols && bad[ci] && cols[ci].n > 0) hint[cols[ci].out] = -1;
+}
+
+}  // namespace
+
+hipError_t enm_launch_hint_keys(const HintSeries* series, int n_series, int64_t maxlen, const double* raw,
+                                double* keys, hipStream_t st) {
+    if (n_series <= 0 || maxlen <= 0) return hipSuccess;
+    hipLaunchKernelGGL(hint_keys_kernel, dim3((unsigned)((maxlen + HT - 1) / HT), (unsigned)n_series), dim3(HT), 0,
+                       st, series, raw, keys);
+    return hipGetLastError();
+}
+
+hipError_t enm_launch_hint_filter(const HintCol* cols, int n_cols, int64_t max_w, const int* sidx,
+                                  const double* skey, const KeptWord* kept, const PrepTaskDev* tasks,
+                                  const double* stats, int rescale, int* hint, double* pre, int* bad,
+                                  hipStream_t st) {
+    if (n_cols <= 0) return hipSuccess;
+    // Kept-row maps of up to 64 KB go to LDS; larger ones are read in place.
+    const int64_t words = (max_w + 31) / 32;
+    const size_t lds = (size_t)(words + (words + 1) / 2) * 4;
+    const int lds_words = lds <= 65536 ? (int)words : 0;
+    hipLaunchKernelGGL(hint_filter_kernel, dim3((unsigned)n_cols), dim3(HT), lds_words ? lds : 0, st, cols, sidx,
+                       skey, kept, tasks, stats, rescale, lds_words, hint, pre, bad);
+    return hipGetLastError();
+}
+
+hipError_t enm_launch_hint_runs(const HintCol* cols, int n_cols, int64_t pstride, const double* noise,
+                                int64_t noise_len, double* noise_max, const double* arena, const double* pre,
+                                int* hint, int* bad, hipStream_t st) {
+    if (n_cols <= 0) return hipSuccess;
+    unsigned long long* nm = reinterpret_cast<unsigned long long*>(noise_max);
+    hipError_t e = enm_launch_fill_u64(nm, 1, 0ull, st);
+    if (e != hipSuccess) return e;
+    const int64_t len = noise ? noise_len : 0;
+    if (len > 0) {
+        const int64_t blocks = std::min<int64_t>(1024, (len + 16 * HT - 1) / (16 * HT));
+        hipLaunchKernelGGL(hint_noise_max_kernel, dim3((unsigned)blocks), dim3(HT), 0, st, noise, len, nm);
+    }
+    if (pstride > 0)
+        hipLaunchKernelGGL(hint_run_kernel, dim3((unsigned)(pstride * n_cols)), dim3(HT), 0, st, cols, pstride, nm,
+                           arena, pre, hint, bad);
+    hipLaunchKernelGGL(hint_mark_kernel, dim3((unsigned)((n_cols + 63) / 64)), dim3(64), 0, st, cols, n_cols, bad,
+                       hint);
+    return hipGetLastError();
+}

@@ -140,13 +140,15 @@ struct Plan {
     int n_inst = 0;
     bool f32 = true, brute = false, per_instance = false;
     bool want_eps = false;   // the call exports radii (in point order)
+    bool hinted = false;     // the call passes sort hints (first-stage segments of whole columns)
     // Kernels that read the FP32 column copies: the brute-force kNN and
     // multi-D counts off the grid (band filter).  Only then are they made.
     bool need32 = false;
     // device
     int64_t heap_bytes = 0;   // global heap lists of kNN groups with k+1 > ENM_LIST_LDS_MAX
     DevBuf tables, sk0, si0, sk1, si1, split, col64, col32, perm, res64, res32, maxabs, counters, stats,
-        deferred, heap, samp_k0, samp_k1, samp_i0, samp_i1, samp_split, bid, ss_counts, ss_tiles, ss_ctl;
+        deferred, heap, samp_k0, samp_k1, samp_i0, samp_i1, samp_split, bid, ss_counts, ss_tiles, ss_ctl,
+        hint_bad;
     size_t off_segs = 0, off_gitems = 0, off_orders = 0, off_kdescs = 0, off_cdescs = 0, off_insts = 0,
            off_cls = 0, off_kitems = 0, off_citems = 0, off_segs2 = 0, off_gitems2 = 0, off_rdescs = 0,
            off_ritems = 0, off_sitems = 0, off_qoffs = 0, off_ssegs = 0, off_samp_segs = 0, off_cons = 0,
@@ -154,7 +156,7 @@ struct Plan {
     void* pinned = nullptr;
     hipEvent_t upload_done = nullptr;
     hipGraphExec_t graph = nullptr;
-    const void* graph_ptrs[4] = {};   // arena, out, eps, counts the graph was captured with
+    const void* graph_ptrs[5] = {};   // arena, out, eps, counts, hints the graph was captured with
     int n_launch = 0;
     bool timing = false;       // events recorded by the last enqueue
     hipEvent_t ev[ENM_N_STAGES + 1] = {};
@@ -197,14 +199,26 @@ struct enm_ctx {
     // enm_prep_write, and their workspace.
     struct Prep {
         std::vector<PrepTaskDev> tasks;
-        std::vector<int64_t> blk0, n;
-        int64_t n_blocks = 0;
+        std::vector<int64_t> n;
+        int64_t stride = 0;       // compaction blocks per task (the most any task needs)
         const void* raw = nullptr;
         const void* mask = nullptr;
         uint32_t drop_nan = 0;
         bool counted = false;
         std::vector<char> blob1, blob2;   // host tables of the last uploads
         DevBuf tables1, tables2, blk_count, blk_off, nan, nout, chunk_val, stats;
+        // Sort hints: the kept-row map written by the last enm_prep_write
+        // with ENM_P_HINTS (pm_total words), and what that call was given.
+        int64_t pm_total = 0;
+        size_t o_tasks = 0;       // the task table in tables2 (constant flags set there)
+        bool written = false;
+        std::vector<int64_t> arena_off;
+        const void* arena = nullptr;
+        const void* noise = nullptr;
+        int64_t noise_len = 0;
+        uint32_t flags = 0;
+        std::vector<char> blob3;
+        DevBuf kept, tables3, hkeys, hk0, hk1, hi0, hi1, hsplit, hpre, hbad, hmax;
     } prep;
     // Workspace of enm_unique_rows.
     struct Unique {
@@ -266,7 +280,8 @@ struct Planner {
                 sg.len = (int32_t)len;
                 sg.first_tile = P.ntiles;
                 sg.p0 = (int32_t)p0;
-                sg.pad = 0;
+                // A whole column's order can come from the caller's hints.
+                sg.hint = P.hinted && !segmented ? 1 : 0;
                 P.ntiles += (int)((len + ENM_SORT_TILE - 1) / ENM_SORT_TILE);
                 P.sort_elems += len;
                 P.maxlen = std::max(P.maxlen, len);
@@ -357,7 +372,7 @@ struct Planner {
                 sg.len = len;
                 sg.first_tile = P.ntiles2;
                 sg.p0 = st;
-                sg.pad = 0;
+                sg.hint = 0;
                 P.ntiles2 += (len + P.tile2 - 1) / P.tile2;
                 P.sort_elems2 += len;
                 P.maxlen2 = std::max<int64_t>(P.maxlen2, len);
@@ -865,7 +880,7 @@ int plan_batch(enm_ctx* ctx, Plan& P, const enm_instance* inst, int n_inst, cons
                 s2.len = q.ns;
                 s2.first_tile = P.samp_tiles;
                 s2.p0 = 0;
-                s2.pad = 0;
+                s2.hint = 0;
                 P.samp_tiles += (q.ns + ENM_SORT_TILE - 1) / ENM_SORT_TILE;
                 P.samp_maxlen = std::max<int64_t>(P.samp_maxlen, q.ns);
                 P.samp_segs.push_back(s2);
@@ -961,6 +976,7 @@ std::vector<BufSpec> workspace_layout(Plan& P) {
         v.push_back({&P.ss_counts, (size_t)P.n_counts * 4 + 4, "bucket counts"});
         v.push_back({&P.ss_tiles, (size_t)P.n_buckets * sizeof(SsTile), "bucket runs"});
         v.push_back({&P.ss_ctl, 2 * ENM_NQ * ENM_CSTRIDE * 4, "run queue counters"});
+        if (P.hinted) v.push_back({&P.hint_bad, P.segs.size() * 4 + 4, "hint checks"});
     }
     return v;
 }
@@ -1000,7 +1016,7 @@ const T* tab(const Plan& P, size_t off) {
 
 // Enqueue the whole launch sequence of a plan on `st`.  With `timing`, an
 // event is recorded on `st` before the first and after every stage.
-int enqueue(enm_ctx* ctx, Plan& P, const double* arena, double* out, double* d_eps, int* d_counts,
+int enqueue(enm_ctx* ctx, Plan& P, const double* arena, const int* hint, double* out, double* d_eps, int* d_counts,
             hipStream_t st, bool timing) {
     hipError_t e;
     int nl = 0;
@@ -1028,6 +1044,11 @@ int enqueue(enm_ctx* ctx, Plan& P, const double* arena, double* out, double* d_e
         return rc;
     if ((rc = chk(enm_launch_psi_table(P.res64.as<double>() + P.psi_tab, P.psi_n, st), "fill"))) return rc;
     nl += 4;
+    const bool hints = P.hinted && P.sample_sort && hint != nullptr;
+    if (hints) {
+        if ((rc = chk(enm_launch_fill_i32(P.hint_bad.as<int>(), (int64_t)P.segs.size(), 0, st), "fill"))) return rc;
+        ++nl;
+    }
     if ((rc = mark(1))) return rc;
     int fb = 0;
     if (P.sample_sort) {
@@ -1059,6 +1080,8 @@ int enqueue(enm_ctx* ctx, Plan& P, const double* arena, double* out, double* d_e
         a.perm = P.perm.as<int>();
         a.maxabs = P.maxabs.as<unsigned long long>();
         a.with32 = P.f32 && P.need32 ? 1 : 0;
+        a.hint = hints ? hint : nullptr;
+        a.hint_bad = hints ? P.hint_bad.as<int>() : nullptr;
         if ((rc = chk(enm_launch_sample_sort(a, arena, P.sk0.as<double>(), P.si0.as<int>(), P.sk1.as<double>(),
                                              P.si1.as<int>(), &nl, st), "sort")))
             return rc;
@@ -1222,13 +1245,13 @@ int enqueue(enm_ctx* ctx, Plan& P, const double* arena, double* out, double* d_e
 // a plan serves any arena and outputs; a captured graph holds the pointers
 // it was captured with and is captured again when they change.
 std::string plan_key(const enm_instance* inst, int n_inst, const int64_t* seg_len, bool eps, bool cnt,
-                     uint32_t flags, uint64_t gen) {
+                     bool hinted, uint32_t flags, uint64_t gen) {
     std::string k;
     k.append((const char*)inst, sizeof(enm_instance) * (size_t)n_inst);
     for (int i = 0; i < n_inst; ++i)
         if (inst[i].n_segs > 1)
             k.append((const char*)(seg_len + inst[i].seg_index), sizeof(int64_t) * (size_t)inst[i].n_segs);
-    const uint64_t misc[4] = {flags, gen, eps ? 1u : 0u, cnt ? 1u : 0u};
+    const uint64_t misc[5] = {flags, gen, eps ? 1u : 0u, cnt ? 1u : 0u, hinted ? 1u : 0u};
     k.append((const char*)misc, sizeof misc);
     return k;
 }
@@ -1301,6 +1324,13 @@ const char* enm_last_error(const enm_ctx* ctx) {
 int enm_run_batch(enm_ctx* ctx, const enm_instance* inst, int32_t n_inst, const int64_t* seg_len,
                   const double* d_arena, double* d_out, double* d_eps, int32_t* d_counts, uint32_t flags,
                   void* stream) {
+    return enm_run_batch_hinted(ctx, inst, n_inst, seg_len, d_arena, nullptr, d_out, d_eps, d_counts, flags,
+                                stream);
+}
+
+int enm_run_batch_hinted(enm_ctx* ctx, const enm_instance* inst, int32_t n_inst, const int64_t* seg_len,
+                         const double* d_arena, const int32_t* d_hint, double* d_out, double* d_eps,
+                         int32_t* d_counts, uint32_t flags, void* stream) {
     if (!ctx) { g_init_error = "ctx is NULL"; return ENM_E_ARG; }
     ctx->err.clear();
     if (!inst || n_inst <= 0) return fail(ctx, ENM_E_ARG, "no instances");
@@ -1317,15 +1347,15 @@ int enm_run_batch(enm_ctx* ctx, const enm_instance* inst, int32_t n_inst, const 
     const bool use_graph = (flags & ENM_F_GRAPH) != 0;
     const bool timing = (flags & ENM_F_TIMING) != 0;
     const std::string key = plan_key(inst, n_inst, seg_len, d_eps != nullptr, d_counts != nullptr,
-                                     flags & ~(uint32_t)ENM_F_TIMING, ctx->tuning_gen);
-    const void* ptrs[4] = {d_arena, d_out, d_eps, d_counts};
+                                     d_hint != nullptr, flags & ~(uint32_t)ENM_F_TIMING, ctx->tuning_gen);
+    const void* ptrs[5] = {d_arena, d_out, d_eps, d_counts, d_hint};
     // Capture the plan's launch sequence with these pointers and launch it.
     auto capture_and_launch = [&](Plan& P) -> int {
         if (P.graph) { (void)hipGraphExecDestroy(P.graph); P.graph = nullptr; }
         hipGraph_t graph = nullptr;
         hipError_t er = hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal);
         if (er != hipSuccess) return hip_fail(ctx, er, "hipStreamBeginCapture");
-        int r = enqueue(ctx, P, d_arena, d_out, d_eps, d_counts, st, false);
+        int r = enqueue(ctx, P, d_arena, d_hint, d_out, d_eps, d_counts, st, false);
         hipError_t e2 = hipStreamEndCapture(st, &graph);
         if (r) { if (graph) (void)hipGraphDestroy(graph); return r; }
         if (e2 != hipSuccess) return hip_fail(ctx, e2, "hipStreamEndCapture");
@@ -1353,7 +1383,7 @@ int enm_run_batch(enm_ctx* ctx, const enm_instance* inst, int32_t n_inst, const 
                 }
                 return capture_and_launch(*P);
             }
-            return enqueue(ctx, *P, d_arena, d_out, d_eps, d_counts, st, timing);
+            return enqueue(ctx, *P, d_arena, d_hint, d_out, d_eps, d_counts, st, timing);
         }
     }
     std::unique_ptr<Plan> up(new Plan());
@@ -1364,6 +1394,7 @@ int enm_run_batch(enm_ctx* ctx, const enm_instance* inst, int32_t n_inst, const 
     P.need32 = P.brute;
     P.per_instance = (flags & ENM_F_PER_INSTANCE) != 0;
     P.want_eps = d_eps != nullptr;
+    P.hinted = d_hint != nullptr;
     int rc = plan_batch(ctx, P, inst, n_inst, seg_len);
     if (rc) return rc;
     // Evict before allocating so a cache full of large plans frees memory first.
@@ -1387,7 +1418,7 @@ int enm_run_batch(enm_ctx* ctx, const enm_instance* inst, int32_t n_inst, const 
         rc = capture_and_launch(P);
         if (rc) return rc;
     } else {
-        rc = enqueue(ctx, P, d_arena, d_out, d_eps, d_counts, st, timing);
+        rc = enqueue(ctx, P, d_arena, d_hint, d_out, d_eps, d_counts, st, timing);
         if (rc) return rc;
     }
     ctx->last = up.get();
@@ -1446,7 +1477,7 @@ int enm_run_batch_static(enm_ctx* ctx, const enm_instance* inst, int32_t n_inst,
     if (e != hipSuccess) return hip_fail(ctx, e, "hipMemcpyAsync(tables)");
     e = hipStreamSynchronize(st);
     if (e != hipSuccess) return hip_fail(ctx, e, "hipStreamSynchronize");
-    rc = enqueue(ctx, P, d_arena, d_out, d_eps, d_counts, st, false);
+    rc = enqueue(ctx, P, d_arena, nullptr, d_out, d_eps, d_counts, st, false);
     if (rc) return rc;
     ctx->static_plan = std::move(up);
     ctx->last = ctx->static_plan.get();
@@ -1465,8 +1496,8 @@ int enm_prep_count(enm_ctx* ctx, const enm_prep_task* tasks, int32_t n_tasks, co
     if (e != hipSuccess) return hip_fail(ctx, e, "hipSetDevice");
     hipStream_t st = (hipStream_t)stream;
     S.tasks.assign(n_tasks, PrepTaskDev{});
-    S.blk0.assign(n_tasks, 0);
-    int64_t nb = 0;
+    S.written = false;
+    int64_t stride = 0, pm = 0;
     for (int t = 0; t < n_tasks; ++t) {
         const enm_prep_task& T = tasks[t];
         if (T.w < 1 || T.w > 0x7fffffffLL || T.dz < 0 || T.dz > ENM_PREP_MAXZ || T.x0 < 0 || T.y0 < 0) {
@@ -1479,13 +1510,15 @@ int enm_prep_count(enm_ctx* ctx, const enm_prep_task* tasks, int32_t n_tasks, co
         D.x_col = T.x_col; D.y_col = T.y_col; D.x0 = T.x0; D.y0 = T.y0; D.w = T.w; D.dz = T.dz;
         for (int c = 0; c < T.dz; ++c) { D.z_col[c] = T.z_col[c]; D.z0[c] = T.z0[c]; }
         D.nblk = (int32_t)((T.w + ENM_PREP_BLOCK - 1) / ENM_PREP_BLOCK);
-        S.blk0[t] = nb;
-        nb += D.nblk;
+        D.pmap = pm;
+        pm += (T.w + 31) / 32;
+        stride = std::max<int64_t>(stride, D.nblk);
     }
-    S.n_blocks = nb;
+    const int64_t nb = stride * n_tasks;
+    S.stride = stride;
+    S.pm_total = pm;
     S.blob1.clear();
     const size_t off_t = append(S.blob1, S.tasks);
-    const size_t off_b = append(S.blob1, S.blk0);
     if (S.tables1.ensure(S.blob1.size()) != hipSuccess || S.blk_count.ensure(nb * 4 + 4) != hipSuccess ||
         S.blk_off.ensure(nb * 8 + 8) != hipSuccess || S.nan.ensure((size_t)n_tasks * 4) != hipSuccess ||
         S.nout.ensure((size_t)n_tasks * 8) != hipSuccess) {
@@ -1495,8 +1528,7 @@ int enm_prep_count(enm_ctx* ctx, const enm_prep_task* tasks, int32_t n_tasks, co
     e = hipMemcpyAsync(S.tables1.p, S.blob1.data(), S.blob1.size(), hipMemcpyHostToDevice, st);
     if (e != hipSuccess) return hip_fail(ctx, e, "hipMemcpyAsync(prep tables)");
     const PrepTaskDev* dt = reinterpret_cast<const PrepTaskDev*>(S.tables1.as<char>() + off_t);
-    const int64_t* db0 = reinterpret_cast<const int64_t*>(S.tables1.as<char>() + off_b);
-    e = enm_launch_prep_count(dt, db0, n_tasks, nb, d_raw, d_mask, (flags & ENM_P_DROP_NAN) ? 1 : 0,
+    e = enm_launch_prep_count(dt, stride, n_tasks, d_raw, d_mask, (flags & ENM_P_DROP_NAN) ? 1 : 0,
                               S.blk_count.as<int>(), S.blk_off.as<int64_t>(), S.nan.as<unsigned>(),
                               S.nout.as<int64_t>(), st);
     if (e != hipSuccess) return hip_fail(ctx, e, "prep count");
@@ -1528,8 +1560,7 @@ int enm_prep_write(enm_ctx* ctx, const enm_prep_task* tasks, int32_t n_tasks, co
     hipStream_t st = (hipStream_t)stream;
     std::vector<PrepSumDev> sums;
     std::vector<int> chunk_sum_of, chunk_index;
-    std::vector<int64_t> sblk0(n_tasks, 0);
-    int64_t nsb = 0, need_noise = 0;
+    int64_t sstride = 0, need_noise = 0;
     int stat = 0;
     for (int t = 0; t < n_tasks; ++t) {
         const enm_prep_task& T = tasks[t];
@@ -1542,8 +1573,7 @@ int enm_prep_write(enm_ctx* ctx, const enm_prep_task* tasks, int32_t n_tasks, co
         D.flags = 0;
         const int64_t n = D.n;
         need_noise = std::max<int64_t>(need_noise, n * (2 + D.dz));
-        sblk0[t] = nsb;
-        nsb += (n + 255) / 256;
+        sstride = std::max<int64_t>(sstride, (n + 255) / 256);
         // x, y and a one-column condition: pairwise sums; the columns of a
         // multi-column condition: row-by-row sums.
         const int narr = 2 + D.dz;
@@ -1575,33 +1605,168 @@ int enm_prep_write(enm_ctx* ctx, const enm_prep_task* tasks, int32_t n_tasks, co
     }
     S.blob2.clear();
     const size_t o_t = append(S.blob2, S.tasks);
-    const size_t o_b = append(S.blob2, S.blk0);
     const size_t o_s = append(S.blob2, sums);
     const size_t o_cs = append(S.blob2, chunk_sum_of);
     const size_t o_ci = append(S.blob2, chunk_index);
-    const size_t o_sb = append(S.blob2, sblk0);
+    const bool hints = (flags & ENM_P_HINTS) != 0;
     if (S.tables2.ensure(S.blob2.size()) != hipSuccess ||
         S.chunk_val.ensure(chunk_sum_of.size() * 8 + 8) != hipSuccess ||
-        S.stats.ensure((size_t)stat * 8 + 8) != hipSuccess) {
+        S.stats.ensure((size_t)stat * 8 + 8) != hipSuccess ||
+        (hints && S.kept.ensure((size_t)S.pm_total * sizeof(KeptWord) + 8) != hipSuccess)) {
         (void)hipGetLastError();
         return fail(ctx, ENM_E_NOMEM, "hipMalloc failed for the preparation workspace");
     }
     e = hipMemcpyAsync(S.tables2.p, S.blob2.data(), S.blob2.size(), hipMemcpyHostToDevice, st);
     if (e != hipSuccess) return hip_fail(ctx, e, "hipMemcpyAsync(prep tables)");
     char* base = S.tables2.as<char>();
-    e = enm_launch_prep_write(reinterpret_cast<PrepTaskDev*>(base + o_t), reinterpret_cast<const int64_t*>(base + o_b),
-                              n_tasks, S.n_blocks, d_raw, d_mask, S.drop_nan ? 1 : 0, rescale ? 1 : 0,
-                              S.blk_off.as<int64_t>(), reinterpret_cast<const PrepSumDev*>(base + o_s),
-                              (int)sums.size(), reinterpret_cast<const int*>(base + o_cs),
-                              reinterpret_cast<const int*>(base + o_ci), (int)chunk_sum_of.size(),
-                              S.chunk_val.as<double>(), S.stats.as<double>(),
-                              reinterpret_cast<const int64_t*>(base + o_sb), nsb, d_noise, d_const, d_arena, st);
+    e = enm_launch_prep_write(reinterpret_cast<PrepTaskDev*>(base + o_t), S.stride, n_tasks, d_raw, d_mask,
+                              S.drop_nan ? 1 : 0, rescale ? 1 : 0, S.blk_off.as<int64_t>(),
+                              reinterpret_cast<const PrepSumDev*>(base + o_s), (int)sums.size(),
+                              reinterpret_cast<const int*>(base + o_cs), reinterpret_cast<const int*>(base + o_ci),
+                              (int)chunk_sum_of.size(), S.chunk_val.as<double>(), S.stats.as<double>(), sstride,
+                              d_noise, d_const, d_arena, hints ? S.kept.as<KeptWord>() : nullptr, st);
     if (e != hipSuccess) return hip_fail(ctx, e, "prep write");
     if (!rescale) {
         e = enm_launch_fill_i32(reinterpret_cast<int*>(d_const), n_tasks, 0, st);
         if (e != hipSuccess) return hip_fail(ctx, e, "prep write");
     }
     S.counted = false;
+    S.written = hints;
+    S.o_tasks = o_t;
+    S.arena_off.assign(arena_off, arena_off + n_tasks);
+    S.arena = d_arena;
+    S.noise = rescale ? d_noise : nullptr;
+    S.noise_len = rescale ? noise_len : 0;
+    S.flags = flags;
+    return ENM_OK;
+}
+
+int enm_prep_hints(enm_ctx* ctx, const enm_prep_task* tasks, int32_t n_tasks, const double* d_raw,
+                   const uint8_t* d_mask, const double* d_noise, int64_t noise_len, uint32_t flags,
+                   const int64_t* arena_off, const double* d_arena, int32_t* d_hint, void* stream) {
+    if (!ctx) { g_init_error = "ctx is NULL"; return ENM_E_ARG; }
+    ctx->err.clear();
+    auto& S = ctx->prep;
+    if (!tasks || !arena_off || !d_arena || !d_hint) return fail(ctx, ENM_E_ARG, "missing argument");
+    bool same = S.written && (int)S.tasks.size() == n_tasks && S.raw == d_raw && S.mask == d_mask &&
+                S.arena == d_arena && S.flags == flags &&
+                S.noise == ((flags & ENM_P_RESCALE) ? (const void*)d_noise : nullptr) &&
+                S.noise_len == ((flags & ENM_P_RESCALE) ? noise_len : 0);
+    for (int t = 0; same && t < n_tasks; ++t) {
+        const enm_prep_task& T = tasks[t];
+        const PrepTaskDev& D = S.tasks[t];
+        same = T.x_col == D.x_col && T.y_col == D.y_col && T.x0 == D.x0 && T.y0 == D.y0 && T.w == D.w &&
+               T.dz == D.dz && arena_off[t] == S.arena_off[t];
+        for (int c = 0; same && c < T.dz; ++c) same = T.z_col[c] == D.z_col[c] && T.z0[c] == D.z0[c];
+    }
+    if (!same)
+        return fail(ctx, ENM_E_ARG,
+                    "enm_prep_hints must follow an enm_prep_write with ENM_P_HINTS of the same tasks and inputs");
+    hipError_t e = hipSetDevice(ctx->device);
+    if (e != hipSuccess) return hip_fail(ctx, e, "hipSetDevice");
+    hipStream_t st = (hipStream_t)stream;
+    // The series: every raw column a window reads, over the union of its windows.
+    std::map<int64_t, std::pair<int64_t, int64_t>> range;
+    auto take = [&](int64_t col, int64_t first, int64_t w) {
+        auto it = range.find(col);
+        if (it == range.end()) range[col] = {first, first + w};
+        else it->second = {std::min(it->second.first, first), std::max(it->second.second, first + w)};
+    };
+    int max_cols = 0;
+    for (int t = 0; t < n_tasks; ++t) {
+        const PrepTaskDev& D = S.tasks[t];
+        take(D.x_col, D.x0, D.w);
+        take(D.y_col, D.y0, D.w);
+        for (int c = 0; c < D.dz; ++c) take(D.z_col[c], D.z0[c], D.w);
+        max_cols = std::max(max_cols, 2 + D.dz);
+    }
+    std::vector<HintSeries> series;
+    std::vector<SortSeg> segs;
+    std::map<int64_t, int> sid;
+    int64_t nkeys = 0, maxlen = 0;
+    int ntiles = 0;
+    for (const auto& kv : range) {
+        const int64_t len = kv.second.second - kv.second.first;
+        if (len > 0x7fffffffLL) return fail(ctx, ENM_E_UNSUPPORTED, "a series range of 2^31 rows or more");
+        sid[kv.first] = (int)series.size();
+        series.push_back(HintSeries{kv.first, kv.second.first, len, nkeys});
+        SortSeg sg;
+        std::memset(&sg, 0, sizeof sg);
+        sg.src = nkeys;
+        sg.sbase = nkeys;
+        sg.len = (int32_t)len;
+        sg.first_tile = ntiles;
+        sg.p0 = 0;
+        sg.hint = 0;
+        segs.push_back(sg);
+        ntiles += (int)((len + ENM_SORT_TILE - 1) / ENM_SORT_TILE);
+        nkeys += len;
+        maxlen = std::max(maxlen, len);
+    }
+    // The columns, by column slot and then by task (consecutive filter
+    // blocks, one XCD's share, read one series).
+    std::vector<HintCol> cols;
+    int64_t max_w = 0, pstride = 0, extent = 0;
+    for (int c = 0; c < max_cols; ++c) {
+        for (int t = 0; t < n_tasks; ++t) {
+            const PrepTaskDev& D = S.tasks[t];
+            if (c >= 2 + D.dz) continue;
+            const int64_t col = c == 0 ? D.x_col : c == 1 ? D.y_col : D.z_col[c - 2];
+            const int64_t first = c == 0 ? D.x0 : c == 1 ? D.y0 : D.z0[c - 2];
+            const HintSeries& H = series[sid[col]];
+            HintCol C;
+            std::memset(&C, 0, sizeof C);
+            C.sb = H.kbase;
+            C.ws = first - H.lo;
+            C.w = D.w;
+            C.pm = D.pmap;
+            C.n = S.n[t];
+            C.out = S.arena_off[t] + c * C.n;
+            C.len = (int32_t)H.len;
+            C.task = t;
+            C.slot = c;
+            max_w = std::max<int64_t>(max_w, D.w);
+            pstride = std::max<int64_t>(pstride, (C.n + ENM_HINT_POS - 1) / ENM_HINT_POS);
+            extent = std::max(extent, C.out + C.n);
+            cols.push_back(C);
+        }
+    }
+    S.blob3.clear();
+    const size_t o_ser = append(S.blob3, series);
+    const size_t o_seg = append(S.blob3, segs);
+    const size_t o_col = append(S.blob3, cols);
+    const size_t ncols = cols.size();
+    if (S.tables3.ensure(S.blob3.size()) != hipSuccess || S.hkeys.ensure((size_t)nkeys * 8 + 8) != hipSuccess ||
+        S.hk0.ensure((size_t)nkeys * 8 + 8) != hipSuccess || S.hk1.ensure((size_t)nkeys * 8 + 8) != hipSuccess ||
+        S.hi0.ensure((size_t)nkeys * 4 + 4) != hipSuccess || S.hi1.ensure((size_t)nkeys * 4 + 4) != hipSuccess ||
+        S.hsplit.ensure((size_t)ntiles * 4 + 4) != hipSuccess || S.hpre.ensure((size_t)extent * 8 + 8) != hipSuccess ||
+        S.hbad.ensure(ncols * 4 + 4) != hipSuccess || S.hmax.ensure(8) != hipSuccess) {
+        (void)hipGetLastError();
+        return fail(ctx, ENM_E_NOMEM, "hipMalloc failed for the sort-hint workspace");
+    }
+    e = hipMemcpyAsync(S.tables3.p, S.blob3.data(), S.blob3.size(), hipMemcpyHostToDevice, st);
+    if (e != hipSuccess) return hip_fail(ctx, e, "hipMemcpyAsync(hint tables)");
+    const char* base = S.tables3.as<char>();
+    const HintSeries* dser = reinterpret_cast<const HintSeries*>(base + o_ser);
+    const SortSeg* dseg = reinterpret_cast<const SortSeg*>(base + o_seg);
+    const HintCol* dcol = reinterpret_cast<const HintCol*>(base + o_col);
+    e = enm_launch_hint_keys(dser, (int)series.size(), maxlen, d_raw, S.hkeys.as<double>(), st);
+    if (e != hipSuccess) return hip_fail(ctx, e, "hint keys");
+    int fb = 0, nl = 0;
+    e = enm_launch_sort(dseg, (int)segs.size(), ntiles, maxlen, ENM_SORT_TILE, S.hkeys.as<double>(), nullptr,
+                        S.hk0.as<double>(), S.hi0.as<int>(), S.hk1.as<double>(), S.hi1.as<int>(),
+                        S.hsplit.as<int>(), &fb, &nl, st);
+    if (e != hipSuccess) return hip_fail(ctx, e, "hint series sort");
+    const int* sidx = fb ? S.hi1.as<int>() : S.hi0.as<int>();
+    const double* skey = fb ? S.hk1.as<double>() : S.hk0.as<double>();
+    const PrepTaskDev* dtasks = reinterpret_cast<const PrepTaskDev*>(S.tables2.as<char>() + S.o_tasks);
+    e = enm_launch_hint_filter(dcol, (int)ncols, max_w, sidx, skey, S.kept.as<KeptWord>(), dtasks,
+                               S.stats.as<double>(), (flags & ENM_P_RESCALE) ? 1 : 0, d_hint, S.hpre.as<double>(),
+                               S.hbad.as<int>(), st);
+    if (e != hipSuccess) return hip_fail(ctx, e, "hint filter");
+    e = enm_launch_hint_runs(dcol, (int)ncols, pstride, (flags & ENM_P_RESCALE) ? d_noise : nullptr, noise_len,
+                             S.hmax.as<double>(), d_arena, S.hpre.as<double>(), d_hint, S.hbad.as<int>(), st);
+    if (e != hipSuccess) return hip_fail(ctx, e, "hint runs");
     return ENM_OK;
 }
 
@@ -1640,7 +1805,7 @@ int enm_unique_rows(enm_ctx* ctx, const enm_rowset* sets, int32_t n_sets, const 
             sg.len = (int32_t)R.n;
             sg.first_tile = tiles1;
             sg.p0 = 0;
-            sg.pad = 0;
+            sg.hint = 0;
             tiles1 += (int)((R.n + ENM_SORT_TILE - 1) / ENM_SORT_TILE);
             UqSeg u;
             std::memset(&u, 0, sizeof u);
@@ -1720,7 +1885,7 @@ int enm_unique_rows(enm_ctx* ctx, const enm_rowset* sets, int32_t n_sets, const 
         sg.len = (int32_t)R.n;
         sg.first_tile = tiles2;
         sg.p0 = 0;
-        sg.pad = 0;
+        sg.hint = 0;
         tiles2 += (int)((R.n + ENM_SORT_TILE - 1) / ENM_SORT_TILE);
         UqSeg u;
         std::memset(&u, 0, sizeof u);

@@ -54,7 +54,7 @@ struct SortSeg {
     int32_t len;
     int32_t first_tile;  // index of its first ENM_SORT_TILE chunk
     int32_t p0;          // instance point index of the first key
-    int32_t pad;
+    int32_t hint;        // 1: the caller's hint array may hold this segment's order at src
 };
 
 // Sample sort of the first-stage segments (segments of up to ENM_SS_MAXLEN
@@ -228,7 +228,40 @@ struct PrepTaskDev {
     int64_t stat;               // its first stats slot: mean, std of x, y, then each z column
     int64_t noise_x, noise_y, noise_z;   // noise offsets (set on the device)
     uint32_t flags, pad;        // bit 0/1/2: x/y/z constant (not rescaled)
+    int64_t pmap;               // first word of its kept-row map (sort hints): ceil(w / 32) words
 };
+
+// One word of a task's kept-row map: bit b is window row 32 * i + b (1:
+// kept), prefix the kept rows before row 32 * i.  A kept row r is at task
+// position prefix + popcount(bits below r).
+struct KeptWord {
+    uint32_t bits;
+    int32_t prefix;
+};
+
+// Sort hints (enm_prep_hints).  A series: the rows [lo, lo + len) of a raw
+// column that some task windows, sorted once at key-buffer position kbase.
+struct HintSeries {
+    int64_t col, lo, len, kbase;
+};
+
+// One task column: its series' sorted rows filtered to the task's kept
+// rows give the column's order before the noise.
+struct HintCol {
+    int64_t sb;      // key-buffer position of the series' first sorted row
+    int64_t ws;      // the window's first row, relative to the series range
+    int64_t w;       // window rows
+    int64_t pm;      // the task's kept-row map (first word)
+    int64_t out;     // arena (and hint) offset of the column
+    int64_t n;       // kept rows
+    int32_t len;     // rows of the series range
+    int32_t task;    // the task (its statistics and constant flags)
+    int32_t slot;    // the column: 0 x, 1 y, 2 + c condition column c
+    int32_t pad;
+};
+#define ENM_HINT_BLOCK 2048   // sorted series rows per filter block (256 threads x 8)
+#define ENM_HINT_POS 1024     // column positions per run block (256 threads x 4)
+#define ENM_HINT_RUN 32       // longest run of near-equal values re-sorted (longer: no hint)
 
 // One array whose mean and standard deviation NumPy's way are needed.
 struct PrepSumDev {

@@ -39,6 +39,8 @@ struct SampleSortArgs {
     int* perm;
     unsigned long long* maxabs;
     int with32;
+    const int* hint;            // caller's sort hints, parallel to the arena (NULL: none)
+    int* hint_bad;              // per segment: 1 when its hint was absent or wrong (zeroed before)
 };
 hipError_t enm_launch_sample_sort(const SampleSortArgs& a, const double* arena, double* k0, int* i0,
                                   double* k1, int* i1, int* n_launch, hipStream_t st);
@@ -82,15 +84,28 @@ hipError_t enm_launch_reduce_pt(const RedItem* items, int n_items, const RedDev*
 hipError_t enm_launch_export_eps(const double* src, const int* perm, int64_t n, double* dst, hipStream_t st);
 hipError_t enm_launch_export_cnt(const int* src, const int* perm, int64_t n, int* dst, hipStream_t st);
 
-hipError_t enm_launch_prep_count(const PrepTaskDev* tasks, const int64_t* blk0, int n_tasks, int64_t n_blocks,
+hipError_t enm_launch_prep_count(const PrepTaskDev* tasks, int64_t stride, int n_tasks,
                                  const double* raw, const unsigned char* mask, int drop_nan, int* blk_count,
                                  int64_t* blk_off, unsigned* nan_flag, int64_t* n_out, hipStream_t st);
-hipError_t enm_launch_prep_write(PrepTaskDev* tasks, const int64_t* blk0, int n_tasks, int64_t n_blocks,
+hipError_t enm_launch_prep_write(PrepTaskDev* tasks, int64_t stride, int n_tasks,
                                  const double* raw, const unsigned char* mask, int drop_nan, int rescale,
                                  const int64_t* blk_off, const PrepSumDev* sums, int n_sums,
                                  const int* chunk_sum_of, const int* chunk_index, int n_chunks,
-                                 double* chunk_val, double* stats, const int64_t* sblk0, int64_t n_sblocks,
-                                 const double* noise, unsigned* const_flag, double* arena, hipStream_t st);
+                                 double* chunk_val, double* stats, int64_t sstride,
+                                 const double* noise, unsigned* const_flag, double* arena, KeptWord* kept,
+                                 hipStream_t st);
+// Sort hints (enm_hint.hip).
+hipError_t enm_launch_hint_keys(const HintSeries* series, int n_series, int64_t maxlen, const double* raw,
+                                double* keys, hipStream_t st);
+// One filter block per column; column ci owns run blocks
+// [ci * pstride, (ci + 1) * pstride).
+hipError_t enm_launch_hint_filter(const HintCol* cols, int n_cols, int64_t max_w, const int* sidx,
+                                  const double* skey, const KeptWord* kept, const PrepTaskDev* tasks,
+                                  const double* stats, int rescale, int* hint, double* pre, int* bad,
+                                  hipStream_t st);
+hipError_t enm_launch_hint_runs(const HintCol* cols, int n_cols, int64_t pstride, const double* noise,
+                                int64_t noise_len, double* noise_max, const double* arena, const double* pre,
+                                int* hint, int* bad, hipStream_t st);
 
 hipError_t enm_launch_uq_codes(const UqSeg* segs, int nsegs, const double* skeys, const int* sidx, int* codes,
                                int64_t* starts, int64_t* counts, int64_t* distinct, hipStream_t st);

@@ -31,50 +31,51 @@ constexpr int PB_ROUNDS = PB_ROWS / PB_THREADS;  // rows per thread
 constexpr int PW_CHUNK = 8192;                   // NumPy's reduction buffer
 constexpr int PW_LEAF = 128;                     // NumPy's pairwise block
 
-__device__ __forceinline__ bool row_valid(const PrepTaskDev& t, const double* __restrict__ raw,
+// Whether window row r of task t is kept (the mask first, then, with
+// drop_nan, no NaN in x, y or the condition), and whether a row that passes
+// the mask holds a NaN.  x and y are returned for the caller to store; rows
+// at or past the window's end are neither kept nor NaN (row 0 is read
+// instead, so every load stays inside the series).
+__device__ __forceinline__ bool row_state(const PrepTaskDev& t, const double* __restrict__ raw,
                                           const unsigned char* __restrict__ mask, int drop_nan, int64_t r,
-                                          bool& has_nan) {
-    if (mask && !mask[t.y0 + r]) {
-        has_nan = false;
-        return false;
-    }
-    bool nan = isnan(raw[t.x_col + t.x0 + r]) || isnan(raw[t.y_col + t.y0 + r]);
-    for (int c = 0; c < t.dz; ++c) nan = nan || isnan(raw[t.z_col[c] + t.z0[c] + r]);
-    has_nan = nan;
-    return !(drop_nan && nan);
+                                          bool& has_nan, double& xv, double& yv) {
+    const bool in = r < t.w;
+    const int64_t rr = in ? r : 0;
+    xv = raw[t.x_col + t.x0 + rr];
+    yv = raw[t.y_col + t.y0 + rr];
+    const bool pass = in && (mask == nullptr || mask[t.y0 + rr] != 0);
+    bool nan = isnan(xv) || isnan(yv);
+    for (int c = 0; c < t.dz; ++c) nan = nan || isnan(raw[t.z_col[c] + t.z0[c] + rr]);
+    has_nan = pass && nan;
+    return pass && !(drop_nan && nan);
 }
 
-__device__ __forceinline__ int block_task(const int64_t* __restrict__ blk0, int n_tasks, int64_t b) {
-    int lo = 0, hi = n_tasks - 1;
-    while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        if (blk0[mid] <= b) lo = mid; else hi = mid - 1;
-    }
-    return lo;
-}
+// Blocks are laid out `stride` per task (the most any task needs; the
+// rest of a shorter task's blocks find no rows), so a block's task is a
+// division, not a search.
 
 // Phase 1: kept rows per block, and a NaN among the masked rows per task.
+// Every row's loads are issued before any of them is used.
 __global__ __launch_bounds__(PB_THREADS)
-void prep_count_kernel(const PrepTaskDev* __restrict__ tasks, const int64_t* __restrict__ blk0, int n_tasks,
-                       const double* __restrict__ raw, const unsigned char* __restrict__ mask, int drop_nan,
-                       int* __restrict__ blk_count, unsigned* __restrict__ nan_flag) {
+void prep_count_kernel(const PrepTaskDev* __restrict__ tasks, int64_t stride, const double* __restrict__ raw,
+                       const unsigned char* __restrict__ mask, int drop_nan, int* __restrict__ blk_count,
+                       unsigned* __restrict__ nan_flag) {
     __shared__ int s_cnt[PB_THREADS / ENM_WAVE];
     __shared__ int s_nan;
-    const int64_t b = blockIdx.x;
-    const int ti = block_task(blk0, n_tasks, b);
-    const PrepTaskDev t = tasks[ti];
-    const int64_t r0 = (b - blk0[ti]) * PB_ROWS;
+    const int64_t b = xcd_block(blockIdx.x, gridDim.x);   // a task's blocks on one XCD
+    const int ti = (int)(b / stride);
+    const PrepTaskDev& t = tasks[ti];   // read in place: a copy indexed by c would sit in scratch
+    const int64_t r0 = (b - ti * stride) * PB_ROWS;
     if (threadIdx.x == 0) s_nan = 0;
     __syncthreads();
     int cnt = 0;
     bool any_nan = false;
+#pragma unroll
     for (int j = 0; j < PB_ROUNDS; ++j) {
-        const int64_t r = r0 + j * PB_THREADS + threadIdx.x;
-        if (r < t.w) {
-            bool nan;
-            cnt += row_valid(t, raw, mask, drop_nan, r, nan) ? 1 : 0;
-            any_nan = any_nan || nan;
-        }
+        bool nan;
+        double xv, yv;
+        cnt += row_state(t, raw, mask, drop_nan, r0 + j * PB_THREADS + threadIdx.x, nan, xv, yv) ? 1 : 0;
+        any_nan = any_nan || nan;
     }
     const int lane = threadIdx.x & (ENM_WAVE - 1), wv = threadIdx.x / ENM_WAVE;
     for (int off = 32; off > 0; off >>= 1) cnt += __shfl_xor(cnt, off, ENM_WAVE);
@@ -92,11 +93,11 @@ void prep_count_kernel(const PrepTaskDev* __restrict__ tasks, const int64_t* __r
 // Per task: block offsets (exclusive scan of the block counts, in place in
 // blk_off) and the kept rows n.  One wave per task.
 __global__ __launch_bounds__(ENM_WAVE)
-void prep_scan_kernel(const PrepTaskDev* __restrict__ tasks, const int64_t* __restrict__ blk0, int n_tasks,
+void prep_scan_kernel(const PrepTaskDev* __restrict__ tasks, int64_t stride, int n_tasks,
                       const int* __restrict__ blk_count, int64_t* __restrict__ blk_off, int64_t* __restrict__ n_out) {
     const int ti = blockIdx.x;
     if (ti >= n_tasks) return;
-    const int64_t b0 = blk0[ti], nb = tasks[ti].nblk;
+    const int64_t b0 = ti * stride, nb = tasks[ti].nblk;
     const int lane = threadIdx.x;
     int64_t carry = 0;
     for (int64_t base = 0; base < nb; base += ENM_WAVE) {
@@ -113,38 +114,66 @@ void prep_scan_kernel(const PrepTaskDev* __restrict__ tasks, const int64_t* __re
 }
 
 // Phase 2a: the kept rows of every block, in row order, to the task's
-// arena columns (x, y, then the condition columns, n values each).
+// arena columns (x, y, then the condition columns, n values each); with a
+// kept-row map, each 32 window rows' kept bits and the kept rows before
+// them, for the sort hints.  Every
+// row's state and x, y values are read first; one barrier then gives each
+// wave the kept rows of the waves before it, round by round.
 __global__ __launch_bounds__(PB_THREADS)
-void prep_compact_kernel(const PrepTaskDev* __restrict__ tasks, const int64_t* __restrict__ blk0, int n_tasks,
+void prep_compact_kernel(const PrepTaskDev* __restrict__ tasks, int64_t stride,
                          const double* __restrict__ raw, const unsigned char* __restrict__ mask, int drop_nan,
-                         const int64_t* __restrict__ blk_off, double* __restrict__ arena) {
-    __shared__ int s_wcnt[PB_THREADS / ENM_WAVE];
-    const int64_t b = blockIdx.x;
-    const int ti = block_task(blk0, n_tasks, b);
-    const PrepTaskDev t = tasks[ti];
-    const int64_t r0 = (b - blk0[ti]) * PB_ROWS;
+                         const int64_t* __restrict__ blk_off, double* __restrict__ arena,
+                         KeptWord* __restrict__ kept) {
+    constexpr int NW = PB_THREADS / ENM_WAVE;
+    __shared__ int s_wcnt[PB_ROUNDS][NW];
+    const int64_t b = xcd_block(blockIdx.x, gridDim.x);   // a task's blocks on one XCD
+    const int ti = (int)(b / stride);
+    const PrepTaskDev& t = tasks[ti];   // read in place: a copy indexed by c would sit in scratch
+    const int64_t r0 = (b - ti * stride) * PB_ROWS;
     const int lane = threadIdx.x & (ENM_WAVE - 1), wv = threadIdx.x / ENM_WAVE;
-    int64_t pos = blk_off[b];
+    bool keep[PB_ROUNDS];
+    double xv[PB_ROUNDS], yv[PB_ROUNDS];
+    int below[PB_ROUNDS];
+#pragma unroll
     for (int j = 0; j < PB_ROUNDS; ++j) {
-        const int64_t r = r0 + j * PB_THREADS + threadIdx.x;
         bool nan;
-        const bool keep = r < t.w && row_valid(t, raw, mask, drop_nan, r, nan);
-        const unsigned long long m = __builtin_amdgcn_ballot_w64(keep);
-        const int below = __popcll(m & ((1ull << lane) - 1ull));
-        if (lane == 0) s_wcnt[wv] = __popcll(m);
-        __syncthreads();
+        keep[j] = row_state(t, raw, mask, drop_nan, r0 + j * PB_THREADS + threadIdx.x, nan, xv[j], yv[j]);
+    }
+    unsigned long long wm[PB_ROUNDS];
+#pragma unroll
+    for (int j = 0; j < PB_ROUNDS; ++j) {
+        const unsigned long long m = __builtin_amdgcn_ballot_w64(keep[j]);
+        wm[j] = m;
+        below[j] = __popcll(m & ((1ull << lane) - 1ull));
+        if (lane == 0) s_wcnt[j][wv] = __popcll(m);
+    }
+    __syncthreads();
+    int64_t pos = blk_off[b];
+    double* out = arena + t.arena_off;
+#pragma unroll
+    for (int j = 0; j < PB_ROUNDS; ++j) {
         int before = 0, total = 0;
-        for (int w = 0; w < PB_THREADS / ENM_WAVE; ++w) {
-            before += w < wv ? s_wcnt[w] : 0;
-            total += s_wcnt[w];
+#pragma unroll
+        for (int w = 0; w < NW; ++w) {
+            before += w < wv ? s_wcnt[j][w] : 0;
+            total += s_wcnt[j][w];
         }
-        __syncthreads();
-        if (keep) {
-            const int64_t p = pos + before + below;
-            double* out = arena + t.arena_off;
-            out[p] = raw[t.x_col + t.x0 + r];
-            out[t.n + p] = raw[t.y_col + t.y0 + r];
+        const int64_t r = r0 + j * PB_THREADS + threadIdx.x;
+        const int64_t p = pos + before + below[j];
+        if (keep[j]) {
+            out[p] = xv[j];
+            out[t.n + p] = yv[j];
             for (int c = 0; c < t.dz; ++c) out[(2 + c) * t.n + p] = raw[t.z_col[c] + t.z0[c] + r];
+        }
+        if (kept != nullptr && lane < 2) {
+            // The wave's 64 rows are two words of the map.
+            const int64_t row = r0 + j * PB_THREADS + wv * ENM_WAVE + 32 * lane;
+            if (row < t.w) {
+                const uint32_t lo = (uint32_t)wm[j];
+                const uint32_t bits = lane ? (uint32_t)(wm[j] >> 32) : lo;
+                const int64_t pre = pos + before + (lane ? __popc(lo) : 0);
+                kept[t.pmap + row / 32] = KeptWord{bits, (int32_t)pre};
+            }
         }
         pos += total;
     }
@@ -244,9 +273,42 @@ __device__ double pw_combine(int n, const double* leaf) {
     }
 }
 
+// The sum of a full PW_CHUNK-value chunk in NumPy's order, by one wave: the
+// recursion's 64 leaves of 128 values form a balanced tree.  Lane l takes
+// accumulator l % 8 of leaf 8 * pass + l / 8 (sixteen values, added in
+// order), so one load instruction reads eight 64-byte runs.  Butterfly
+// shuffles then combine the accumulators as the leaf does,
+// ((r0 + r1) + (r2 + r3)) + ((r4 + r5) + (r6 + r7)), and the pass's eight
+// leaves pairwise; IEEE addition commutes, so both lanes of a pair hold the
+// same bits.  The eight passes' subtree sums end the tree.
+__device__ __forceinline__ double pw_full_chunk(const Term& f) {
+    const int lane = threadIdx.x & (ENM_WAVE - 1);
+    const int j = lane & 7, g = lane >> 3;
+    double part[8];
+#pragma unroll
+    for (int pass = 0; pass < 8; ++pass) {
+        const int64_t base = (int64_t)(pass * 8 + g) * PW_LEAF + j;
+        double v[PW_LEAF / 8];
+#pragma unroll
+        for (int i = 0; i < PW_LEAF / 8; ++i) v[i] = f(base + 8 * i);
+        double acc = v[0];
+#pragma unroll
+        for (int i = 1; i < PW_LEAF / 8; ++i) acc += v[i];
+        acc += __shfl_xor(acc, 1, ENM_WAVE);
+        acc += __shfl_xor(acc, 2, ENM_WAVE);
+        acc += __shfl_xor(acc, 4, ENM_WAVE);
+        acc += __shfl_xor(acc, 8, ENM_WAVE);
+        acc += __shfl_xor(acc, 16, ENM_WAVE);
+        acc += __shfl_xor(acc, 32, ENM_WAVE);
+        part[pass] = acc;
+    }
+    return ((part[0] + part[1]) + (part[2] + part[3])) + ((part[4] + part[5]) + (part[6] + part[7]));
+}
+
 // Phase 2b: one wave per (array, chunk): the pairwise sum of a chunk of
 // values (pass 0) or of squared deviations from the array's mean (pass 1).
-// Lanes sum the leaves, lane 0 combines them in NumPy's order.
+// A full chunk takes pw_full_chunk; otherwise lanes sum the leaves and
+// lane 0 combines them in NumPy's order.
 __global__ __launch_bounds__(ENM_WAVE)
 void prep_chunk_kernel(const PrepSumDev* __restrict__ sums, const int* __restrict__ chunk_sum_of,
                        const int* __restrict__ chunk_index, int n_chunks, const double* __restrict__ arena,
@@ -259,9 +321,14 @@ void prep_chunk_kernel(const PrepSumDev* __restrict__ sums, const int* __restric
     const PrepSumDev sd = sums[chunk_sum_of[c]];
     const int64_t first = (int64_t)chunk_index[c] * PW_CHUNK;
     const int m = (int)min((int64_t)PW_CHUNK, sd.n - first);
+    const Term f{arena + sd.src + first, pass ? stats[sd.stat] : 0.0, pass != 0};
+    if (m == PW_CHUNK) {
+        const double sum = pw_full_chunk(f);
+        if (threadIdx.x == 0) chunk_val[c] = sum;
+        return;
+    }
     if (threadIdx.x == 0) s_nl = pw_leaves(m, s_lo, s_len);
     __syncthreads();
-    const Term f{arena + sd.src + first, pass ? stats[sd.stat] : 0.0, pass != 0};
     for (int l = threadIdx.x; l < s_nl; l += ENM_WAVE) s_leaf[l] = pw_leaf(f, s_lo[l], s_len[l]);
     __syncthreads();
     if (threadIdx.x == 0) chunk_val[c] = pw_combine(m, s_leaf);
@@ -322,12 +389,12 @@ __global__ void prep_noise_kernel(PrepTaskDev* __restrict__ tasks, int n_tasks, 
 
 // Phase 2e: (v - mean) / std + noise for every rescaled value, in place.
 __global__ __launch_bounds__(PB_THREADS)
-void prep_scale_kernel(const PrepTaskDev* __restrict__ tasks, const int64_t* __restrict__ sblk0, int n_tasks,
-                       const double* __restrict__ stats, const double* __restrict__ noise, double* __restrict__ arena) {
-    const int64_t b = blockIdx.x;
-    const int ti = block_task(sblk0, n_tasks, b);
-    const PrepTaskDev t = tasks[ti];
-    const int64_t i = (b - sblk0[ti]) * PB_THREADS + threadIdx.x;
+void prep_scale_kernel(const PrepTaskDev* __restrict__ tasks, int64_t stride, const double* __restrict__ stats,
+                       const double* __restrict__ noise, double* __restrict__ arena) {
+    const int64_t b = xcd_block(blockIdx.x, gridDim.x);   // a task's blocks on one XCD
+    const int ti = (int)(b / stride);
+    const PrepTaskDev& t = tasks[ti];   // read in place: a copy indexed by c would sit in scratch
+    const int64_t i = (b - ti * stride) * PB_THREADS + threadIdx.x;
     if (i >= t.n) return;
     double* col = arena + t.arena_off;
     if (!(t.flags & 1u)) {
@@ -349,30 +416,33 @@ void prep_scale_kernel(const PrepTaskDev* __restrict__ tasks, const int64_t* __r
 
 }  // namespace
 
-hipError_t enm_launch_prep_count(const PrepTaskDev* tasks, const int64_t* blk0, int n_tasks, int64_t n_blocks,
+hipError_t enm_launch_prep_count(const PrepTaskDev* tasks, int64_t stride, int n_tasks,
                                  const double* raw, const unsigned char* mask, int drop_nan, int* blk_count,
                                  int64_t* blk_off, unsigned* nan_flag, int64_t* n_out, hipStream_t st) {
+    const int64_t n_blocks = stride * n_tasks;
     if (n_tasks <= 0) return hipSuccess;
     hipError_t e = enm_launch_fill_i32(reinterpret_cast<int*>(nan_flag), n_tasks, 0, st);
     if (e != hipSuccess) return e;
     if (n_blocks > 0)
-        hipLaunchKernelGGL(prep_count_kernel, dim3((unsigned)n_blocks), dim3(PB_THREADS), 0, st, tasks, blk0, n_tasks,
+        hipLaunchKernelGGL(prep_count_kernel, dim3((unsigned)n_blocks), dim3(PB_THREADS), 0, st, tasks, stride,
                            raw, mask, drop_nan, blk_count, nan_flag);
-    hipLaunchKernelGGL(prep_scan_kernel, dim3((unsigned)n_tasks), dim3(ENM_WAVE), 0, st, tasks, blk0, n_tasks,
+    hipLaunchKernelGGL(prep_scan_kernel, dim3((unsigned)n_tasks), dim3(ENM_WAVE), 0, st, tasks, stride, n_tasks,
                        blk_count, blk_off, n_out);
     return hipGetLastError();
 }
 
-hipError_t enm_launch_prep_write(PrepTaskDev* tasks, const int64_t* blk0, int n_tasks, int64_t n_blocks,
+hipError_t enm_launch_prep_write(PrepTaskDev* tasks, int64_t stride, int n_tasks,
                                  const double* raw, const unsigned char* mask, int drop_nan, int rescale,
                                  const int64_t* blk_off, const PrepSumDev* sums, int n_sums,
                                  const int* chunk_sum_of, const int* chunk_index, int n_chunks,
-                                 double* chunk_val, double* stats, const int64_t* sblk0, int64_t n_sblocks,
-                                 const double* noise, unsigned* const_flag, double* arena, hipStream_t st) {
+                                 double* chunk_val, double* stats, int64_t sstride,
+                                 const double* noise, unsigned* const_flag, double* arena, KeptWord* kept,
+                                 hipStream_t st) {
     if (n_tasks <= 0) return hipSuccess;
+    const int64_t n_blocks = stride * n_tasks, n_sblocks = sstride * n_tasks;
     if (n_blocks > 0)
-        hipLaunchKernelGGL(prep_compact_kernel, dim3((unsigned)n_blocks), dim3(PB_THREADS), 0, st, tasks, blk0,
-                           n_tasks, raw, mask, drop_nan, blk_off, arena);
+        hipLaunchKernelGGL(prep_compact_kernel, dim3((unsigned)n_blocks), dim3(PB_THREADS), 0, st, tasks, stride,
+                           raw, mask, drop_nan, blk_off, arena, kept);
     if (!rescale) return hipGetLastError();
     for (int pass = 0; pass < 2; ++pass) {
         if (n_chunks > 0)
@@ -384,7 +454,7 @@ hipError_t enm_launch_prep_write(PrepTaskDev* tasks, const int64_t* blk0, int n_
     hipLaunchKernelGGL(prep_noise_kernel, dim3((unsigned)((n_tasks + 63) / 64)), dim3(64), 0, st, tasks, n_tasks,
                        stats, const_flag);
     if (n_sblocks > 0)
-        hipLaunchKernelGGL(prep_scale_kernel, dim3((unsigned)n_sblocks), dim3(PB_THREADS), 0, st, tasks, sblk0,
-                           n_tasks, stats, noise, arena);
+        hipLaunchKernelGGL(prep_scale_kernel, dim3((unsigned)n_sblocks), dim3(PB_THREADS), 0, st, tasks, sstride,
+                           stats, noise, arena);
     return hipGetLastError();
 }

@@ -24,6 +24,15 @@
 // sorts each run in LDS.  56 B of traffic per key instead of 24 B per merge
 // pass.  A run of one bucket larger than ENM_SS_TILE (sampling bad luck) is
 // merge-sorted by one block through global memory, so every input sorts.
+//
+// Sort hints: a caller may pass, parallel to the arena, the ascending order
+// of a key column (hint[src + i] = point index of the i-th smallest key,
+// ties by index; hint[src] < 0 for none).  ss_hint_kernel checks a hinted
+// segment's order element by element against its neighbour -- every index
+// inside the segment and every (key, index) pair strictly above the one
+// before, which makes it the sorted permutation -- and lays it out into the
+// orders directly; the sample sort then passes over the segments whose hint
+// held and sorts the others (overwriting whatever the check laid out).
 #include "enm_device.h"
 #include "enm_launch.h"
 
@@ -512,13 +521,15 @@ void ss_sample_kernel(const SortSeg* __restrict__ segs, const SampleSeg* __restr
 __global__ __launch_bounds__(ENM_SORT_THREADS)
 void ss_count_kernel(const SortSeg* __restrict__ segs, int nsegs, const SampleSeg* __restrict__ ss,
                      const double* __restrict__ arena, const double* __restrict__ samp_k,
-                     const int* __restrict__ samp_i, unsigned short* __restrict__ bid, int* __restrict__ counts) {
+                     const int* __restrict__ samp_i, unsigned short* __restrict__ bid, int* __restrict__ counts,
+                     const int* __restrict__ hint_bad) {
     __shared__ double spk[ENM_SS_MAXB];
     __shared__ int spi[ENM_SS_MAXB];
     __shared__ int hist[ENM_SS_MAXB];
     const int tile = blockIdx.x;
     const int s = seg_of_tile(segs, nsegs, tile);
     const SortSeg sg = segs[s];
+    if (sg.hint && hint_bad != nullptr && hint_bad[s] == 0) return;   // laid out from its hint
     const SampleSeg q = ss[s];
     const int nb = q.nb, nsp = nb - 1;
     for (int j = threadIdx.x; j < nsp; j += ENM_SORT_THREADS) {
@@ -556,11 +567,12 @@ void ss_count_kernel(const SortSeg* __restrict__ segs, int nsegs, const SampleSe
 __global__ __launch_bounds__(ENM_SORT_THREADS)
 void ss_scan_kernel(const SortSeg* __restrict__ segs, const SampleSeg* __restrict__ ss,
                     int* __restrict__ counts, SsTile* __restrict__ tiles, const int* __restrict__ qbase,
-                    unsigned* __restrict__ qctl) {
+                    unsigned* __restrict__ qctl, const int* __restrict__ hint_bad) {
     __shared__ int tot[ENM_SS_MAXB];
     __shared__ int start[ENM_SS_MAXB];
     __shared__ int part[ENM_SORT_THREADS / ENM_WAVE];
     const SortSeg sg = segs[blockIdx.x];
+    if (sg.hint && hint_bad != nullptr && hint_bad[blockIdx.x] == 0) return;   // no runs to sort
     const SampleSeg q = ss[blockIdx.x];
     const int nb = q.nb;
     const int nch = (sg.len + ENM_SORT_TILE - 1) / ENM_SORT_TILE;
@@ -612,7 +624,8 @@ void ss_scan_kernel(const SortSeg* __restrict__ segs, const SampleSeg* __restric
 __global__ __launch_bounds__(ENM_SORT_THREADS)
 void ss_scatter_kernel(const SortSeg* __restrict__ segs, int nsegs, const SampleSeg* __restrict__ ss,
                        const double* __restrict__ arena, const unsigned short* __restrict__ bid,
-                       const int* __restrict__ offs, double* __restrict__ kout, int* __restrict__ iout) {
+                       const int* __restrict__ offs, double* __restrict__ kout, int* __restrict__ iout,
+                       const int* __restrict__ hint_bad) {
     constexpr int PER = ENM_SORT_TILE / ENM_SORT_THREADS;
     __shared__ double sk[ENM_SORT_TILE];
     __shared__ int si[ENM_SORT_TILE];
@@ -623,6 +636,7 @@ void ss_scatter_kernel(const SortSeg* __restrict__ segs, int nsegs, const Sample
     const int tile = blockIdx.x;
     const int s = seg_of_tile(segs, nsegs, tile);
     const SortSeg sg = segs[s];
+    if (sg.hint && hint_bad != nullptr && hint_bad[s] == 0) return;   // laid out from its hint
     const SampleSeg q = ss[s];
     const int nb = q.nb;
     const int off = (tile - sg.first_tile) * ENM_SORT_TILE;
@@ -798,6 +812,85 @@ __device__ void ss_layout_run(const SsLayout& L, const SsTile& tl, const double*
     }
 }
 
+// One block per ENM_SORT_TILE chunk of a hinted first-stage segment: check
+// the hint's order over the chunk (each element against the one before it,
+// so the chunks together cover the whole segment) and lay the chunk out
+// into every order of the segment, as ss_tile_kernel would.  The chunk's
+// indices and keys (and the element before it) are gathered once into LDS.
+// A failed check marks the segment for the sample sort.
+__global__ __launch_bounds__(ENM_SORT_THREADS)
+void ss_hint_kernel(const SortSeg* __restrict__ segs, int nsegs, const int* __restrict__ hint,
+                    int* __restrict__ hint_bad, SsLayout L) {
+    __shared__ double s_key[ENM_SORT_TILE + 1];
+    __shared__ int s_idx[ENM_SORT_TILE + 1];
+    __shared__ int s_bad;
+    __shared__ double red[ENM_SORT_THREADS / ENM_WAVE];
+    // A segment's chunks on one XCD: its columns' gathers hit one L2.
+    const int tile = (int)xcd_block(blockIdx.x, gridDim.x);
+    const int s = seg_of_tile(segs, nsegs, tile);
+    const SortSeg sg = segs[s];
+    if (!sg.hint) return;
+    const SampleSeg q = L.ss[s];
+    const int off = (tile - sg.first_tile) * ENM_SORT_TILE;
+    const int len = min(ENM_SORT_TILE, sg.len - off);
+    const int* h = hint + sg.src;
+    if (threadIdx.x == 0) s_bad = h[0] < 0 ? 1 : 0;
+    // Slot t holds the element at position off - 1 + t (-1: none or out of range).
+    for (int t = threadIdx.x; t <= len; t += ENM_SORT_THREADS) {
+        const int p = off - 1 + t;
+        const int i = p >= 0 ? h[p] : -1;
+        const bool in = i >= 0 && i < sg.len;
+        s_idx[t] = in ? i : -1;
+        s_key[t] = in ? L.arena[sg.src + i] : 0.0;
+    }
+    __syncthreads();
+    bool bad = false;
+    for (int t = threadIdx.x; t < len; t += ENM_SORT_THREADS) {
+        const int i = s_idx[t + 1], j = s_idx[t];
+        const double v = s_key[t + 1], u = s_key[t];
+        // NaN keys fail both comparisons.
+        const bool ok = i >= 0 && (off + t == 0 || (j >= 0 && (u < v || (u == v && j < i))));
+        bad = bad || !ok;
+    }
+    if (wave_any(bad) && (threadIdx.x & (ENM_WAVE - 1)) == 0) s_bad = 1;
+    __syncthreads();
+    if (s_bad) {
+        if (threadIdx.x == 0) hint_bad[s] = 1;
+        return;
+    }
+    for (int c = 0; c < q.ncons; ++c) {
+        const SsCons cn = L.cons[q.cons0 + c];
+        const OrderDev& od = L.orders[cn.order];
+        const int64_t n = od.n;
+        const int ncol = od.ncol;
+        double m = 0.0;
+        for (int t = threadIdx.x; t < len; t += ENM_SORT_THREADS) {
+            const int64_t p = cn.pos0 + off + t;
+            const int i = s_idx[t + 1];
+            const double key = s_key[t + 1];
+            L.perm[od.perm + p] = sg.p0 + i;
+            L.col64[od.c64 + p] = key;
+            if (L.with32) L.col32[od.c32 + p] = (float)key;
+            m = fmax(m, fabs(key));
+            for (int cc = 1; cc < ncol; ++cc) {
+                const double v = L.arena[od.src[cc] + sg.p0 + i];
+                L.col64[od.c64 + cc * n + p] = v;
+                if (L.with32) L.col32[od.c32 + cc * n + p] = (float)v;
+                m = fmax(m, fabs(v));
+            }
+        }
+        m = wave_max(m);
+        if ((threadIdx.x & (ENM_WAVE - 1)) == 0) red[threadIdx.x / ENM_WAVE] = m;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            double mm = red[0];
+            for (int w = 1; w < ENM_SORT_THREADS / ENM_WAVE; ++w) mm = fmax(mm, red[w]);
+            atomicMax(L.maxabs + od.inst, (unsigned long long)__double_as_longlong(mm));
+        }
+        __syncthreads();
+    }
+}
+
 // Persistent blocks sort the runs in LDS, from the scattered buffers (k1,
 // i1), and lay them out into the orders.  Runs sit in ENM_NQ queues by
 // instance; block b starts on queue b mod ENM_NQ (its XCD, so one
@@ -920,12 +1013,6 @@ hipError_t enm_launch_sample_sort(const SampleSortArgs& a, const double* arena, 
     }
     const double* spk = fb ? a.samp_k1 : a.samp_k0;
     const int* spi = fb ? a.samp_i1 : a.samp_i0;
-    hipLaunchKernelGGL(ss_count_kernel, dim3(a.ntiles), dim3(ENM_SORT_THREADS), 0, st,
-                       a.segs, a.nsegs, a.ss, arena, spk, spi, a.bid, a.counts);
-    hipLaunchKernelGGL(ss_scan_kernel, dim3(a.nsegs), dim3(ENM_SORT_THREADS), 0, st,
-                       a.segs, a.ss, a.counts, a.tiles, a.qbase, a.qctl);
-    hipLaunchKernelGGL(ss_scatter_kernel, dim3(a.ntiles), dim3(ENM_SORT_THREADS), 0, st,
-                       a.segs, a.nsegs, a.ss, arena, a.bid, a.counts, k1, i1);
     SsLayout L;
     L.segs = a.segs;
     L.ss = a.ss;
@@ -937,6 +1024,20 @@ hipError_t enm_launch_sample_sort(const SampleSortArgs& a, const double* arena, 
     L.perm = a.perm;
     L.maxabs = a.maxabs;
     L.with32 = a.with32;
+    const int* hint_bad = nullptr;
+    if (a.hint != nullptr) {
+        if (a.hint_bad == nullptr) return hipErrorInvalidValue;
+        hipLaunchKernelGGL(ss_hint_kernel, dim3(a.ntiles), dim3(ENM_SORT_THREADS), 0, st,
+                           a.segs, a.nsegs, a.hint, a.hint_bad, L);
+        ++*n_launch;
+        hint_bad = a.hint_bad;
+    }
+    hipLaunchKernelGGL(ss_count_kernel, dim3(a.ntiles), dim3(ENM_SORT_THREADS), 0, st,
+                       a.segs, a.nsegs, a.ss, arena, spk, spi, a.bid, a.counts, hint_bad);
+    hipLaunchKernelGGL(ss_scan_kernel, dim3(a.nsegs), dim3(ENM_SORT_THREADS), 0, st,
+                       a.segs, a.ss, a.counts, a.tiles, a.qbase, a.qctl, hint_bad);
+    hipLaunchKernelGGL(ss_scatter_kernel, dim3(a.ntiles), dim3(ENM_SORT_THREADS), 0, st,
+                       a.segs, a.nsegs, a.ss, arena, a.bid, a.counts, k1, i1, hint_bad);
     hipLaunchKernelGGL(ss_tile_kernel, dim3(a.tile_grid), dim3(ENM_SS_THREADS), 0, st,
                        a.tiles, a.qbase, a.qctl, k1, i1, k0, i0, L);
     *n_launch += 4;

@@ -149,6 +149,35 @@ int enm_run_batch(enm_ctx* ctx,
                   void* stream);
 
 /*
+ * enm_run_batch_hinted: enm_run_batch with sort hints, for batches whose
+ * columns come from a few series (lag scans, pairwise scans), whose orders
+ * one sort of each series yields (enm_prep_hints).  Replaces the same
+ * reference code as enm_run_batch; the hints stand in for the kd-tree builds
+ * (ennemi/_entropy_estimators.py:36,100-102,135-139).
+ *
+ * d_hint (device, int32, may be NULL) is parallel to d_arena: for an
+ * instance column at d_arena[col .. col+n) that a sort is keyed on,
+ * d_hint[col .. col+n) may hold the column's ascending order -- the point
+ * indices of its values from smallest to largest, equal values by index --
+ * or d_hint[col] < 0 for none.  Every hinted order is checked in full
+ * (indices in range, each (value, index) pair above the one before, which
+ * makes it the sorted permutation), and columns whose hint is absent or
+ * wrong are sorted as enm_run_batch sorts them: the results are
+ * enm_run_batch's whatever d_hint holds.  Hints are read when the
+ * batch's longest sorted column has 2049 to 2^20 points (the sample-sort
+ * path), for columns outside class segments.
+ */
+int enm_run_batch_hinted(enm_ctx* ctx,
+                         const enm_instance* inst, int32_t n_inst,
+                         const int64_t* seg_len,
+                         const double* d_arena,
+                         const int32_t* d_hint,
+                         double* d_out,
+                         double* d_eps,
+                         int32_t* d_counts,
+                         uint32_t flags,
+                         void* stream);
+/*
  * enm_run_batch_static: enm_run_batch without any device allocation.
  *
  * No counterpart in the reference.  It is for callers that cannot allocate
@@ -226,6 +255,7 @@ typedef struct enm_prep_task {
 
 #define ENM_P_DROP_NAN 0x1u
 #define ENM_P_RESCALE  0x2u
+#define ENM_P_HINTS    0x4u  /* enm_prep_write keeps what enm_prep_hints needs */
 
 int enm_prep_count(enm_ctx* ctx, const enm_prep_task* tasks, int32_t n_tasks,
                    const double* d_raw, const uint8_t* d_mask, uint32_t flags,
@@ -234,6 +264,26 @@ int enm_prep_write(enm_ctx* ctx, const enm_prep_task* tasks, int32_t n_tasks,
                    const double* d_raw, const uint8_t* d_mask,
                    const double* d_noise, int64_t noise_len, uint32_t flags,
                    const int64_t* arena_off, double* d_arena, uint32_t* d_const,
+                   void* stream);
+/*
+ * enm_prep_hints: sort hints for enm_run_batch_hinted, for the columns the
+ * preceding enm_prep_write (with ENM_P_HINTS; same tasks, inputs, noise,
+ * flags, arena_off and d_arena) made.  d_hint (device, int32) is parallel
+ * to d_arena: column c of task t (x, y, then the condition columns) gets at
+ * d_hint[arena_off[t] + c * n[t] ...] its ascending order (task positions of
+ * its values, smallest first, equal values by position), or
+ * d_hint[arena_off[t] + c * n[t]] = -1 where it was not derived.
+ *
+ * Each raw series is sorted once over the rows that any window reads; a
+ * column's order is its series' sorted rows filtered to the task's kept
+ * rows, with the runs of near-equal values that the noise may have
+ * reordered sorted again (a run longer than 32 values, ties in the raw
+ * data, leaves the column without a hint).  Asynchronous.
+ */
+int enm_prep_hints(enm_ctx* ctx, const enm_prep_task* tasks, int32_t n_tasks,
+                   const double* d_raw, const uint8_t* d_mask,
+                   const double* d_noise, int64_t noise_len, uint32_t flags,
+                   const int64_t* arena_off, const double* d_arena, int32_t* d_hint,
                    void* stream);
 
 /*
