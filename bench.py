@@ -7,11 +7,14 @@ coupling 0.2, burn-in 1000); y = series 0, x = all eight series, each x column
 with NaNs at a random 0-30 % of rows; estimate_mi(y, x, lag=-48..48, k=3,
 drop_nan=True) = 776 KSG estimates with uneven n (about 70,000-100,000).
 
-A step is one run of all 776 estimates with the preprocessed inputs already
-resident in HBM: sort, gather, kNN, counts, digamma means (one enm_run_batch,
-replayed as one hipGraph).  Host validation and rescaling (the reference's
-NumPy operations) run once before the timed region and are reported
-separately as host_prep_s.
+A step is the whole call's device path with the raw series already resident
+in HBM: the tasks' kept rows counted (enm_prep_count; the host validates with
+the counts as ennemi does), the lag windows compacted and standardised with
+the reference's noise (enm_prep_write), each column's sort hint derived from
+one sort per series (enm_prep_hints), then sorts, kNN, counts and digamma
+means of all 776 estimates (enm_run_batch_hinted, replayed as one hipGraph).
+The one-time host work (packing the series, the upload, the noise draw) is
+reported separately as host_setup_s.
 
 The JSON line also carries
   roofline     the dominant kernel's algorithmic throughput against its peak,
@@ -53,6 +56,20 @@ PEAK_HBM_GBS = 8000.0
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
+
+
+def time_steps(step, steps: int, warmup: int):
+    """Run warmup + timed calls of step(); ms per step between two device
+    synchronisations."""
+    import torch
+    for _ in range(warmup):
+        step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        step()
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t0) * 1e3 / steps
 
 
 def time_batch(db, steps: int, warmup: int, stream):
@@ -259,9 +276,10 @@ def main() -> int:
     t0 = time.perf_counter()
     batch, indices = plan_mi(y, x, lag=lags, k=3, drop_nan=True)
     host_prep_s = time.perf_counter() - t0
+    prep = batch.prep   # the device preparation: raw series in HBM
     n_est = len(batch)
     ns = np.array([t[2] for t in batch._inst])
-    log(f"planned {n_est} estimates, n in [{ns.min()}, {ns.max()}], host prep {host_prep_s:.2f} s")
+    log(f"planned {n_est} estimates, n in [{ns.min()}, {ns.max()}], setup {host_prep_s:.2f} s")
 
     mode_flags = nat.F_FP64 if args.fp64 else 0
     t0 = time.perf_counter()
@@ -277,14 +295,39 @@ def main() -> int:
     tuning = tdb.ctx.tuning_for(2, 3, int(ns.max()))
     log(f"tuning: strip={tuning[0]} waves/CU={tuning[1]} list slots={tuning[2]} items/ticket={tuning[3]}")
 
-    # Headline: the whole batch as one hipGraph replay per step.
+    # Headline: per step, the device preparation from the raw series (count,
+    # host validation, compaction and rescaling, sort hints) and the whole
+    # batch as one hipGraph replay, on one stream.
     if args.eager:
         db = tdb
     else:
         db = batch.device(mode_flags | nat.F_GRAPH)
-    wall, st = time_batch(db, args.steps, args.warmup, db.stream)
+
+    def step():
+        prep.count(3)
+        prep.write()
+        db.launch()
+
+    wall = time_steps(step, args.steps, args.warmup)
     values = db.results()
+    st = db.stats()
     log(f"headline: {wall:.3f} ms/step ({'eager' if args.eager else 'graph replays'}); stats {st}")
+    batch_ms, _ = time_batch(db, args.steps, args.warmup, db.stream)
+    log(f"batch alone: {batch_ms:.3f} ms/step")
+    # The preparation's device time: events on its stream around count and write.
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
+    prep_ms = {"count": [], "write_and_hints": []}
+    for _ in range(args.steps):
+        ev[0].record(prep.st)
+        prep.count(3)
+        ev[1].record(prep.st)
+        prep.write()
+        ev[2].record(prep.st)
+        prep.st.synchronize()
+        prep_ms["count"].append(ev[0].elapsed_time(ev[1]))
+        prep_ms["write_and_hints"].append(ev[1].elapsed_time(ev[2]))
+    prep_ms = {k: round(float(np.mean(v)), 4) for k, v in prep_ms.items()}
+    log(f"preparation: {prep_ms}")
 
     # Stage times: eager launches with events between the stages.
     stats = stage_pass(tdb, args.steps)
@@ -293,23 +336,24 @@ def main() -> int:
     eager_values = tdb.results()
     log(f"stage pass: {event_ms:.2f} ms events; stages {stage_mean}")
 
-    # Roofline of the dominant kernel: the kNN walk (one launch per step for
-    # this workload, timed by the events around the kNN stage).  It is
-    # HBM-roofed by its algorithmic traffic: each query point's two
-    # coordinates read once (16 B), its radius written in point order (8 B;
-    # no count of this workload reads it in kNN order, and nothing is
-    # exported) and the permutation read (4 B) -- 28 B per point; candidate
-    # reads are cache hits by design.  `traffic` is the measured HBM bytes per launch
-    # (FETCH_SIZE + WRITE_SIZE, copy-calibrated) from the committed profile.
+    # Roofline of the dominant kernel: the strip kNN (knn_strip_kernel, one
+    # launch per step for this workload, timed by the events around the kNN
+    # stage).  It is HBM-roofed by its algorithmic traffic: each query
+    # point's (x, y) pair read once (16 B), its radius written in point order
+    # (8 B; no count of this workload reads it in kNN order, and nothing is
+    # exported) and the permutation read (4 B) -- 28 B per point; the
+    # neighbouring strips' pairs are staged in LDS from the L2 by design.
+    # `traffic` is the measured HBM bytes per launch (FETCH_SIZE + WRITE_SIZE,
+    # copy-calibrated) from the committed profile.
     n_points = int(ns.sum())
     knn_bytes = 28 * n_points
     knn_ach = knn_bytes / (stage_mean["knn"] * 1e-3) / 1e9
     traffic = None
-    tpath = os.path.join(ROOT, "profiles", "lag_scan_776_ksg_hbm_traffic.json")
+    tpath = os.path.join(ROOT, "profiles", "lag_scan_776_hbm_traffic.json")
     if os.path.exists(tpath):
         with open(tpath) as f:
-            traffic = json.load(f).get("knn_walk_kernel<2, 4>", {}).get("hbm_bytes_per_launch")
-    roofline = {"bound": "hbm", "kernel": "knn_walk_kernel<D=2,P=4>", "achieved": knn_ach,
+            traffic = json.load(f).get("knn_strip_kernel<4>", {}).get("hbm_bytes_per_launch")
+    roofline = {"bound": "hbm", "kernel": "knn_strip_kernel<P=4>", "achieved": knn_ach,
                 "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": knn_ach / PEAK_HBM_GBS, "traffic": traffic,
                 "algorithmic": f"{n_points} query points x 28 B (2 coordinates read, radius written "
                                f"in point order, permutation read)",
@@ -317,17 +361,21 @@ def main() -> int:
                 "pair_evaluations": st["knn_pairs"],
                 "pair_rate_T_per_s": st["knn_pairs"] / (stage_mean["knn"] * 1e-3) / 1e12}
     # Every stage against its own roof (algorithmic bytes / stage time).
-    # sort: two orders per instance (x, y).  Segments of 2049 .. 2^20 keys
-    # are sample-sorted at 44 B per key (bucket count: key read, bucket id
-    # written; scatter: key and bucket id read, key and index written; run
-    # sort: key and index read) and laid out into the orders by the same
-    # kernel (64 B per point for the two orders' columns and permutations);
-    # otherwise a block pass and merge passes at 24 B per key each, then the
-    # gather (64 B per point).  gather: the grid order's strip sort (24 B)
-    # and copy (44 B); count: 20 B per 1-D query.  sort_16B: the sort stage
-    # against the plain 16 B per key (key read and written once).
+    # sort: the headline's columns come with sort hints, so each order is
+    # checked and laid out from its hint: hint read (4 B), key gathered
+    # (8 B), permutation and key written (12 B), and for the x order the y
+    # column gathered and written (16 B) -- 64 B per point over its two
+    # orders.  Unhinted segments of 2049 .. 2^20 keys are sample-sorted at
+    # 44 B per key plus the same layout; otherwise a block pass and merge
+    # passes at 24 B per key each, then the gather (64 B per point).
+    # gather: the grid order's strip sort (24 B) and copy (44 B); count:
+    # 20 B per 1-D query.  sort_16B: the sort stage against the plain 16 B
+    # per key (key read and written once).
     longest = int(ns.max())
-    if 2048 < longest <= 1 << 20:
+    if getattr(batch, "device_hint", None) is not None and 2048 < longest <= 1 << 20:
+        sort_bytes = n_points * 64.0
+        gather_bytes = n_points * (24.0 + 44)
+    elif 2048 < longest <= 1 << 20:
         sort_bytes = 2.0 * n_points * 44 + n_points * 64.0
         gather_bytes = n_points * (24.0 + 44)
     else:
@@ -381,8 +429,11 @@ def main() -> int:
         t0 = time.perf_counter()
         out = ennemi_amd.estimate_mi(y, x, lag=lags, k=3, drop_nan=True)
         e2e_s = time.perf_counter() - t0
+        by_task = np.empty_like(out)
+        for i, (li, v) in enumerate(indices):
+            by_task[li, v] = values[i]
         e2e = {"s": e2e_s, "estimates_per_s": n_est / e2e_s,
-               "values_equal_batch": bool(np.array_equal(out.ravel(), values))}
+               "values_equal_batch": bool(np.array_equal(out, by_task))}
         log("end to end", e2e)
 
     configs = {}
@@ -434,10 +485,13 @@ def main() -> int:
         "config": {"workload": "masked lag scan: 8 coupled AR(1) series x 100,096, "
                                "lags -48..48, k=3, drop_nan (776 KSG estimates, N~1e5)",
                    "n_estimates": n_est, "n_min": int(ns.min()), "n_max": int(ns.max()),
-                   "mode": "one hipGraph replay per step of the batched launch sequence: "
-                           "segmented sorts, strip-grid orders, per-lane FP64 strip walks for the "
-                           "kNN, LDS binary-search 1-D counts (FP32 filter + FP64 recheck only "
-                           "where it pays: brute force, off-grid multi-D counts)",
+                   "mode": "per step: device preparation from the raw series (kept-row counts, "
+                           "host validation, compaction, NumPy-exact standardisation with the "
+                           "reference's noise, sort hints from one sort per series), then one "
+                           "hipGraph replay of the batch: hinted orders checked and laid out, "
+                           "strip-grid orders, strip kNN with LDS-staged neighbours in FP64, LDS "
+                           "binary-search 1-D counts, fixed-order digamma means (FP32 filter + "
+                           "FP64 recheck only where it pays: brute force, off-grid multi-D counts)",
                    "graph_replayed": st["graph_replayed"], "n_launches_eager": stats[-1]["n_launches"],
                    "tuning": {"strip_points": tuning[0], "waves_per_cu": tuning[1],
                               "list_slots": tuning[2], "items_per_ticket": tuning[3]}},
@@ -446,8 +500,9 @@ def main() -> int:
         "end_to_end": e2e,
         "comparators": comparators,
         "configs": configs,
-        "host_prep_s": host_prep_s,
-        "upload_s": upload_s,
+        "host_setup_s": host_prep_s,
+        "batch_only_ms_per_step": batch_ms,
+        "prep_device_ms": prep_ms,
         "event_ms_per_step": event_ms,
         "graph_values_equal_eager": bool(np.array_equal(values, eager_values)),
     }

@@ -431,6 +431,21 @@ def _device_prep_eligible(x, y, cond, discrete_x: bool, discrete_y: bool) -> boo
     return True
 
 
+_noise_cache = {}
+
+
+def _device_noise(n: int, dev):
+    """The first n draws of default_rng(NOISE_SEED).normal(0, 1e-10, .) in
+    HBM: every task draws a prefix of this one stream, so the longest one
+    drawn so far serves every shorter call (kept per device)."""
+    import torch
+    have = _noise_cache.get(str(dev))
+    if have is None or have.numel() < n:
+        have = torch.from_numpy(np.random.default_rng(NOISE_SEED).normal(0.0, 1e-10, n)).to(dev)
+        _noise_cache[str(dev)] = have
+    return have
+
+
 class DevicePrep:
     """The device preparation of one estimate_mi call's continuous tasks
     (windows, mask and NaN drop, rescaling with the reference's noise, and
@@ -494,11 +509,12 @@ class DevicePrep:
         with self.ctx.lock:
             self.ctx.prep_count(self.tasks, self.n_tasks, raw, mask, self.flags & ~nat.P_HINTS,
                                 self.h_n, self.h_nan, self.st.cuda_stream)
-        for t in range(self.n_tasks):
-            if self.h_n[self.place[t]] <= k:
-                raise ValueError(K_TOO_LARGE_MSG)
-            if self.h_nan[self.place[t]]:
-                raise ValueError(NANS_LEFT_MSG)
+        small = self.h_n[self.place] <= k
+        nan = self.h_nan[self.place] != 0
+        failing = np.flatnonzero(small | nan)
+        if len(failing):
+            # The first failing task in task order, its first failed check.
+            raise ValueError(K_TOO_LARGE_MSG if small[failing[0]] else NANS_LEFT_MSG)
 
     def write(self) -> None:
         """The arena (and hints) from the counts of the last count(); the
@@ -514,8 +530,7 @@ class DevicePrep:
                 self.hint = torch.empty(total, dtype=torch.int32, device=self.dev) if self.hints else None
                 self.d_const = torch.zeros(self.n_tasks, dtype=torch.int32, device=self.dev)
             if self.preprocess and noise_len > self.noise_len:
-                self.d_noise = torch.from_numpy(
-                    np.random.default_rng(NOISE_SEED).normal(0.0, 1e-10, noise_len)).to(self.dev)
+                self.d_noise = _device_noise(noise_len, self.dev)
                 self.noise_len = noise_len
         raw, mask = self._ptrs()
         noise = self.d_noise.data_ptr() if self.preprocess else 0

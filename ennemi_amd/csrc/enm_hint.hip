@@ -17,10 +17,12 @@
 //   3. the noise can only swap values whose a differ by about 2M or less:
 //      the column cuts into runs wherever neighbours differ by
 //      a - a' > 2M + 2^-48 (|a| + |a'| + 2M + 1), more than the noise and
-//      the rounding of a + N on both sides, so no value crosses a cut, and
-//      each run of two to ENM_HINT_RUN values is re-sorted by (s, position)
-//      in place.  A longer run (ties or near-ties in the raw data) leaves the
-//      column without a hint (hint[out] = -1) and the batch sorts it.
+//      the rounding of a + N on both sides, so no value crosses a cut.  The
+//      filter marks every position not cut from the one before it (a
+//      continuation bit), and each run of two to ENM_HINT_RUN values is
+//      re-sorted by (s, position) in place.  A longer run (ties or
+//      near-ties in the raw data) leaves the column without a hint
+//      (hint[out] = -1) and the batch sorts it.
 // The batch checks every hint in full before it trusts it, so a hint is an
 // accelerator, never a source of results.
 #include "enm_device.h"
@@ -32,7 +34,6 @@ namespace {
 
 constexpr int HT = 256;                           // threads per block
 constexpr int HPER = ENM_HINT_BLOCK / HT;         // sorted rows per thread
-constexpr int PPER = ENM_HINT_POS / HT;           // column positions per thread
 
 // Keys of every series range: the raw values, NaN as +inf.
 __global__ __launch_bounds__(HT)
@@ -45,15 +46,14 @@ void hint_keys_kernel(const HintSeries* __restrict__ series, const double* __res
     keys[S.kbase + i] = isnan(v) ? INFINITY : v;
 }
 
-// The task position of sorted series row e of column C, or -1 when the row
-// is outside the window or dropped.  The task's kept-row map is either in
-// LDS, as its words' bits and the prefix of every other word (6 bytes per
-// 32 rows, so eight blocks fit a CU), or read in place.
-__device__ __forceinline__ int kept_pos(const HintCol& C, const int* __restrict__ sidx, bool in_lds,
-                                        const uint32_t* lbits, const int* lpre, const KeptWord* __restrict__ gmap,
-                                        int64_t e) {
-    if (e >= C.len) return -1;
-    const int64_t r = (int64_t)sidx[C.sb + e] - C.ws;
+// The task position of series row R (relative to the series range) in
+// column C, or -1 when the row is outside the window or dropped.  The task's
+// kept-row map is either in LDS, as its words' bits and the prefix of every
+// other word (6 bytes per 32 rows, so eight blocks fit a CU), or read in
+// place.
+__device__ __forceinline__ int kept_pos(const HintCol& C, int64_t R, bool in_lds, const uint32_t* lbits,
+                                        const int* lpre, const KeptWord* __restrict__ gmap) {
+    const int64_t r = R - C.ws;
     if (r < 0 || r >= C.w) return -1;
     const int64_t w = r >> 5;
     const uint32_t bit = 1u << (r & 31);
@@ -66,21 +66,34 @@ __device__ __forceinline__ int kept_pos(const HintCol& C, const int* __restrict_
     return (kw.bits & bit) ? kw.prefix + __popc(kw.bits & (bit - 1u)) : -1;
 }
 
+// Whether neighbours with values a' <= a before the noise are far enough
+// apart that the noise cannot swap anything across them (m2 = 2M).
+__device__ __forceinline__ bool is_cut(double ap, double a, double m2) {
+    return a - ap > m2 + 0x1p-48 * (fabs(a) + fabs(ap) + m2 + 1.0);
+}
+
 // One block per column: the task's kept-row map into LDS (when it fits),
 // then the series' sorted rows in chunks of ENM_HINT_BLOCK, each compacted
 // in order (ballots, wave offsets) with the running offset carried from
 // chunk to chunk.  Writes the kept rows' positions, in sorted order, to the
-// hint and each one's value before the noise to pre[] (same offsets); a
-// column whose kept rows do not add up to n is marked.  Columns of one
-// series run on one XCD.
-__global__ __launch_bounds__(HT)
+// hint; a kept row whose value before the noise is not cut from the one
+// before it (in the column's order) sets its continuation bit (cont, the
+// column's words from C.cbits; zeroed before).  A column whose kept rows
+// do not add up to n is marked.  Columns of one series run on one XCD.
+__global__ __launch_bounds__(HT, 8)   // eight waves per SIMD: one round for every column
 void hint_filter_kernel(const HintCol* __restrict__ cols, const int* __restrict__ sidx,
                         const double* __restrict__ skey, const KeptWord* __restrict__ kept,
                         const PrepTaskDev* __restrict__ tasks, const double* __restrict__ stats, int rescale,
-                        int lds_words, int* __restrict__ hint, double* __restrict__ pre, int* __restrict__ bad) {
+                        const unsigned long long* __restrict__ noise_max, int lds_words, int* __restrict__ hint,
+                        unsigned* __restrict__ cont, int* __restrict__ bad) {
     constexpr int NW = HT / ENM_WAVE;
     extern __shared__ uint32_t smap[];   // lds_words bits, then (lds_words + 1) / 2 prefixes
     __shared__ int s_wcnt[HPER][NW];
+    __shared__ double s_last[HPER][NW];  // value of each (round, wave)'s last kept row
+    __shared__ double s_prev[HPER][NW];  // value of the last kept row before each (round, wave)
+    __shared__ int s_has[HPER][NW];      // whether there is one
+    __shared__ double s_carry;
+    __shared__ int s_hascarry;
     const int ci = (int)xcd_block(blockIdx.x, gridDim.x);
     const HintCol C = cols[ci];
     const int64_t words = (C.w + 31) / 32;
@@ -94,6 +107,7 @@ void hint_filter_kernel(const HintCol* __restrict__ cols, const int* __restrict_
             if (!(t & 1)) lpre[t >> 1] = kw.prefix;
         }
     const KeptWord* gmap = kept + C.pm;
+    if (threadIdx.x == 0) s_hascarry = 0;
     // The column's standardisation, as the preparation applied it (a
     // constant variable is left as it is).
     const PrepTaskDev& T = tasks[C.task];
@@ -101,19 +115,74 @@ void hint_filter_kernel(const HintCol* __restrict__ cols, const int* __restrict_
     const bool scaled = rescale && !(T.flags & cbit);
     const double mean = scaled ? stats[T.stat + 2 * C.slot] : 0.0;
     const double sd = scaled ? stats[T.stat + 2 * C.slot + 1] : 1.0;
+    const double m2 = 2.0 * __longlong_as_double((long long)*noise_max);
     const int lane = threadIdx.x & (ENM_WAVE - 1), wv = threadIdx.x / ENM_WAVE;
     __syncthreads();
     int64_t at = 0;
-    for (int64_t e0 = 0; e0 < C.len; e0 += ENM_HINT_BLOCK) {
-        int pos[HPER], below[HPER];
+    // Rows and keys of a chunk are loaded together, one chunk ahead.
+    int nR[HPER];
+    double nkey[HPER];
+    auto load = [&](int64_t e0) {
 #pragma unroll
-        for (int j = 0; j < HPER; ++j)
-            pos[j] = kept_pos(C, sidx, in_lds, lbits, lpre, gmap, e0 + j * HT + threadIdx.x);
+        for (int j = 0; j < HPER; ++j) {
+            const int64_t e = e0 + j * HT + threadIdx.x;
+            const bool in = e < C.len;
+            nR[j] = in ? sidx[C.sb + e] : -1;
+            nkey[j] = in ? skey[C.sb + e] : 0.0;
+        }
+    };
+    load(0);
+    for (int64_t e0 = 0; e0 < C.len; e0 += ENM_HINT_BLOCK) {
+        int pos[HPER];
+        double av[HPER];
+        int R[HPER];
+        double key[HPER];
+#pragma unroll
+        for (int j = 0; j < HPER; ++j) {
+            R[j] = nR[j];
+            key[j] = nkey[j];
+        }
+        load(e0 + ENM_HINT_BLOCK);
+#pragma unroll
+        for (int j = 0; j < HPER; ++j) {
+            pos[j] = R[j] >= 0 ? kept_pos(C, R[j], in_lds, lbits, lpre, gmap) : -1;
+            av[j] = scaled ? (key[j] - mean) / sd : key[j];
+        }
 #pragma unroll
         for (int j = 0; j < HPER; ++j) {
             const unsigned long long m = __builtin_amdgcn_ballot_w64(pos[j] >= 0);
-            below[j] = __popcll(m & ((1ull << lane) - 1ull));
-            if (lane == 0) s_wcnt[j][wv] = __popcll(m);
+            const int top = m ? 63 - __clzll(m) : -1;   // the wave's last kept lane
+            const double last = __shfl(av[j], top < 0 ? 0 : top, ENM_WAVE);
+            if (lane == 0) {
+                s_wcnt[j][wv] = __popcll(m);
+                s_last[j][wv] = last;
+            }
+        }
+        __syncthreads();
+        if (threadIdx.x < ENM_WAVE) {
+            // The value before each (round, wave) in the column's order: lane
+            // l = round * NW + wave takes the last (round, wave) below it with
+            // kept rows, else the carry from the previous chunk.
+            static_assert(HPER * NW <= ENM_WAVE, "one lane per (round, wave)");
+            const int l = threadIdx.x;
+            const bool mine = l < HPER * NW && s_wcnt[l / NW][l % NW] > 0;
+            const unsigned long long any = __builtin_amdgcn_ballot_w64(mine);
+            const unsigned long long lower = any & ((1ull << l) - 1ull);
+            const double carry = s_carry;
+            const int hascarry = s_hascarry;
+            if (l < HPER * NW) {
+                const int src = lower ? 63 - __clzll(lower) : -1;
+                s_prev[l / NW][l % NW] = src >= 0 ? s_last[src / NW][src % NW] : carry;
+                s_has[l / NW][l % NW] = src >= 0 ? 1 : hascarry;
+            }
+            // Every lane read the old carry before lane 0 replaces it (the
+            // wave runs in lockstep and the reads came first).
+            __builtin_amdgcn_wave_barrier();
+            if (l == 0 && any) {
+                const int top = 63 - __clzll(any);
+                s_carry = s_last[top / NW][top % NW];
+                s_hascarry = 1;
+            }
         }
         __syncthreads();
 #pragma unroll
@@ -124,15 +193,22 @@ void hint_filter_kernel(const HintCol* __restrict__ cols, const int* __restrict_
                 before += w < wv ? s_wcnt[j][w] : 0;
                 total += s_wcnt[j][w];
             }
-            const int64_t o = at + before + below[j];
+            // The kept row before this one: a lower kept lane of the wave,
+            // else the last one before the wave.
+            const unsigned long long lower =
+                __builtin_amdgcn_ballot_w64(pos[j] >= 0) & ((1ull << lane) - 1ull);
+            const int64_t o = at + before + __popcll(lower);
+            const int pl = lower ? 63 - __clzll(lower) : 0;
+            const double pa = __shfl(av[j], pl, ENM_WAVE);
             if (pos[j] >= 0 && o < C.n) {
-                const double v = skey[C.sb + e0 + j * HT + threadIdx.x];
                 hint[C.out + o] = pos[j];
-                pre[C.out + o] = scaled ? (v - mean) / sd : v;
+                const bool has = lower ? true : s_has[j][wv] != 0;
+                const double ap = lower ? pa : s_prev[j][wv];
+                if (has && !is_cut(ap, av[j], m2)) atomicOr(cont + C.cbits + (o >> 5), 1u << (o & 31));
             }
             at += total;
         }
-        __syncthreads();   // s_wcnt is rewritten by the next chunk
+        __syncthreads();   // s_wcnt, s_last, s_prev are rewritten by the next chunk
     }
     if (threadIdx.x == 0) bad[ci] = at != C.n ? 1 : 0;
 }
@@ -156,46 +232,40 @@ void hint_noise_max_kernel(const double* __restrict__ noise, int64_t len, unsign
     }
 }
 
-// Whether neighbours with values a' <= a before the noise are far enough
-// apart that the noise cannot swap anything across them (m2 = 2M).
-__device__ __forceinline__ bool is_cut(double ap, double a, double m2) {
-    return a - ap > m2 + 0x1p-48 * (fabs(a) + fabs(ap) + m2 + 1.0);
-}
-
-// Re-sort each run of two to ENM_HINT_RUN positions (positions i .. j-1 with
-// no cut between them, cuts before i and at j) by (value, position), in
-// place, by the thread at its start; a longer run marks the column.  Only
-// run members' values are gathered.
+// Re-sort each run of two to ENM_HINT_RUN positions (a position without
+// its continuation bit followed by positions with it) by (value, position)
+// in place, by the thread of the word holding its start; a longer run marks
+// the column.  Only run members' values are gathered.
 __global__ __launch_bounds__(HT)
-void hint_run_kernel(const HintCol* __restrict__ cols, int64_t pstride,
-                     const unsigned long long* __restrict__ noise_max, const double* __restrict__ arena,
-                     const double* __restrict__ pre, int* __restrict__ hint, int* __restrict__ bad) {
-    const int64_t b = xcd_block(blockIdx.x, gridDim.x);   // a column's blocks on one XCD
-    const int ci = (int)(b / pstride);
+void hint_run_kernel(const HintCol* __restrict__ cols, int64_t wstride, const double* __restrict__ arena,
+                     const unsigned* __restrict__ cont, int* __restrict__ hint, int* __restrict__ bad) {
+    const int64_t b = xcd_block(blockIdx.x, gridDim.x);
+    const int ci = (int)(b / wstride);
     const HintCol C = cols[ci];
-    const int64_t p0 = (b - ci * pstride) * ENM_HINT_POS;
-    if (p0 >= C.n || bad[ci]) return;
-    const double m2 = 2.0 * __longlong_as_double((long long)*noise_max);
-    const double* a = pre + C.out;
+    const int64_t t = (b - ci * wstride) * HT + threadIdx.x;   // the word
+    const int64_t words = (C.n + 31) / 32;
+    if (t >= words || bad[ci]) return;
+    const unsigned* cw = cont + C.cbits;
+    const uint64_t both = ((uint64_t)(t + 1 < words ? cw[t + 1] : 0u) << 32) | cw[t];
+    uint64_t starts = ~both & (both >> 1) & 0xffffffffull;   // no bit at i, bit at i + 1
+    if (!starts) return;
     const double* v = arena + C.out;
     int* h = hint + C.out;
-#pragma unroll
-    for (int jj = 0; jj < PPER; ++jj) {
-        const int64_t i = p0 + jj * HT + threadIdx.x;
-        if (i + 1 >= C.n) continue;
-        const double ai = a[i];
-        if ((i > 0 && !is_cut(a[i - 1], ai, m2)) || is_cut(ai, a[i + 1], m2)) continue;   // no run starts here
-        int64_t j = i + 1;
-        while (j < C.n && j - i <= ENM_HINT_RUN && !is_cut(a[j - 1], a[j], m2)) ++j;
-        if (j - i > ENM_HINT_RUN) {
+    auto is_cont = [&](int64_t p) { return (cw[p >> 5] >> (p & 31)) & 1u; };
+    while (starts) {
+        const int64_t s = 32 * t + __builtin_ctzll(starts);
+        starts &= starts - 1;
+        int64_t j = s + 1;
+        while (j < C.n && j - s <= ENM_HINT_RUN && is_cont(j)) ++j;
+        if (j - s > ENM_HINT_RUN) {
             bad[ci] = 1;
-            continue;
+            return;
         }
-        for (int64_t u = i + 1; u < j; ++u) {
+        for (int64_t u = s + 1; u < j; ++u) {
             const int hu = h[u];
             const double su = v[hu];
             int64_t q = u - 1;
-            for (; q >= i; --q) {
+            for (; q >= s; --q) {
                 const int hq = h[q];
                 const double sq = v[hq];
                 if (sq < su || (sq == su && hq < hu)) break;
@@ -223,35 +293,31 @@ hipError_t enm_launch_hint_keys(const HintSeries* series, int n_series, int64_t 
     return hipGetLastError();
 }
 
-hipError_t enm_launch_hint_filter(const HintCol* cols, int n_cols, int64_t max_w, const int* sidx,
-                                  const double* skey, const KeptWord* kept, const PrepTaskDev* tasks,
-                                  const double* stats, int rescale, int* hint, double* pre, int* bad,
-                                  hipStream_t st) {
-    if (n_cols <= 0) return hipSuccess;
-    // Kept-row maps of up to 64 KB go to LDS; larger ones are read in place.
-    const int64_t words = (max_w + 31) / 32;
-    const size_t lds = (size_t)(words + (words + 1) / 2) * 4;
-    const int lds_words = lds <= 65536 ? (int)words : 0;
-    hipLaunchKernelGGL(hint_filter_kernel, dim3((unsigned)n_cols), dim3(HT), lds_words ? lds : 0, st, cols, sidx,
-                       skey, kept, tasks, stats, rescale, lds_words, hint, pre, bad);
-    return hipGetLastError();
-}
-
-hipError_t enm_launch_hint_runs(const HintCol* cols, int n_cols, int64_t pstride, const double* noise,
-                                int64_t noise_len, double* noise_max, const double* arena, const double* pre,
-                                int* hint, int* bad, hipStream_t st) {
+hipError_t enm_launch_hint_order(const HintCol* cols, int n_cols, int64_t max_w, int64_t max_n,
+                                 const int* sidx, const double* skey, const KeptWord* kept,
+                                 const PrepTaskDev* tasks, const double* stats, int rescale, const double* noise,
+                                 int64_t noise_len, double* noise_max, unsigned* cont, int64_t cont_words,
+                                 const double* arena, int* hint, int* bad, hipStream_t st) {
     if (n_cols <= 0) return hipSuccess;
     unsigned long long* nm = reinterpret_cast<unsigned long long*>(noise_max);
     hipError_t e = enm_launch_fill_u64(nm, 1, 0ull, st);
+    if (e == hipSuccess) e = enm_launch_fill_i32(reinterpret_cast<int*>(cont), cont_words, 0, st);
     if (e != hipSuccess) return e;
     const int64_t len = noise ? noise_len : 0;
     if (len > 0) {
         const int64_t blocks = std::min<int64_t>(1024, (len + 16 * HT - 1) / (16 * HT));
         hipLaunchKernelGGL(hint_noise_max_kernel, dim3((unsigned)blocks), dim3(HT), 0, st, noise, len, nm);
     }
-    if (pstride > 0)
-        hipLaunchKernelGGL(hint_run_kernel, dim3((unsigned)(pstride * n_cols)), dim3(HT), 0, st, cols, pstride, nm,
-                           arena, pre, hint, bad);
+    // Kept-row maps of up to 64 KB go to LDS; larger ones are read in place.
+    const int64_t words = (max_w + 31) / 32;
+    const size_t lds = (size_t)(words + (words + 1) / 2) * 4;
+    const int lds_words = lds <= 65536 ? (int)words : 0;
+    hipLaunchKernelGGL(hint_filter_kernel, dim3((unsigned)n_cols), dim3(HT), lds_words ? lds : 0, st, cols, sidx,
+                       skey, kept, tasks, stats, rescale, nm, lds_words, hint, cont, bad);
+    const int64_t wstride = ((max_n + 31) / 32 + HT - 1) / HT;
+    if (wstride > 0)
+        hipLaunchKernelGGL(hint_run_kernel, dim3((unsigned)(wstride * n_cols)), dim3(HT), 0, st, cols, wstride, arena,
+                           cont, hint, bad);
     hipLaunchKernelGGL(hint_mark_kernel, dim3((unsigned)((n_cols + 63) / 64)), dim3(64), 0, st, cols, n_cols, bad,
                        hint);
     return hipGetLastError();

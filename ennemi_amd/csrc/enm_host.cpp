@@ -218,7 +218,7 @@ struct enm_ctx {
         int64_t noise_len = 0;
         uint32_t flags = 0;
         std::vector<char> blob3;
-        DevBuf kept, tables3, hkeys, hk0, hk1, hi0, hi1, hsplit, hpre, hbad, hmax;
+        DevBuf kept, tables3, hkeys, hk0, hk1, hi0, hi1, hsplit, hcont, hbad, hmax;
     } prep;
     // Workspace of enm_unique_rows.
     struct Unique {
@@ -1706,7 +1706,7 @@ int enm_prep_hints(enm_ctx* ctx, const enm_prep_task* tasks, int32_t n_tasks, co
     // The columns, by column slot and then by task (consecutive filter
     // blocks, one XCD's share, read one series).
     std::vector<HintCol> cols;
-    int64_t max_w = 0, pstride = 0, extent = 0;
+    int64_t max_w = 0, max_n = 0, cont_words = 0;
     for (int c = 0; c < max_cols; ++c) {
         for (int t = 0; t < n_tasks; ++t) {
             const PrepTaskDev& D = S.tasks[t];
@@ -1725,9 +1725,10 @@ int enm_prep_hints(enm_ctx* ctx, const enm_prep_task* tasks, int32_t n_tasks, co
             C.len = (int32_t)H.len;
             C.task = t;
             C.slot = c;
+            C.cbits = cont_words;
+            cont_words += (C.n + 31) / 32;
             max_w = std::max<int64_t>(max_w, D.w);
-            pstride = std::max<int64_t>(pstride, (C.n + ENM_HINT_POS - 1) / ENM_HINT_POS);
-            extent = std::max(extent, C.out + C.n);
+            max_n = std::max<int64_t>(max_n, C.n);
             cols.push_back(C);
         }
     }
@@ -1739,7 +1740,7 @@ int enm_prep_hints(enm_ctx* ctx, const enm_prep_task* tasks, int32_t n_tasks, co
     if (S.tables3.ensure(S.blob3.size()) != hipSuccess || S.hkeys.ensure((size_t)nkeys * 8 + 8) != hipSuccess ||
         S.hk0.ensure((size_t)nkeys * 8 + 8) != hipSuccess || S.hk1.ensure((size_t)nkeys * 8 + 8) != hipSuccess ||
         S.hi0.ensure((size_t)nkeys * 4 + 4) != hipSuccess || S.hi1.ensure((size_t)nkeys * 4 + 4) != hipSuccess ||
-        S.hsplit.ensure((size_t)ntiles * 4 + 4) != hipSuccess || S.hpre.ensure((size_t)extent * 8 + 8) != hipSuccess ||
+        S.hsplit.ensure((size_t)ntiles * 4 + 4) != hipSuccess || S.hcont.ensure((size_t)cont_words * 4 + 4) != hipSuccess ||
         S.hbad.ensure(ncols * 4 + 4) != hipSuccess || S.hmax.ensure(8) != hipSuccess) {
         (void)hipGetLastError();
         return fail(ctx, ENM_E_NOMEM, "hipMalloc failed for the sort-hint workspace");
@@ -1760,13 +1761,11 @@ int enm_prep_hints(enm_ctx* ctx, const enm_prep_task* tasks, int32_t n_tasks, co
     const int* sidx = fb ? S.hi1.as<int>() : S.hi0.as<int>();
     const double* skey = fb ? S.hk1.as<double>() : S.hk0.as<double>();
     const PrepTaskDev* dtasks = reinterpret_cast<const PrepTaskDev*>(S.tables2.as<char>() + S.o_tasks);
-    e = enm_launch_hint_filter(dcol, (int)ncols, max_w, sidx, skey, S.kept.as<KeptWord>(), dtasks,
-                               S.stats.as<double>(), (flags & ENM_P_RESCALE) ? 1 : 0, d_hint, S.hpre.as<double>(),
-                               S.hbad.as<int>(), st);
-    if (e != hipSuccess) return hip_fail(ctx, e, "hint filter");
-    e = enm_launch_hint_runs(dcol, (int)ncols, pstride, (flags & ENM_P_RESCALE) ? d_noise : nullptr, noise_len,
-                             S.hmax.as<double>(), d_arena, S.hpre.as<double>(), d_hint, S.hbad.as<int>(), st);
-    if (e != hipSuccess) return hip_fail(ctx, e, "hint runs");
+    e = enm_launch_hint_order(dcol, (int)ncols, max_w, max_n, sidx, skey, S.kept.as<KeptWord>(), dtasks,
+                              S.stats.as<double>(), (flags & ENM_P_RESCALE) ? 1 : 0,
+                              (flags & ENM_P_RESCALE) ? d_noise : nullptr, noise_len, S.hmax.as<double>(),
+                              S.hcont.as<unsigned>(), cont_words, d_arena, d_hint, S.hbad.as<int>(), st);
+    if (e != hipSuccess) return hip_fail(ctx, e, "hint order");
     return ENM_OK;
 }
 

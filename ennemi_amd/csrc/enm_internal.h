@@ -254,13 +254,13 @@ struct HintCol {
     int64_t pm;      // the task's kept-row map (first word)
     int64_t out;     // arena (and hint) offset of the column
     int64_t n;       // kept rows
+    int64_t cbits;   // first word of its continuation bits (one per position)
     int32_t len;     // rows of the series range
     int32_t task;    // the task (its statistics and constant flags)
     int32_t slot;    // the column: 0 x, 1 y, 2 + c condition column c
     int32_t pad;
 };
-#define ENM_HINT_BLOCK 2048   // sorted series rows per filter block (256 threads x 8)
-#define ENM_HINT_POS 1024     // column positions per run block (256 threads x 4)
+#define ENM_HINT_BLOCK 1024   // sorted series rows per filter chunk (256 threads x 4)
 #define ENM_HINT_RUN 32       // longest run of near-equal values re-sorted (longer: no hint)
 
 // One array whose mean and standard deviation NumPy's way are needed.

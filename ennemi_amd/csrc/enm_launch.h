@@ -97,15 +97,13 @@ hipError_t enm_launch_prep_write(PrepTaskDev* tasks, int64_t stride, int n_tasks
 // Sort hints (enm_hint.hip).
 hipError_t enm_launch_hint_keys(const HintSeries* series, int n_series, int64_t maxlen, const double* raw,
                                 double* keys, hipStream_t st);
-// One filter block per column; column ci owns run blocks
-// [ci * pstride, (ci + 1) * pstride).
-hipError_t enm_launch_hint_filter(const HintCol* cols, int n_cols, int64_t max_w, const int* sidx,
-                                  const double* skey, const KeptWord* kept, const PrepTaskDev* tasks,
-                                  const double* stats, int rescale, int* hint, double* pre, int* bad,
-                                  hipStream_t st);
-hipError_t enm_launch_hint_runs(const HintCol* cols, int n_cols, int64_t pstride, const double* noise,
-                                int64_t noise_len, double* noise_max, const double* arena, const double* pre,
-                                int* hint, int* bad, hipStream_t st);
+// The hint of every column: noise maximum, filter (one block per column),
+// runs re-sorted, columns without a hint marked.
+hipError_t enm_launch_hint_order(const HintCol* cols, int n_cols, int64_t max_w, int64_t max_n,
+                                 const int* sidx, const double* skey, const KeptWord* kept,
+                                 const PrepTaskDev* tasks, const double* stats, int rescale, const double* noise,
+                                 int64_t noise_len, double* noise_max, unsigned* cont, int64_t cont_words,
+                                 const double* arena, int* hint, int* bad, hipStream_t st);
 
 hipError_t enm_launch_uq_codes(const UqSeg* segs, int nsegs, const double* skeys, const int* sidx, int* codes,
                                int64_t* starts, int64_t* counts, int64_t* distinct, hipStream_t st);
