@@ -147,7 +147,7 @@ struct Plan {
     // device
     int64_t heap_bytes = 0;   // global heap lists of kNN groups with k+1 > ENM_LIST_LDS_MAX
     DevBuf tables, sk0, si0, sk1, si1, split, col64, col32, perm, res64, res32, maxabs, counters, stats,
-        deferred, heap, samp_k0, samp_k1, samp_i0, samp_i1, samp_split, bid, ss_counts, ss_tiles, ss_ctl,
+        deferred, heap, samp_k0, samp_k1, samp_i0, samp_i1, samp_split, bid, ss_counts, ss_tiles, ss_long, ss_ctl,
         hint_bad;
     size_t off_segs = 0, off_gitems = 0, off_orders = 0, off_kdescs = 0, off_cdescs = 0, off_insts = 0,
            off_cls = 0, off_kitems = 0, off_citems = 0, off_segs2 = 0, off_gitems2 = 0, off_rdescs = 0,
@@ -975,7 +975,8 @@ std::vector<BufSpec> workspace_layout(Plan& P) {
         v.push_back({&P.bid, (size_t)P.sort_elems * 2 + 2, "buckets"});
         v.push_back({&P.ss_counts, (size_t)P.n_counts * 4 + 4, "bucket counts"});
         v.push_back({&P.ss_tiles, (size_t)P.n_buckets * sizeof(SsTile), "bucket runs"});
-        v.push_back({&P.ss_ctl, 2 * ENM_NQ * ENM_CSTRIDE * 4, "run queue counters"});
+        v.push_back({&P.ss_long, (size_t)P.n_buckets * sizeof(SsTile), "long bucket runs"});
+        v.push_back({&P.ss_ctl, (2 * ENM_NQ + 1) * ENM_CSTRIDE * 4, "run queue counters"});
         if (P.hinted) v.push_back({&P.hint_bad, P.segs.size() * 4 + 4, "hint checks"});
     }
     return v;
@@ -1071,6 +1072,8 @@ int enqueue(enm_ctx* ctx, Plan& P, const double* arena, const int* hint, double*
         a.tiles = P.ss_tiles.as<SsTile>();
         a.qbase = tab<int>(P, P.off_qbase);
         a.qctl = P.ss_ctl.as<unsigned>();
+        a.long_runs = P.ss_long.as<SsTile>();
+        a.long_grid = std::max(1, ctx->num_cus);
         // ss_tile_kernel holds 50 KB of LDS: three blocks per CU.
         a.tile_grid = std::max(1, ctx->num_cus * 3);
         a.cons = tab<SsCons>(P, P.off_cons);
@@ -1603,11 +1606,24 @@ int enm_prep_write(enm_ctx* ctx, const enm_prep_task* tasks, int32_t n_tasks, co
                       (long long)noise_len, (long long)need_noise);
         return fail(ctx, ENM_E_ARG, msg);
     }
+    // Chunks listed full ones first (8192 values: one wave each, four to a
+    // block), then the last chunk of each array.
+    std::vector<int> chunk_list;
+    chunk_list.reserve(chunk_sum_of.size());
+    for (int pass = 0; pass < 2; ++pass)
+        for (size_t c = 0; c < chunk_sum_of.size(); ++c) {
+            const bool full = ((int64_t)chunk_index[c] + 1) * 8192 <= sums[chunk_sum_of[c]].n;
+            if (full == (pass == 0)) chunk_list.push_back((int)c);
+        }
+    int n_full = 0;
+    for (size_t c = 0; c < chunk_sum_of.size(); ++c)
+        n_full += ((int64_t)chunk_index[c] + 1) * 8192 <= sums[chunk_sum_of[c]].n ? 1 : 0;
     S.blob2.clear();
     const size_t o_t = append(S.blob2, S.tasks);
     const size_t o_s = append(S.blob2, sums);
     const size_t o_cs = append(S.blob2, chunk_sum_of);
     const size_t o_ci = append(S.blob2, chunk_index);
+    const size_t o_cl = append(S.blob2, chunk_list);
     const bool hints = (flags & ENM_P_HINTS) != 0;
     if (S.tables2.ensure(S.blob2.size()) != hipSuccess ||
         S.chunk_val.ensure(chunk_sum_of.size() * 8 + 8) != hipSuccess ||
@@ -1623,7 +1639,8 @@ int enm_prep_write(enm_ctx* ctx, const enm_prep_task* tasks, int32_t n_tasks, co
                               S.drop_nan ? 1 : 0, rescale ? 1 : 0, S.blk_off.as<int64_t>(),
                               reinterpret_cast<const PrepSumDev*>(base + o_s), (int)sums.size(),
                               reinterpret_cast<const int*>(base + o_cs), reinterpret_cast<const int*>(base + o_ci),
-                              (int)chunk_sum_of.size(), S.chunk_val.as<double>(), S.stats.as<double>(), sstride,
+                              reinterpret_cast<const int*>(base + o_cl), (int)chunk_sum_of.size(), n_full,
+                              S.chunk_val.as<double>(), S.stats.as<double>(), sstride,
                               d_noise, d_const, d_arena, hints ? S.kept.as<KeptWord>() : nullptr, st);
     if (e != hipSuccess) return hip_fail(ctx, e, "prep write");
     if (!rescale) {

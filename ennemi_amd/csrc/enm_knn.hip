@@ -27,8 +27,6 @@ hipError_t enm_launch_knn(const Item* items, const int* qoff, int n_items, unsig
                           unsigned long long* stats, int dclass, int pclass, bool f32, bool brute,
                           int walk, int grid, int k1max, double* heap, int ticket, int smax, hipStream_t st) {
     if (n_items <= 0) return hipSuccess;
-    // Tickets of up to ENM_KCHUNK items, but at least four per wave of the
-    // grid, so small launches (one instance) still fill the machine.
     // Tickets of up to ENM_KCHUNK items (or the tuned items per ticket), but
     // at least four per wave of the grid, so small launches still fill it.
     const int chunk = std::max(1, std::min(ticket > 0 ? ticket : ENM_KCHUNK, n_items / (4 * std::max(1, grid))));

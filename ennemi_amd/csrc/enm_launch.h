@@ -30,7 +30,9 @@ struct SampleSortArgs {
     int* counts;                // (chunk, bucket) counts, then scatter offsets
     SsTile* tiles;              // run queues (queue q from qbase[q]; at most its buckets)
     const int* qbase;           // ENM_NQ + 1 queue offsets (device)
-    unsigned* qctl;             // 2 * ENM_NQ counters, ENM_CSTRIDE apart: appends, pops
+    unsigned* qctl;             // 2 * ENM_NQ + 1 counters, ENM_CSTRIDE apart: appends, pops, long runs
+    SsTile* long_runs;          // runs of one bucket larger than an LDS run (at most its buckets)
+    int long_grid;
     int tile_grid;
     const SsCons* cons;         // consumers of each segment (SampleSeg::cons0, ncons)
     const OrderDev* orders;
@@ -90,8 +92,8 @@ hipError_t enm_launch_prep_count(const PrepTaskDev* tasks, int64_t stride, int n
 hipError_t enm_launch_prep_write(PrepTaskDev* tasks, int64_t stride, int n_tasks,
                                  const double* raw, const unsigned char* mask, int drop_nan, int rescale,
                                  const int64_t* blk_off, const PrepSumDev* sums, int n_sums,
-                                 const int* chunk_sum_of, const int* chunk_index, int n_chunks,
-                                 double* chunk_val, double* stats, int64_t sstride,
+                                 const int* chunk_sum_of, const int* chunk_index, const int* chunk_list,
+                                 int n_chunks, int n_full, double* chunk_val, double* stats, int64_t sstride,
                                  const double* noise, unsigned* const_flag, double* arena, KeptWord* kept,
                                  hipStream_t st);
 // Sort hints (enm_hint.hip).

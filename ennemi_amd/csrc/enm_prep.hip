@@ -311,14 +311,15 @@ __device__ __forceinline__ double pw_full_chunk(const Term& f) {
 // lane 0 combines them in NumPy's order.
 __global__ __launch_bounds__(ENM_WAVE)
 void prep_chunk_kernel(const PrepSumDev* __restrict__ sums, const int* __restrict__ chunk_sum_of,
-                       const int* __restrict__ chunk_index, int n_chunks, const double* __restrict__ arena,
-                       const double* __restrict__ stats, int pass, double* __restrict__ chunk_val) {
+                       const int* __restrict__ chunk_index, const int* __restrict__ list, int n_list,
+                       const double* __restrict__ arena, const double* __restrict__ stats, int pass,
+                       double* __restrict__ chunk_val) {
     __shared__ int s_lo[PW_LEAF], s_len[PW_LEAF];
     __shared__ double s_leaf[PW_LEAF];
     __shared__ int s_nl;
-    const int c = blockIdx.x;
-    if (c >= n_chunks) return;
-    const PrepSumDev sd = sums[chunk_sum_of[c]];
+    if ((int)blockIdx.x >= n_list) return;
+    const int c = list[blockIdx.x];
+    const PrepSumDev& sd = sums[chunk_sum_of[c]];
     const int64_t first = (int64_t)chunk_index[c] * PW_CHUNK;
     const int m = (int)min((int64_t)PW_CHUNK, sd.n - first);
     const Term f{arena + sd.src + first, pass ? stats[sd.stat] : 0.0, pass != 0};
@@ -332,6 +333,23 @@ void prep_chunk_kernel(const PrepSumDev* __restrict__ sums, const int* __restric
     for (int l = threadIdx.x; l < s_nl; l += ENM_WAVE) s_leaf[l] = pw_leaf(f, s_lo[l], s_len[l]);
     __syncthreads();
     if (threadIdx.x == 0) chunk_val[c] = pw_combine(m, s_leaf);
+}
+
+// Full chunks only, one per wave, four waves per block (no barriers: each
+// wave sums its own chunk).
+__global__ __launch_bounds__(4 * ENM_WAVE)
+void prep_full_chunk_kernel(const PrepSumDev* __restrict__ sums, const int* __restrict__ chunk_sum_of,
+                            const int* __restrict__ chunk_index, const int* __restrict__ list, int n_list,
+                            const double* __restrict__ arena, const double* __restrict__ stats, int pass,
+                            double* __restrict__ chunk_val) {
+    const int idx = (int)blockIdx.x * 4 + (int)(threadIdx.x / ENM_WAVE);
+    if (idx >= n_list) return;
+    const int c = list[idx];
+    const PrepSumDev& sd = sums[chunk_sum_of[c]];
+    const int64_t first = (int64_t)chunk_index[c] * PW_CHUNK;
+    const Term f{arena + sd.src + first, pass ? stats[sd.stat] : 0.0, pass != 0};
+    const double sum = pw_full_chunk(f);
+    if ((threadIdx.x & (ENM_WAVE - 1)) == 0) chunk_val[c] = sum;
 }
 
 // Lane k's copy of v, in every lane (k uniform over the wave).
@@ -468,8 +486,8 @@ hipError_t enm_launch_prep_count(const PrepTaskDev* tasks, int64_t stride, int n
 hipError_t enm_launch_prep_write(PrepTaskDev* tasks, int64_t stride, int n_tasks,
                                  const double* raw, const unsigned char* mask, int drop_nan, int rescale,
                                  const int64_t* blk_off, const PrepSumDev* sums, int n_sums,
-                                 const int* chunk_sum_of, const int* chunk_index, int n_chunks,
-                                 double* chunk_val, double* stats, int64_t sstride,
+                                 const int* chunk_sum_of, const int* chunk_index, const int* chunk_list,
+                                 int n_chunks, int n_full, double* chunk_val, double* stats, int64_t sstride,
                                  const double* noise, unsigned* const_flag, double* arena, KeptWord* kept,
                                  hipStream_t st) {
     if (n_tasks <= 0) return hipSuccess;
@@ -479,9 +497,17 @@ hipError_t enm_launch_prep_write(PrepTaskDev* tasks, int64_t stride, int n_tasks
                            raw, mask, drop_nan, blk_off, arena, kept);
     if (!rescale) return hipGetLastError();
     for (int pass = 0; pass < 2; ++pass) {
-        if (n_chunks > 0)
-            hipLaunchKernelGGL(prep_chunk_kernel, dim3((unsigned)n_chunks), dim3(ENM_WAVE), 0, st, sums,
-                               chunk_sum_of, chunk_index, n_chunks, arena, stats, pass, chunk_val);
+        if (n_chunks > 0) {
+            // Full chunks four to a block; the other (last) chunks one each.
+            if (n_full > 0)
+                hipLaunchKernelGGL(prep_full_chunk_kernel, dim3((unsigned)((n_full + 3) / 4)), dim3(4 * ENM_WAVE), 0,
+                                   st, sums, chunk_sum_of, chunk_index, chunk_list, n_full, arena, stats, pass,
+                                   chunk_val);
+            if (n_chunks > n_full)
+                hipLaunchKernelGGL(prep_chunk_kernel, dim3((unsigned)(n_chunks - n_full)), dim3(ENM_WAVE), 0, st,
+                                   sums, chunk_sum_of, chunk_index, chunk_list + n_full, n_chunks - n_full, arena,
+                                   stats, pass, chunk_val);
+        }
         hipLaunchKernelGGL(prep_finish_kernel, dim3((unsigned)n_sums), dim3(ENM_WAVE), 0, st, sums, n_sums, arena,
                            chunk_val, pass, stats);
     }

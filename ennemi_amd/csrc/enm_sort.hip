@@ -498,13 +498,14 @@ __device__ __forceinline__ int64_t ss_sample_pos(int i, int ns, int64_t len) {
 }
 
 // One block per segment: copy its samples (key, point index) out.  Block 0
-// also clears the run-queue counters (appends, then pops: 2 * ENM_NQ, each
+// also clears the run-queue counters (appends, then pops: 2 * ENM_NQ, then
+// the long runs' count: each
 // on its own line) that ss_scan_kernel and ss_tile_kernel use.
 __global__ __launch_bounds__(ENM_SORT_THREADS)
 void ss_sample_kernel(const SortSeg* __restrict__ segs, const SampleSeg* __restrict__ ss,
                       const double* __restrict__ arena, double* __restrict__ sk, int* __restrict__ si,
                       unsigned* __restrict__ qctl) {
-    if (blockIdx.x == 0 && threadIdx.x < 2 * ENM_NQ) qctl[threadIdx.x * ENM_CSTRIDE] = 0u;
+    if (blockIdx.x == 0 && threadIdx.x <= 2 * ENM_NQ) qctl[threadIdx.x * ENM_CSTRIDE] = 0u;
     const SortSeg sg = segs[blockIdx.x];
     const SampleSeg q = ss[blockIdx.x];
     for (int i = threadIdx.x; i < q.ns; i += ENM_SORT_THREADS) {
@@ -567,7 +568,7 @@ void ss_count_kernel(const SortSeg* __restrict__ segs, int nsegs, const SampleSe
 __global__ __launch_bounds__(ENM_SORT_THREADS)
 void ss_scan_kernel(const SortSeg* __restrict__ segs, const SampleSeg* __restrict__ ss,
                     int* __restrict__ counts, SsTile* __restrict__ tiles, const int* __restrict__ qbase,
-                    unsigned* __restrict__ qctl, const int* __restrict__ hint_bad) {
+                    unsigned* __restrict__ qctl, SsTile* __restrict__ long_runs, const int* __restrict__ hint_bad) {
     __shared__ int tot[ENM_SS_MAXB];
     __shared__ int start[ENM_SS_MAXB];
     __shared__ int part[ENM_SORT_THREADS / ENM_WAVE];
@@ -594,26 +595,33 @@ void ss_scan_kernel(const SortSeg* __restrict__ segs, const SampleSeg* __restric
         }
     }
     if (threadIdx.x == 0) {
+        // Runs of one bucket larger than an LDS run (sampling bad luck) go
+        // to the long-run list for ss_long_kernel; the others to the queue.
         int nt = 0, cur = 0;
         for (int b = 0; b < nb; ++b) {
             const int z = tot[b];
-            if (cur > 0 && cur + z > ENM_SS_TILE) { ++nt; cur = 0; }
+            if (cur > 0 && cur + z > ENM_SS_TILE) { nt += cur <= ENM_SS_TILE; cur = 0; }
             cur += z;
         }
-        if (cur > 0) ++nt;
+        if (cur > 0) nt += cur <= ENM_SS_TILE;
         SsTile* out = tiles + qbase[q.q] + atomicAdd(qctl + q.q * ENM_CSTRIDE, (unsigned)nt);
         int k = 0, t0 = 0;
         cur = 0;
+        auto emit = [&](int len) {
+            const SsTile tl{sg.sbase + t0, len, (int)blockIdx.x};
+            if (len <= ENM_SS_TILE) out[k++] = tl;
+            else long_runs[atomicAdd(qctl + 2 * ENM_NQ * ENM_CSTRIDE, 1u)] = tl;
+        };
         for (int b = 0; b < nb; ++b) {
             const int z = tot[b];
             if (cur > 0 && cur + z > ENM_SS_TILE) {
-                out[k++] = SsTile{sg.sbase + t0, cur, (int)blockIdx.x};
+                emit(cur);
                 t0 += cur;
                 cur = 0;
             }
             cur += z;
         }
-        if (cur > 0) out[k] = SsTile{sg.sbase + t0, cur, (int)blockIdx.x};
+        if (cur > 0) emit(cur);
     }
 }
 
@@ -902,7 +910,6 @@ void ss_tile_kernel(const SsTile* __restrict__ tiles, const int* __restrict__ qb
                     double* k1, int* i1, double* k0, int* i0, SsLayout L) {
     __shared__ double sk[ENM_LDS_PAD(ENM_SS_TILE)];
     __shared__ int si[ENM_LDS_PAD(ENM_SS_TILE)];
-    __shared__ int sp[2];
     __shared__ unsigned s_ticket;
     __shared__ double red[ENM_SS_THREADS / ENM_WAVE];
     const int home = (int)(blockIdx.x % ENM_NQ);
@@ -917,11 +924,6 @@ void ss_tile_kernel(const SsTile* __restrict__ tiles, const int* __restrict__ qb
             __syncthreads();
             if (t >= qn) break;
             const SsTile tl = tiles[qbase[qq] + (int)t];
-            if (tl.len > ENM_SS_TILE) {
-                ss_sort_long(sk, si, sp, tl.pos, tl.len, k1, i1, k0, i0);
-                ss_layout_run<false>(L, tl, k0 + tl.pos, i0 + tl.pos, red);
-                continue;
-            }
             for (int u = threadIdx.x; u < ENM_SS_TILE; u += ENM_SS_THREADS) {
                 if (u < tl.len) {
                     sk[lp(u)] = k1[tl.pos + u];
@@ -934,6 +936,25 @@ void ss_tile_kernel(const SsTile* __restrict__ tiles, const int* __restrict__ qb
             block_sort_lds<ENM_SS_TILE, ENM_SS_THREADS>(sk, si);
             ss_layout_run<true>(L, tl, sk, si, red);
         }
+    }
+}
+
+// Runs of one bucket larger than an LDS run (rare: sampling bad luck),
+// listed by ss_scan_kernel: each merge-sorted by one block through global
+// memory (ss_sort_long) and laid out into the orders.  Kept out of
+// ss_tile_kernel, whose registers then hold its LDS runs without spilling.
+__global__ __launch_bounds__(ENM_SS_THREADS)
+void ss_long_kernel(const SsTile* __restrict__ runs, const unsigned* __restrict__ qctl, double* k1, int* i1,
+                    double* k0, int* i0, SsLayout L) {
+    __shared__ double sk[ENM_LDS_PAD(ENM_SS_TILE)];
+    __shared__ int si[ENM_LDS_PAD(ENM_SS_TILE)];
+    __shared__ int sp[2];
+    __shared__ double red[ENM_SS_THREADS / ENM_WAVE];
+    const unsigned n = qctl[2 * ENM_NQ * ENM_CSTRIDE];   // final: ss_scan_kernel has finished
+    for (unsigned t = blockIdx.x; t < n; t += gridDim.x) {
+        const SsTile tl = runs[t];
+        ss_sort_long(sk, si, sp, tl.pos, tl.len, k1, i1, k0, i0);
+        ss_layout_run<false>(L, tl, k0 + tl.pos, i0 + tl.pos, red);
     }
 }
 
@@ -1035,11 +1056,14 @@ hipError_t enm_launch_sample_sort(const SampleSortArgs& a, const double* arena, 
     hipLaunchKernelGGL(ss_count_kernel, dim3(a.ntiles), dim3(ENM_SORT_THREADS), 0, st,
                        a.segs, a.nsegs, a.ss, arena, spk, spi, a.bid, a.counts, hint_bad);
     hipLaunchKernelGGL(ss_scan_kernel, dim3(a.nsegs), dim3(ENM_SORT_THREADS), 0, st,
-                       a.segs, a.ss, a.counts, a.tiles, a.qbase, a.qctl, hint_bad);
+                       a.segs, a.ss, a.counts, a.tiles, a.qbase, a.qctl, a.long_runs, hint_bad);
     hipLaunchKernelGGL(ss_scatter_kernel, dim3(a.ntiles), dim3(ENM_SORT_THREADS), 0, st,
                        a.segs, a.nsegs, a.ss, arena, a.bid, a.counts, k1, i1, hint_bad);
     hipLaunchKernelGGL(ss_tile_kernel, dim3(a.tile_grid), dim3(ENM_SS_THREADS), 0, st,
                        a.tiles, a.qbase, a.qctl, k1, i1, k0, i0, L);
+    hipLaunchKernelGGL(ss_long_kernel, dim3(a.long_grid), dim3(ENM_SS_THREADS), 0, st,
+                       a.long_runs, a.qctl, k1, i1, k0, i0, L);
+    ++*n_launch;
     *n_launch += 4;
     return hipGetLastError();
 }
