@@ -218,7 +218,8 @@ struct enm_ctx {
         int64_t noise_len = 0;
         uint32_t flags = 0;
         std::vector<char> blob3;
-        DevBuf kept, tables3, hkeys, hk0, hk1, hi0, hi1, hsplit, hcont, hbad, hmax;
+        DevBuf kept, tables3, hkeys, hk0, hk1, hi0, hi1, hsplit, hcont, hbad, hmax, sg_cnt, sg_last, sg_off,
+            sg_cin, sg_chas;
     } prep;
     // Workspace of enm_unique_rows.
     struct Unique {
@@ -1723,7 +1724,7 @@ int enm_prep_hints(enm_ctx* ctx, const enm_prep_task* tasks, int32_t n_tasks, co
     // The columns, by column slot and then by task (consecutive filter
     // blocks, one XCD's share, read one series).
     std::vector<HintCol> cols;
-    int64_t max_w = 0, max_n = 0, cont_words = 0;
+    int64_t max_w = 0, max_n = 0, cont_words = 0, max_len = 0;
     for (int c = 0; c < max_cols; ++c) {
         for (int t = 0; t < n_tasks; ++t) {
             const PrepTaskDev& D = S.tasks[t];
@@ -1746,6 +1747,7 @@ int enm_prep_hints(enm_ctx* ctx, const enm_prep_task* tasks, int32_t n_tasks, co
             cont_words += (C.n + 31) / 32;
             max_w = std::max<int64_t>(max_w, D.w);
             max_n = std::max<int64_t>(max_n, C.n);
+            max_len = std::max<int64_t>(max_len, H.len);
             cols.push_back(C);
         }
     }
@@ -1754,11 +1756,16 @@ int enm_prep_hints(enm_ctx* ctx, const enm_prep_task* tasks, int32_t n_tasks, co
     const size_t o_seg = append(S.blob3, segs);
     const size_t o_col = append(S.blob3, cols);
     const size_t ncols = cols.size();
+    const int target_blocks = 8 * ctx->num_cus;   // resident filter workgroups
+    const size_t nsg = (size_t)hint_segments((int)ncols, max_len, target_blocks) * ncols + 1;
     if (S.tables3.ensure(S.blob3.size()) != hipSuccess || S.hkeys.ensure((size_t)nkeys * 8 + 8) != hipSuccess ||
         S.hk0.ensure((size_t)nkeys * 8 + 8) != hipSuccess || S.hk1.ensure((size_t)nkeys * 8 + 8) != hipSuccess ||
         S.hi0.ensure((size_t)nkeys * 4 + 4) != hipSuccess || S.hi1.ensure((size_t)nkeys * 4 + 4) != hipSuccess ||
         S.hsplit.ensure((size_t)ntiles * 4 + 4) != hipSuccess || S.hcont.ensure((size_t)cont_words * 4 + 4) != hipSuccess ||
-        S.hbad.ensure(ncols * 4 + 4) != hipSuccess || S.hmax.ensure(8) != hipSuccess) {
+        S.hbad.ensure(ncols * 4 + 4) != hipSuccess || S.hmax.ensure(8) != hipSuccess ||
+        S.sg_cnt.ensure(nsg * 4) != hipSuccess || S.sg_last.ensure(nsg * 8) != hipSuccess ||
+        S.sg_off.ensure(nsg * 8) != hipSuccess || S.sg_cin.ensure(nsg * 8) != hipSuccess ||
+        S.sg_chas.ensure(nsg * 4) != hipSuccess) {
         (void)hipGetLastError();
         return fail(ctx, ENM_E_NOMEM, "hipMalloc failed for the sort-hint workspace");
     }
@@ -1778,10 +1785,13 @@ int enm_prep_hints(enm_ctx* ctx, const enm_prep_task* tasks, int32_t n_tasks, co
     const int* sidx = fb ? S.hi1.as<int>() : S.hi0.as<int>();
     const double* skey = fb ? S.hk1.as<double>() : S.hk0.as<double>();
     const PrepTaskDev* dtasks = reinterpret_cast<const PrepTaskDev*>(S.tables2.as<char>() + S.o_tasks);
-    e = enm_launch_hint_order(dcol, (int)ncols, max_w, max_n, sidx, skey, S.kept.as<KeptWord>(), dtasks,
+    const HintSegs hsg{S.sg_cnt.as<int>(), S.sg_last.as<double>(), S.sg_off.as<int64_t>(), S.sg_cin.as<double>(),
+                       S.sg_chas.as<int>()};
+    e = enm_launch_hint_order(dcol, (int)ncols, max_w, max_n, max_len, target_blocks, sidx, skey,
+                              S.kept.as<KeptWord>(), dtasks,
                               S.stats.as<double>(), (flags & ENM_P_RESCALE) ? 1 : 0,
                               (flags & ENM_P_RESCALE) ? d_noise : nullptr, noise_len, S.hmax.as<double>(),
-                              S.hcont.as<unsigned>(), cont_words, d_arena, d_hint, S.hbad.as<int>(), st);
+                              S.hcont.as<unsigned>(), cont_words, hsg, d_arena, d_hint, S.hbad.as<int>(), st);
     if (e != hipSuccess) return hip_fail(ctx, e, "hint order");
     return ENM_OK;
 }

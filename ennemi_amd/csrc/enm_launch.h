@@ -99,13 +99,27 @@ hipError_t enm_launch_prep_write(PrepTaskDev* tasks, int64_t stride, int n_tasks
 // Sort hints (enm_hint.hip).
 hipError_t enm_launch_hint_keys(const HintSeries* series, int n_series, int64_t maxlen, const double* raw,
                                 double* keys, hipStream_t st);
-// The hint of every column: noise maximum, filter (one block per column),
-// runs re-sorted, columns without a hint marked.
-hipError_t enm_launch_hint_order(const HintCol* cols, int n_cols, int64_t max_w, int64_t max_n,
+// Per (column, segment of ENM_HINT_SEG sorted rows) of the hint filter.
+struct HintSegs {
+    int* cnt;        // kept rows of the segment
+    double* last;    // value before the noise of its last kept row
+    int64_t* off;    // its first output position
+    double* cin;     // value before the noise of the last kept row before it
+    int* chas;       // whether there is one
+};
+// The hint of every column: noise maximum, segmented filter (count, scan,
+// filter), runs re-sorted, columns without a hint marked.
+// Filter segments per column for n_cols columns of at most max_len sorted
+// rows and a GPU holding target_blocks workgroups at once, and the rows of
+// a segment (whole chunks).
+int hint_segments(int n_cols, int64_t max_len, int target_blocks);
+int64_t hint_segment_rows(int64_t max_len, int segs_per_col);
+hipError_t enm_launch_hint_order(const HintCol* cols, int n_cols, int64_t max_w, int64_t max_n, int64_t max_len,
+                                 int target_blocks,
                                  const int* sidx, const double* skey, const KeptWord* kept,
                                  const PrepTaskDev* tasks, const double* stats, int rescale, const double* noise,
                                  int64_t noise_len, double* noise_max, unsigned* cont, int64_t cont_words,
-                                 const double* arena, int* hint, int* bad, hipStream_t st);
+                                 const HintSegs& sg, const double* arena, int* hint, int* bad, hipStream_t st);
 
 hipError_t enm_launch_uq_codes(const UqSeg* segs, int nsegs, const double* skeys, const int* sidx, int* codes,
                                int64_t* starts, int64_t* counts, int64_t* distinct, hipStream_t st);
