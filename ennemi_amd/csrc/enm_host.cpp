@@ -190,6 +190,10 @@ struct enm_ctx {
     // (joint dimension class, list class, floor(log2 n)).  Plans are cached
     // by tuning_gen, which every change bumps.
     enm_tuning tuning{1024, 32, 0, 0};
+    // Until a tuning is set, instances of three or more joint columns take
+    // wide strips: their grid sweeps scan the union of a wave's windows,
+    // which narrows as strips grow.
+    bool wide_multid = true;
     std::map<std::tuple<int, int, int>, enm_tuning> tuned;
     uint64_t tuning_gen = 0;
     std::list<std::unique_ptr<Plan>> cache;   // most recent first
@@ -441,7 +445,10 @@ std::tuple<int, int, int> shape_key(int dims, int k, int64_t n) {
 // The tile shape the planner uses for an instance of `dims` joint columns.
 enm_tuning tuning_for(const enm_ctx* ctx, int dims, int k, int64_t n) {
     auto it = ctx->tuned.find(shape_key(dims, k, n));
-    return it != ctx->tuned.end() ? it->second : ctx->tuning;
+    if (it != ctx->tuned.end()) return it->second;
+    enm_tuning t = ctx->tuning;
+    if (ctx->wide_multid && dims >= 3 && t.strip_points > 0) t.strip_points = ENM_STRIP_WIDE;
+    return t;
 }
 
 int plan_batch(enm_ctx* ctx, Plan& P, const enm_instance* inst, int n_inst, const int64_t* seg_len) {
@@ -2026,6 +2033,7 @@ int enm_set_tuning_ex(enm_ctx* ctx, const enm_tuning* tuning) {
     int rc = check_tuning(ctx, *tuning);
     if (rc) return rc;
     ctx->tuning = *tuning;
+    ctx->wide_multid = false;
     ctx->tuned.clear();
     ++ctx->tuning_gen;
     return ENM_OK;
@@ -2058,6 +2066,8 @@ int enm_autotune(enm_ctx* ctx, const enm_instance* inst, int32_t n_inst, const i
         shapes.insert(shape_key(inst[i].dx + inst[i].dy + inst[i].dz, inst[i].k, inst[i].n));
     for (const auto& sk : shapes) ctx->tuned.erase(sk);
     const enm_tuning saved = ctx->tuning;
+    const bool saved_wide = ctx->wide_multid;
+    ctx->wide_multid = false;
     const uint32_t fl = flags & ~(uint32_t)(ENM_F_GRAPH | ENM_F_TIMING);
     float best_ms = 1e30f;
     enm_tuning best = saved;
@@ -2080,7 +2090,7 @@ int enm_autotune(enm_ctx* ctx, const enm_instance* inst, int32_t n_inst, const i
     };
     // Coordinate descent: strip, then waves per CU, then list slots, then
     // items per ticket, each swept with the best of the ones before.
-    const int strips[] = {0, 256, 512, 1024, 2048, 4096};
+    const int strips[] = {0, 256, 512, 1024, 2048, 4096, 8192};
     const int waves[] = {8, 16, 32};
     const int slots[] = {0, 8, 16};
     const int tickets[] = {0, 1, 2, 4};
@@ -2097,6 +2107,7 @@ int enm_autotune(enm_ctx* ctx, const enm_instance* inst, int32_t n_inst, const i
     // The winner is kept for each of the batch's shapes; other shapes keep
     // the default.
     ctx->tuning = saved;
+    ctx->wide_multid = saved_wide;
     if (!rc)
         for (const auto& sk : shapes) ctx->tuned[sk] = best;
     ++ctx->tuning_gen;

@@ -45,6 +45,8 @@
 // Two-column kNN: strips up to this size are walked a strip per workgroup
 // with the strip and its two neighbours staged in LDS (3 x 1024 x 16 B).
 #define ENM_STRIP_LDS_MAX 1024
+// Default strip of instances of three or more joint columns.
+#define ENM_STRIP_WIDE 4096
 
 // One contiguous key range to sort: keys at arena[src + t], t in [0, len),
 // carried with their instance point index p0 + t.

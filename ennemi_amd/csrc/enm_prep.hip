@@ -352,63 +352,83 @@ void prep_full_chunk_kernel(const PrepSumDev* __restrict__ sums, const int* __re
     if ((threadIdx.x & (ENM_WAVE - 1)) == 0) chunk_val[c] = sum;
 }
 
-// Lane k's copy of v, in every lane (k uniform over the wave).
-__device__ __forceinline__ double uni_lane(double v, int k) {
-    const unsigned long long b = (unsigned long long)__double_as_longlong(v);
-    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)b, k);
-    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(b >> 32), k);
-    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
-}
+// Rows per LDS tile of a row-by-row sum, and rows each lane loads per tile.
+constexpr int SEQ_TILE = 512;
+constexpr int SEQ_PER = SEQ_TILE / ENM_WAVE;
 
 // Phase 2c: per array, the chunk sums added in order to -0.0 (pairwise
 // arrays), or the whole column accumulated row by row (columns of a
 // multi-column condition); then mean = sum / n (pass 0) or
 // std = sqrt(sum / n) (pass 1).  One wave per array.  The row-by-row sum is
-// one chain of dependent additions, so the wave loads 64 values at a time
-// (the next 64 in flight meanwhile) and every lane adds them in row order,
-// read lane by lane (v_readlane): the same additions, in the same order, at
-// the latency of an addition rather than of a load (182 -> 25 ms for two
-// million rows).
+// one chain of dependent additions.  The wave loads 512 rows at a time
+// (coalesced, the next 512 in flight meanwhile), forms the terms, stages
+// them in LDS and every lane adds them in row order from broadcast LDS
+// reads issued a batch ahead: the same additions, in the same order, with
+// nothing but the additions on the chain.  The sum starts at -0.0, which
+// leaves the first row's term unchanged (x + -0.0 == x for every x), as a
+// sum that starts from the first row does.
 __global__ __launch_bounds__(ENM_WAVE)
 void prep_finish_kernel(const PrepSumDev* __restrict__ sums, int n_sums, const double* __restrict__ arena,
                         const double* __restrict__ chunk_val, int pass, double* __restrict__ stats) {
+    __shared__ double2 s_rows[SEQ_TILE / 2];
     const int s = blockIdx.x;
     if (s >= n_sums) return;
     const PrepSumDev& sd = sums[s];
     const int lane = threadIdx.x;
-    double acc;
+    double acc = -0.0;
     if (sd.sequential) {
         const double* v = arena + sd.src;
+        const int64_t n = sd.n;
         const double mean = pass ? stats[sd.stat] : 0.0;
         auto term = [&](int64_t i) {
-            double t = i < sd.n ? v[i] : 0.0;
+            double t = i < n ? v[i] : 0.0;
             if (pass) {
                 t = t - mean;
                 t = t * t;
             }
             return t;
         };
-        double cur = term(lane);
-        acc = uni_lane(cur, 0);   // the first row starts the sum
-        int k0 = 1;
-        int64_t base = 0;
-        // Whole groups of 64 rows: the additions unrolled, lane indices
-        // constant, so only the chain of additions is serial.
-        for (; base + ENM_WAVE <= sd.n; base += ENM_WAVE) {
-            const double next = term(base + ENM_WAVE + lane);
-            if (k0 == 0) {
+        double cur[SEQ_PER];
 #pragma unroll
-                for (int k = 0; k < ENM_WAVE; ++k) acc = acc + uni_lane(cur, k);
+        for (int j = 0; j < SEQ_PER; ++j) cur[j] = term(j * ENM_WAVE + lane);
+        double* rows = reinterpret_cast<double*>(s_rows);
+        for (int64_t base = 0; base < n; base += SEQ_TILE) {
+            double nxt[SEQ_PER];
+#pragma unroll
+            for (int j = 0; j < SEQ_PER; ++j) nxt[j] = term(base + SEQ_TILE + j * ENM_WAVE + lane);
+#pragma unroll
+            for (int j = 0; j < SEQ_PER; ++j) rows[j * ENM_WAVE + lane] = cur[j];
+            __syncthreads();
+            if (n - base >= SEQ_TILE) {
+                // Sixteen rows per batch; the next batch's reads are issued
+                // before this batch's additions.
+                double2 w[8];
+#pragma unroll
+                for (int q = 0; q < 8; ++q) w[q] = s_rows[q];
+#pragma unroll
+                for (int r = 0; r < SEQ_TILE / 2; r += 8) {
+                    double2 u[8];
+                    if (r + 8 < SEQ_TILE / 2) {
+#pragma unroll
+                        for (int q = 0; q < 8; ++q) u[q] = s_rows[r + 8 + q];
+                    }
+#pragma unroll
+                    for (int q = 0; q < 8; ++q) {
+                        acc = acc + w[q].x;
+                        acc = acc + w[q].y;
+                    }
+#pragma unroll
+                    for (int q = 0; q < 8; ++q) w[q] = u[q];
+                }
             } else {
-#pragma unroll
-                for (int k = 1; k < ENM_WAVE; ++k) acc = acc + uni_lane(cur, k);
-                k0 = 0;
+                const int m = (int)(n - base);
+                for (int r = 0; r < m; ++r) acc = acc + rows[r];
             }
-            cur = next;
+            __syncthreads();
+#pragma unroll
+            for (int j = 0; j < SEQ_PER; ++j) cur[j] = nxt[j];
         }
-        for (int k = k0; k < (int)(sd.n - base); ++k) acc = acc + uni_lane(cur, k);
     } else {
-        acc = -0.0;
         for (int c = 0; c < sd.n_chunks; ++c) acc += chunk_val[sd.chunk0 + c];
     }
     if (lane != 0) return;

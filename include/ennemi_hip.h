@@ -334,8 +334,10 @@ int enm_get_stats(enm_ctx* ctx, enm_stats* out);
  *                     that every wave of the grid gets work.
  * The planner takes each instance's shape from the context: the shape
  * enm_autotune kept for its (joint dimension class, k+1 class,
- * floor(log2 n)) if there is one, else the default.  enm_autotune sweeps
- * strips {0, 256, ..., 4096}, then waves {8, 16, 32}, then list slots
+ * floor(log2 n)) if there is one, else the default; until enm_set_tuning(_ex)
+ * is first called, the default gives instances of three or more joint
+ * columns strips of 4096 points.  enm_autotune sweeps
+ * strips {0, 256, ..., 8192}, then waves {8, 16, 32}, then list slots
  * {0, 8, 16}, then items per ticket {0, 1, 2, 4}, each with the best of the
  * ones before (two enm_run_batch calls per shape, the second timed), and
  * keeps the winner for each shape of the batch (the default is unchanged).
