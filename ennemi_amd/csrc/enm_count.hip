@@ -18,7 +18,9 @@
 // Multi-D spaces: on a grid order (the space's first two columns are the
 // order's columns 0 and 1) a wave takes 64 queries of one strip and scans its
 // own strip and the strips outward, each within the y window of the tile
-// widened by the wave's largest radius, in FP64 (count_grid).  Otherwise the
+// widened by the wave's largest radius, in FP64 (count_grid); with two
+// columns, a strip whose whole key range lies within a lane's radius is
+// counted for that lane by binary searches of its y column.  Otherwise the
 // wave bounds the union of its 64 balls on the key axis and scans that window
 // of the key-sorted order with wave-uniform candidate loads; in FP32-filter
 // mode each lane counts candidates whose FP32 distance is <= the radius
@@ -328,12 +330,40 @@ __device__ __forceinline__ int count_range64(const double* const (&c64)[D], cons
     return cnt;
 }
 
+// Number of positions j in [a, b) of an ascending column with
+// fl64(|y_j - q|) <= r (r >= 0), for one lane: fl64(y - q) is monotone in y,
+// so they are the run from the first j with fl64(y_j - q) >= -r to the first
+// with fl64(y_j - q) > r, found by two binary searches run together (their
+// loads are independent).  [a, b) must hold every such j.
+__device__ __forceinline__ int count_sorted(const double* __restrict__ y, int a, int b, double q, double r) {
+    int l0 = a, n0 = b - a;   // first j with fl(y_j - q) >= -r
+    int l1 = a, n1 = b - a;   // first j with fl(y_j - q) > r
+    while (n0 > 0 || n1 > 0) {
+        const int h0 = n0 >> 1, h1 = n1 >> 1;
+        const double v0 = n0 > 0 ? y[l0 + h0] : 0.0;
+        const double v1 = n1 > 0 ? y[l1 + h1] : 0.0;
+        if (n0 > 0) {
+            if (v0 - q >= -r) n0 = h0;
+            else { l0 += h0 + 1; n0 -= h0 + 1; }
+        }
+        if (n1 > 0) {
+            if (v1 - q > r) n1 = h1;
+            else { l1 += h1 + 1; n1 -= h1 + 1; }
+        }
+    }
+    return l1 - l0;
+}
+
 // Grid count: the tile's own strip and then strips outward, each within the
 // union of the y intervals [y_i - r_i, y_i + r_i] of the lanes that can
 // reach it (strip_window).  A side stops at the first strip whose column-0
 // bound is more than every live lane's radius away (strictly: a count
 // includes distance == r), and all strips beyond it are further still.
-// Returns candidates visited.
+// With two columns, a lane whose ball covers a strip's whole column-0 range
+// (both end keys within r: fl64(x_j - x_i) is monotone in x_j) counts that
+// strip as a run of its y column, by two binary searches inside the window
+// of such lanes, instead of scanning; the other lanes scan theirs.
+// Returns candidates scanned.
 template <int D>
 __device__ int count_grid(const CountDev& cd, const double* __restrict__ col64,
                           const double* const (&c64)[D], const double (&q64)[D], double r, bool live,
@@ -343,13 +373,31 @@ __device__ int count_grid(const CountDev& cd, const double* __restrict__ col64,
     const double* yc = c64[1];
     const int s = (t0 - lo) / S;
     const int nstrip = (hi - lo + S - 1) / S;
-    const int ss = lo + s * S, se = min(hi, ss + S);
     if (!wave_any(live)) return 0;
     int visited = 0;
-    int a, b;
-    strip_window(yc, ss, se, q64[1], r, live, a, b);
-    cnt += count_range64<D>(c64, q64, r, a, b);
-    visited += b - a;
+    // Strip [ks, ke) for the lanes in `need`.
+    auto visit = [&](int ks, int ke, bool need) {
+        int a, b;
+        if constexpr (D == 2) {
+            const bool inner = need && fabs(xk[ks] - q64[0]) <= r && fabs(xk[ke - 1] - q64[0]) <= r;
+            const bool edge = need && !inner;
+            if (wave_any(inner)) {
+                strip_window(yc, ks, ke, q64[1], r, inner, a, b);
+                if (inner) cnt += count_sorted(yc, a, b, q64[1], r);
+            }
+            if (wave_any(edge)) {
+                strip_window(yc, ks, ke, q64[1], r, edge, a, b);
+                const int c = count_range64<D>(c64, q64, r, a, b);
+                if (!inner) cnt += c;   // lanes outside `need` find no candidate here
+                visited += b - a;
+            }
+        } else {
+            strip_window(yc, ks, ke, q64[1], r, need, a, b);
+            cnt += count_range64<D>(c64, q64, r, a, b);
+            visited += b - a;
+        }
+    };
+    visit(lo + s * S, min(hi, lo + s * S + S), live);
     bool ldone = s == 0, rdone = s == nstrip - 1;
     for (int d = 1; !(ldone && rdone); ++d) {
         if (!ldone) {
@@ -361,11 +409,7 @@ __device__ int count_grid(const CountDev& cd, const double* __restrict__ col64,
                 const double xb = xk[le - 1];
                 const bool need = live && !(q64[0] - xb > r);
                 ldone = !wave_any(need);
-                if (!ldone) {
-                    strip_window(yc, le - S, le, q64[1], r, need, a, b);
-                    cnt += count_range64<D>(c64, q64, r, a, b);
-                    visited += b - a;
-                }
+                if (!ldone) visit(le - S, le, need);
             }
         }
         if (!rdone) {
@@ -374,15 +418,10 @@ __device__ int count_grid(const CountDev& cd, const double* __restrict__ col64,
                 rdone = true;
             } else {
                 const int rs = lo + sr * S;
-                const int re = min(hi, rs + S);
                 const double xb = xk[rs];
                 const bool need = live && !(xb - q64[0] > r);
                 rdone = !wave_any(need);
-                if (!rdone) {
-                    strip_window(yc, rs, re, q64[1], r, need, a, b);
-                    cnt += count_range64<D>(c64, q64, r, a, b);
-                    visited += b - a;
-                }
+                if (!rdone) visit(rs, min(hi, rs + S), need);
             }
         }
     }

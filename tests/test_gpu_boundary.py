@@ -95,3 +95,64 @@ def test_boundary_multid_counts(strip):
             assert abs(vals[0] - w) <= 1e-12 or (np.isinf(w) and vals[0] == w)
     finally:
         ctx.set_tuning(*old)
+
+
+
+def boundary_groups_on_y(m: int, seed: int, layout: str):
+    """(x0, x1, y) of 4 m points for a KSG with a two-column x, with
+    candidates exactly at the count radius on x1, the grid's second column.
+    "ties": radius E = 2^20 (fl(E - 1e-12) = E), per group a query at
+    x1 = 4 E g and candidates at x1 +- E; every x0 is 0, so every strip
+    lies within every query's radius on x0.  "blocks": radius 1, count
+    radius r = fl(1 - 1e-12), as in boundary_groups on x1, with sub-groups
+    4 apart on y; blocks of 16 sub-groups (64 points) share x0 = 4 b, so a
+    strip of 64 points is one block (within every radius on x0) and wider
+    strips hold several (not within)."""
+    rng = np.random.default_rng(seed)
+    rows = []
+    E = 2.0 ** 20
+    for g in range(m):
+        if layout == "ties":
+            base = 4.0 * E * g
+            rows += [(0.0, base, 0.0), (0.0, base + E, 0.0), (0.0, base - E, 0.0), (0.0, base, 0.5 * E)]
+        else:
+            u, yg = 4.0 * (g // 16), 4.0 * (g % 16)
+            if rng.random() < 0.5:
+                q, c, far = 1.0, 1.0 - R, 0.0
+            else:
+                q, c, far = 0.0, R, 1.0
+            rows += [(u, q, yg), (u, c, yg), (u, far, yg), (u, q, yg + 0.5)]
+    pts = np.array(rows)
+    return pts[rng.permutation(len(pts))]
+
+
+@pytest.mark.parametrize("strip", [None, 64, 128])
+@pytest.mark.parametrize("layout", ["ties", "blocks"])
+def test_boundary_2d_grid_counts(strip, layout):
+    """A two-column count space on a grid (the x space of a KSG with a
+    two-column x): strips whose whole key range lies within a query's
+    radius are counted by binary searches of the y column, the others
+    scanned; both must include a candidate at exactly the count radius."""
+    import ennemi_amd._native as nat
+    from ennemi_amd._batch import Batch
+    pts = boundary_groups_on_y(400, 3, layout)
+    x, y = pts[:, :2], pts[:, 2]
+    w, ref = orc.estimate_single_mi(x, y, 3, detail=True)
+    assert _boundary_hits(x, ref["eps"]) >= 400
+    assert (_strict(x, ref["eps"] - 1e-12) != ref["nx"]).any()
+    ctx = nat.context()
+    old = ctx.get_tuning()
+    try:
+        if strip is not None:
+            ctx.set_tuning(strip, old[1])
+        for flags in (0, nat.F_FP64):
+            b = Batch()
+            b.add(nat.KIND_KSG, 3, [np.ascontiguousarray(x[:, 0]), np.ascontiguousarray(x[:, 1])],
+                  [np.ascontiguousarray(y)])
+            vals, eps, cnts = b.run(flags=flags, export=True)
+            np.testing.assert_array_equal(eps[0], ref["eps"])
+            np.testing.assert_array_equal(cnts[0][0], ref["nx"], err_msg=f"nx flags={flags}")
+            np.testing.assert_array_equal(cnts[0][1], ref["ny"], err_msg=f"ny flags={flags}")
+            assert abs(vals[0] - w) <= 1e-12
+    finally:
+        ctx.set_tuning(*old)

@@ -394,7 +394,7 @@ def test_sample_sorted_modes_graphs_and_exports_agree():
 # Tuning invariance: every shape the autotuner may pick gives the same bits
 # ---------------------------------------------------------------------------
 
-AUTOTUNE_STRIPS = [0, 256, 512, 1024, 2048, 4096]
+AUTOTUNE_STRIPS = [0, 256, 512, 1024, 2048, 4096, 8192]
 AUTOTUNE_WAVES = [8, 16, 32]
 
 
@@ -487,6 +487,38 @@ def test_autotune_keeps_the_winner_per_shape():
             np.testing.assert_array_equal(got[2][i], ref[2][i])
     finally:
         ctx.set_tuning(*old)
+
+
+def test_default_strips_widen_for_three_or_more_columns():
+    """A fresh context gives instances of three or more joint columns strips
+    of 4096 points and two-column ones 1024 until a tuning is set; results
+    are those of one strip size for all."""
+    nat = _nat()
+    fresh = nat.Context(0)
+    try:
+        assert fresh.tuning_for(2, 3, 100_000) == (1024, 32, 0, 0)
+        assert fresh.tuning_for(3, 3, 100_000) == (4096, 32, 0, 0)
+        assert fresh.tuning_for(4, 6, 2_000_000) == (4096, 32, 0, 0)
+        b = _mixed_tuning_batch()
+        shared = nat.context()
+        old = shared.get_tuning()
+        try:
+            shared.set_tuning(1024, 32)
+            ref = b.run(export=True)
+        finally:
+            shared.set_tuning(*old)
+        db = b.device(0, export=True)
+        db.ctx = fresh   # the same batch, launched through the fresh context
+        db.launch()
+        got = db.results()
+        np.testing.assert_array_equal(got[0], ref[0])
+        for i in range(len(got[0])):
+            np.testing.assert_array_equal(got[1][i], ref[1][i])
+            np.testing.assert_array_equal(got[2][i], ref[2][i])
+        fresh.set_tuning(1024, 32)
+        assert fresh.tuning_for(3, 3, 100_000) == (1024, 32, 0, 0)
+    finally:
+        fresh.lib.enm_shutdown(fresh.ctx)
 
 
 def test_plans_are_reused_across_arenas_of_one_shape():
