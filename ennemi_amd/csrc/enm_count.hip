@@ -330,6 +330,50 @@ __device__ __forceinline__ int count_range64(const double* const (&c64)[D], cons
     return cnt;
 }
 
+// count_range64 for three or more columns, candidates through LDS: the wave
+// loads 64 candidates at a time, one per lane (coalesced per column, the
+// next 64 in flight meanwhile), into `stage` (D x 64 doubles), and every
+// lane reads them back with broadcast LDS reads.  [a, b) is wave-uniform.
+template <int D>
+__device__ __forceinline__ int count_range_staged(const double* const (&c64)[D], const double (&q64)[D],
+                                                  double r, int a, int b, double* __restrict__ stage) {
+    const int lane = (int)(threadIdx.x & (ENM_WAVE - 1));
+    int cnt = 0;
+    double nx[D];
+    auto fetch = [&](int j0) {
+        const int p = j0 + lane;
+        const bool ok = p < b;
+#pragma unroll
+        for (int d = 0; d < D; ++d) nx[d] = ok ? c64[d][p] : 0.0;
+    };
+    if (a < b) fetch(a);
+    for (int j = a; j < b; j += ENM_WAVE) {
+        __syncthreads();   // every lane has read the previous block
+#pragma unroll
+        for (int d = 0; d < D; ++d) stage[d * ENM_WAVE + lane] = nx[d];
+        __syncthreads();
+        const int m = min(ENM_WAVE, b - j);
+        if (j + ENM_WAVE < b) fetch(j + ENM_WAVE);
+        int c = 0;
+        for (; c + CH <= m; c += CH) {
+#pragma unroll
+            for (int u = 0; u < CH; ++u) {
+                double e = fabs(stage[c + u] - q64[0]);
+#pragma unroll
+                for (int d = 1; d < D; ++d) e = fmax(e, fabs(stage[d * ENM_WAVE + c + u] - q64[d]));
+                cnt += (e <= r);
+            }
+        }
+        for (; c < m; ++c) {
+            double e = fabs(stage[c] - q64[0]);
+#pragma unroll
+            for (int d = 1; d < D; ++d) e = fmax(e, fabs(stage[d * ENM_WAVE + c] - q64[d]));
+            cnt += (e <= r);
+        }
+    }
+    return cnt;
+}
+
 // Number of positions j in [a, b) of an ascending column with
 // fl64(|y_j - q|) <= r (r >= 0), for one lane: fl64(y - q) is monotone in y,
 // so they are the run from the first j with fl64(y_j - q) >= -r to the first
@@ -367,7 +411,7 @@ __device__ __forceinline__ int count_sorted(const double* __restrict__ y, int a,
 template <int D>
 __device__ int count_grid(const CountDev& cd, const double* __restrict__ col64,
                           const double* const (&c64)[D], const double (&q64)[D], double r, bool live,
-                          int lo, int hi, int t0, int t1, int& cnt) {
+                          int lo, int hi, int t0, int t1, int& cnt, double* stage) {
     const int S = cd.strip;
     const double* xk = col64 + cd.base_key;
     const double* yc = c64[1];
@@ -393,7 +437,7 @@ __device__ int count_grid(const CountDev& cd, const double* __restrict__ col64,
             }
         } else {
             strip_window(yc, ks, ke, q64[1], r, need, a, b);
-            cnt += count_range64<D>(c64, q64, r, a, b);
+            cnt += count_range_staged<D>(c64, q64, r, a, b, stage);
             visited += b - a;
         }
     };
@@ -432,7 +476,7 @@ template <int D, bool F32>
 __device__ void countnd_item(const CountArgs& A, const Item& it, const CountDev& cd,
                              const double* __restrict__ col64, const float* __restrict__ col32,
                              double* __restrict__ res64, int* __restrict__ res32,
-                             unsigned long long& pairs, unsigned long long& rechecks) {
+                             unsigned long long& pairs, unsigned long long& rechecks, double* stage) {
     const OrderDev od = A.orders[cd.order];
     const int64_t n = od.n;
     const int i = it.t0 + (int)threadIdx.x;
@@ -459,7 +503,7 @@ __device__ void countnd_item(const CountArgs& A, const Item& it, const CountDev&
     if constexpr (D >= 2) {
         if (cd.strip > 0) {
             int cnt = 0;
-            const int visited = count_grid<D>(cd, col64, c64, q64, r, live, it.lo, it.hi, it.t0, it.t1, cnt);
+            const int visited = count_grid<D>(cd, col64, c64, q64, r, live, it.lo, it.hi, it.t0, it.t1, cnt, stage);
             finish_item(cd, it, active, cnt, res32, res64, i, A.perm, A.psi_tab);
             pairs += (unsigned long long)(it.t1 - it.t0) * (unsigned long long)visited;
             return;
@@ -550,6 +594,7 @@ void countnd_kernel(CountArgs A, const double* __restrict__ col64, const float* 
                     unsigned long long* __restrict__ stats) {
     // Counters accumulate per wave and go out once (see knn_kernel).
     unsigned long long pairs = 0, rechecks = 0;
+    __shared__ double stage[D >= 3 ? D * ENM_WAVE : 1];   // grid candidates of three or more columns
     for_each_wave_item(A.qoff, A.counter, A.chunk, [&](int idx) {
         Item it = A.items[idx];
         it.desc = uni(it.desc); it.tile = uni(it.tile);
@@ -558,10 +603,10 @@ void countnd_kernel(CountArgs A, const double* __restrict__ col64, const float* 
         if (F32) {
             bool ok = true;
             (void)enm_guard(A.maxabs[A.orders[cd.order].inst], &ok);
-            if (ok) countnd_item<D, true>(A, it, cd, col64, col32, res64, res32, pairs, rechecks);
-            else    countnd_item<D, false>(A, it, cd, col64, col32, res64, res32, pairs, rechecks);
+            if (ok) countnd_item<D, true>(A, it, cd, col64, col32, res64, res32, pairs, rechecks, stage);
+            else    countnd_item<D, false>(A, it, cd, col64, col32, res64, res32, pairs, rechecks, stage);
         } else {
-            countnd_item<D, false>(A, it, cd, col64, col32, res64, res32, pairs, rechecks);
+            countnd_item<D, false>(A, it, cd, col64, col32, res64, res32, pairs, rechecks, stage);
         }
     });
     if (threadIdx.x == 0) {

@@ -160,6 +160,7 @@ struct KnnTile {
     double guard;
     unsigned rechecks;
     bool dense;   // FP32 mode: evaluating chunks directly in FP64
+    double* stage;   // LDS, 64 candidates x D doubles (three or more columns)
 
     __device__ __forceinline__ void set_thr() {
         if (F32) thr = f32_up(T + guard);
@@ -293,7 +294,59 @@ struct KnnTile {
     // chunk often miss the scalar cache, and their L2 latency then overlaps
     // a chunk of arithmetic instead of stalling the wave); both buffers fit
     // in SGPRs (2 x 2 x 8 doubles).
+    // Three or more columns: candidates come through LDS.  The wave loads
+    // 64 candidates at a time, one per lane (coalesced per column, the next
+    // 64 in flight while these are evaluated), writes them to `stage`, and
+    // every lane reads them back with broadcast LDS reads.  (Scalar loads
+    // of D columns would not fit two chunks in SGPRs, so each chunk would
+    // wait for its loads.)  Distances are exact FP64 in both modes.
+    __device__ __forceinline__ void range_staged(int a, int b) {
+        const int lane = (int)(threadIdx.x & (ENM_WAVE - 1));
+        double nx[D];
+        auto fetch = [&](int j0) {
+            const int p = j0 + lane;
+            const bool ok = p < b;
+#pragma unroll
+            for (int d = 0; d < D; ++d) nx[d] = ok ? c64[d][p] : 0.0;
+        };
+        fetch(a);
+        for (int j = a; j < b; j += ENM_WAVE) {
+            __syncthreads();   // every lane has read the previous block
+#pragma unroll
+            for (int d = 0; d < D; ++d) stage[d * ENM_WAVE + lane] = nx[d];
+            __syncthreads();
+            const int m = min(ENM_WAVE, b - j);
+            if (j + ENM_WAVE < b) fetch(j + ENM_WAVE);
+            int c = 0;
+            for (; c + CH <= m; c += CH) {
+                double dd[CH];
+#pragma unroll
+                for (int u = 0; u < CH; ++u) {
+                    double e = fabs(stage[c + u] - q64[0]);
+#pragma unroll
+                    for (int d = 1; d < D; ++d) e = fmax(e, fabs(stage[d * ENM_WAVE + c + u] - q64[d]));
+                    dd[u] = e;
+                }
+                dense = take(dd);
+            }
+            for (; c < m; ++c) {
+                double e = fabs(stage[c] - q64[0]);
+#pragma unroll
+                for (int d = 1; d < D; ++d) e = fmax(e, fabs(stage[d * ENM_WAVE + c] - q64[d]));
+                if (e < T) {
+                    L.insert(e);
+                    T = L.top();
+                }
+            }
+        }
+        if (F32) set_thr();
+    }
+
     __device__ __forceinline__ void range(int a, int b) {
+        if constexpr (D >= 3) {
+            if (b > a) range_staged(a, b);
+            return;
+        }
         int j = a;
         if constexpr (!F32 && D <= 2) {
             // FP64 with up to two columns: the next chunk's coordinates are
@@ -545,7 +598,7 @@ __device__ int grid_sweep(KnnTile<D, P, F32>& w, const KnnDev& kd, const double*
 template <int D, int P, bool F32>
 __device__ void knn_item(const KnnArgs& A, const Item& it, const double* __restrict__ col64,
                          const float* __restrict__ col32, double* __restrict__ res64,
-                         unsigned long long& pairs, unsigned long long& rechecks) {
+                         unsigned long long& pairs, unsigned long long& rechecks, double* stage) {
     const KnnDev kd = A.descs[it.desc];
     const OrderDev od = A.orders[kd.order];
     const int lane = threadIdx.x;
@@ -573,6 +626,7 @@ __device__ void knn_item(const KnnArgs& A, const Item& it, const double* __restr
     w.thr = active ? INFINITY : -INFINITY;
     w.rechecks = 0;
     w.dense = true;   // a tile starts with its nearest candidates
+    w.stage = stage;
 
     const int lo = it.lo, hi = it.hi, t0 = it.t0, t1 = it.t1;
     int visited = t1 - t0;
@@ -938,6 +992,7 @@ void knn_kernel(KnnArgs A, const double* __restrict__ col64, const float* __rest
     // out once at the end: one atomic per item on one address would queue
     // a million atomics on a single L2 channel.
     unsigned long long pairs = 0, rechecks = 0;
+    __shared__ double stage[D >= 3 ? D * ENM_WAVE : 1];   // candidates of three or more columns
     for_each_wave_item(A.qoff, A.counter, A.chunk, [&](int idx) {
         Item it = A.items[idx];
         it.desc = uni(it.desc); it.tile = uni(it.tile);
@@ -946,10 +1001,10 @@ void knn_kernel(KnnArgs A, const double* __restrict__ col64, const float* __rest
             const KnnDev& kd = A.descs[it.desc];
             bool ok = true;
             (void)enm_guard(A.maxabs[A.orders[kd.order].inst], &ok);
-            if (ok) knn_item<D, P, true>(A, it, col64, col32, res64, pairs, rechecks);
-            else    knn_item<D, P, false>(A, it, col64, col32, res64, pairs, rechecks);
+            if (ok) knn_item<D, P, true>(A, it, col64, col32, res64, pairs, rechecks, stage);
+            else    knn_item<D, P, false>(A, it, col64, col32, res64, pairs, rechecks, stage);
         } else {
-            knn_item<D, P, false>(A, it, col64, col32, res64, pairs, rechecks);
+            knn_item<D, P, false>(A, it, col64, col32, res64, pairs, rechecks, stage);
         }
     });
     if (threadIdx.x == 0) {
