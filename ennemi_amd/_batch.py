@@ -71,14 +71,7 @@ class Batch:
         arena = np.empty(int(offsets[-1]), dtype=np.float64)
         for i, c in enumerate(self._cols):
             arena[offsets[i]:offsets[i + 1]] = c
-        n_inst = len(self._inst)
-        inst_arr = (nat.EnmInstance * n_inst)()
-        for i, (kind, k, n, dx, dy, dz, n_segs, seg_index, cols) in enumerate(self._inst):
-            e = inst_arr[i]
-            e.kind, e.k, e.n, e.dx, e.dy, e.dz = kind, k, n, dx, dy, dz
-            e.n_segs, e.seg_index = n_segs, seg_index
-            for t, c in enumerate(cols):
-                e.col[t] = int(offsets[c])
+        inst_arr = _instance_table(self._inst, offsets)
         seg_arr = (ctypes.c_int64 * max(1, len(self._seg)))(*self._seg)
         return arena, inst_arr, seg_arr
 
@@ -96,6 +89,28 @@ class Batch:
         db = self.device(flags, export)
         db.launch()
         return db.results()
+
+
+def _instance_table(inst: List[tuple], col_offsets: Optional[np.ndarray]):
+    """The ctypes enm_instance table of (kind, k, n, dx, dy, dz, n_segs,
+    seg_index, columns) tuples; a column is an arena offset, or an index
+    into col_offsets."""
+    n_inst = len(inst)
+    inst_arr = (nat.EnmInstance * n_inst)()
+    if n_inst == 0:
+        return inst_arr
+    tab = np.ctypeslib.as_array(inst_arr)   # the same memory, field by field
+    head = np.array([t[:8] for t in inst], dtype=np.int64).reshape(n_inst, 8)
+    for f, name in enumerate(("kind", "k", "n", "dx", "dy", "dz", "n_segs", "seg_index")):
+        tab[name] = head[:, f]
+    width = max(len(t[8]) for t in inst)
+    cols = np.zeros((n_inst, width), dtype=np.int64)
+    for i, t in enumerate(inst):
+        cols[i, :len(t[8])] = t[8]
+    if col_offsets is not None:
+        cols = np.where(np.arange(width) < np.array([len(t[8]) for t in inst])[:, None], col_offsets[cols], 0)
+    tab["col"][:, :width] = cols
+    return inst_arr
 
 
 class DeviceBatch:
@@ -179,14 +194,7 @@ class ArenaBatch:
         return len(self._inst) - 1
 
     def pack(self):
-        n_inst = len(self._inst)
-        inst_arr = (nat.EnmInstance * n_inst)()
-        for i, (kind, k, n, dx, dy, dz, n_segs, seg_index, offs) in enumerate(self._inst):
-            e = inst_arr[i]
-            e.kind, e.k, e.n, e.dx, e.dy, e.dz = kind, k, n, dx, dy, dz
-            e.n_segs, e.seg_index = n_segs, seg_index
-            for t, o in enumerate(offs):
-                e.col[t] = o
+        inst_arr = _instance_table(self._inst, None)
         seg_arr = (ctypes.c_int64 * 1)(0)
         return None, inst_arr, seg_arr
 

@@ -212,6 +212,14 @@ class _Tasks:
         self.batch = Batch()
         self.slots: List[Tuple[str, int]] = []   # ("gpu", inst) or ("host", value index)
         self.host_values: List[float] = []
+        # Work left for after the batch's launch (warnings that read device
+        # results), so that it does not hold the launch back.
+        self.deferred: List[Callable[[], None]] = []
+
+    def finish(self) -> None:
+        for f in self.deferred:
+            f()
+        self.deferred = []
 
     def host(self, value: float) -> None:
         self.slots.append(("host", len(self.host_values)))
@@ -222,6 +230,7 @@ class _Tasks:
 
     def run(self) -> List[float]:
         gpu_vals = self.batch.run() if len(self.batch) else np.empty(0)
+        self.finish()
         return [float(gpu_vals[i]) if kind == "gpu" else self.host_values[i] for kind, i in self.slots]
 
 
@@ -379,6 +388,7 @@ def plan_mi(y, x, lag=0, *, k: int = 3, cond=None, cond_lag=0, mask=None,
     mask_arr = np.asarray(mask) if mask is not None else None
     tasks, indices, _ = _plan_mi(y_arr, x_arr, lag_arr, k, cond_arr, cond_lag_arr, mask_arr,
                                  discrete_x, discrete_y, preprocess, drop_nan, prep)
+    tasks.finish()
     if tasks.host_values:
         raise ValueError("plan_mi covers the GPU estimators only (not discrete-discrete tasks)")
     in_batch = [None] * len(indices)
@@ -466,36 +476,46 @@ class DevicePrep:
         self.dz = dz = 0 if cond is None else cond.shape[1]
         series = [y] + ([x] if x.ndim == 1 else [x[:, v] for v in range(nvar)]) + \
             ([] if cond is None else [cond[:, c] for c in range(dz)])
-        raw = np.empty(n_rows * len(series), dtype=np.float64)
+        # The series go up from pinned host memory, asynchronously on the
+        # preparation stream (ahead of the kernels that read them), while
+        # the host builds the task table.
+        self.raw_host = torch.empty(n_rows * len(series), dtype=torch.float64, pin_memory=True)
+        raw = self.raw_host.numpy()
         for i, col in enumerate(series):
             raw[i * n_rows:(i + 1) * n_rows] = col
+        with torch.cuda.stream(self.st):
+            self.d_raw = self.raw_host.to(self.dev, non_blocking=True)
+            self.d_mask = torch.from_numpy(np.ascontiguousarray(mask, dtype=np.uint8)).to(self.dev) \
+                if mask is not None else None
         min_lag = min(min_lag, 0)
         max_lag = max(max_lag, 0)
         w = n_rows - max_lag + min_lag
         self.n_tasks = n_tasks = len(indices)
         # Device order: by x column, then lag, so a kernel's consecutive
         # blocks (one XCD's share) read one x series.
-        order = sorted(range(n_tasks), key=lambda t: (indices[t][1], indices[t][0]))
+        idx = np.asarray(indices, dtype=np.int64).reshape(n_tasks, 2)
+        order = np.lexsort((idx[:, 0], idx[:, 1]))
         self.place = np.empty(n_tasks, dtype=np.int64)   # task -> its device slot
         self.place[order] = np.arange(n_tasks)
-        self.tasks = tasks = (nat.EnmPrepTask * n_tasks)()
-        for t, (li, v) in enumerate(indices[u] for u in order):
-            T = tasks[t]
-            T.x_col, T.y_col = (1 + v) * n_rows, 0
-            T.x0, T.y0, T.w, T.dz = max_lag - int(lag[li]), max_lag, w, dz
-            for c in range(dz):
-                T.z_col[c] = (1 + nvar + c) * n_rows
-                T.z0[c] = max_lag - int(cond_lag[li][c])
+        self.tasks = (nat.EnmPrepTask * n_tasks)()
+        li, v = idx[order, 0], idx[order, 1]
+        tab = np.ctypeslib.as_array(self.tasks)   # the same memory, field by field
+        tab["x_col"] = (1 + v) * n_rows
+        tab["y_col"] = 0
+        tab["x0"] = max_lag - np.asarray(lag, dtype=np.int64)[li]
+        tab["y0"] = max_lag
+        tab["w"] = w
+        tab["dz"] = dz
+        if dz:
+            cl = np.asarray(cond_lag, dtype=np.int64).reshape(len(lag), -1)
+            tab["z_col"][:, :dz] = (1 + nvar + np.arange(dz)) * n_rows
+            tab["z0"][:, :dz] = max_lag - cl[li, :dz]
         self.preprocess = preprocess
         self.hints = hints
         self.flags = (nat.P_DROP_NAN if drop_nan else 0) | (nat.P_RESCALE if preprocess else 0) | \
             (nat.P_HINTS if hints else 0)
         self.h_n = np.zeros(n_tasks, dtype=np.int64)
         self.h_nan = np.zeros(n_tasks, dtype=np.uint32)
-        with torch.cuda.stream(self.st):
-            self.d_raw = torch.from_numpy(raw).to(self.dev, non_blocking=False)
-            self.d_mask = torch.from_numpy(np.ascontiguousarray(mask, dtype=np.uint8)).to(self.dev) \
-                if mask is not None else None
         self.arena = self.hint = self.d_const = self.d_noise = None
         self.noise_len = 0
 
@@ -562,22 +582,27 @@ def _plan_mi_device(y, x, lag, k, cond, cond_lag, mask, preprocess, drop_nan, mi
     prep = DevicePrep(y, x, lag, cond, cond_lag, mask, preprocess, drop_nan, min_lag, max_lag, indices)
     prep.count(k)
     prep.write()
-    const = prep.constant_flags()
     dz = prep.dz
-    for t in range(prep.n_tasks):
-        for bit in (1, 2, 4):
-            if const[t] & bit and (bit != 4 or dz > 0):
-                warn(PREPROCESS_CONSTANT_DATA_WARNING)
+
+    def constant_warnings():
+        # Read after the batch is launched (the flags are ready by then):
+        # reading them first would wait for the preparation before the
+        # batch could be planned and launched.
+        const = prep.constant_flags()
+        for t in range(prep.n_tasks):
+            for bit in (1, 2, 4):
+                if const[t] & bit and (bit != 4 or dz > 0):
+                    warn(PREPROCESS_CONSTANT_DATA_WARNING)
+
     tasks_out = _Tasks()
+    tasks_out.deferred.append(constant_warnings)
     tasks_out.batch = ArenaBatch(prep.arena, prep.hint)
     tasks_out.batch.prep = prep
     kind = nat.KIND_KSG if dz == 0 else nat.KIND_CMI
-    for t in range(prep.n_tasks):
-        n = int(prep.h_n[t])
-        offs = [int(prep.arena_off[t]) + c * n for c in range(2 + dz)]
-        tasks_out.batch.add(kind, k, n, offs, 1, 1, dz)
-    for t in range(prep.n_tasks):
-        tasks_out.gpu(int(prep.place[t]))
+    offs = (prep.arena_off[:, None] + prep.h_n[:, None] * np.arange(2 + dz)).tolist()
+    for n, o in zip(prep.h_n.tolist(), offs):
+        tasks_out.batch.add(kind, k, n, o, 1, 1, dz)
+    tasks_out.slots = [("gpu", p) for p in prep.place.tolist()]
     return tasks_out
 
 

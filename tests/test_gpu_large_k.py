@@ -38,7 +38,7 @@ def test_large_k_ksg(k, mode):
 
 
 @pytest.mark.parametrize("mode", ["f32", "fp64", "brute"])
-@pytest.mark.parametrize("k", [32, 100, 200])
+@pytest.mark.parametrize("k", [32, 100, 127, 200])
 def test_large_k_cmi_and_entropy(k, mode):
     nat = _nat()
     a = _data(zlib.crc32(b"lk_cmi") + k, 1000, 4)
