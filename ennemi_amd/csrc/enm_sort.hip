@@ -564,31 +564,57 @@ void ss_count_kernel(const SortSeg* __restrict__ segs, int nsegs, const SampleSe
 // positions (bucket-major, chunk-minor: the first position of chunk c's keys
 // of bucket b), and pack consecutive buckets greedily into runs of at most
 // ENM_SS_TILE keys (a larger bucket is a run of its own), appended to the
-// segment's run queue (queue q's runs start at tiles[qbase[q]]).
-__global__ __launch_bounds__(ENM_SORT_THREADS)
+// segment's run queue (queue q's runs start at tiles[qbase[q]]).  A thread
+// per bucket walks its column of the count table SCAN_BATCH chunks at a
+// time, all loads of a batch issued before its additions and stores (a
+// store between loads would keep the next load from issuing early: a
+// 2^20-key segment has 512 chunks, one wait each).
+constexpr int SCAN_THREADS = ENM_SS_MAXB;
+constexpr int SCAN_BATCH = 16;
+__global__ __launch_bounds__(SCAN_THREADS)
 void ss_scan_kernel(const SortSeg* __restrict__ segs, const SampleSeg* __restrict__ ss,
                     int* __restrict__ counts, SsTile* __restrict__ tiles, const int* __restrict__ qbase,
                     unsigned* __restrict__ qctl, SsTile* __restrict__ long_runs, const int* __restrict__ hint_bad) {
     __shared__ int tot[ENM_SS_MAXB];
     __shared__ int start[ENM_SS_MAXB];
-    __shared__ int part[ENM_SORT_THREADS / ENM_WAVE];
+    __shared__ int part[SCAN_THREADS / ENM_WAVE];
     const SortSeg sg = segs[blockIdx.x];
     if (sg.hint && hint_bad != nullptr && hint_bad[blockIdx.x] == 0) return;   // no runs to sort
     const SampleSeg q = ss[blockIdx.x];
     const int nb = q.nb;
     const int nch = (sg.len + ENM_SORT_TILE - 1) / ENM_SORT_TILE;
     int* cnt = counts + q.cbase;
-    for (int b = threadIdx.x; b < nb; b += ENM_SORT_THREADS) {
+    const int b = threadIdx.x;   // nb <= ENM_SS_MAXB = SCAN_THREADS
+    if (b < nb) {
         int sum = 0;
-        for (int c = 0; c < nch; ++c) sum += cnt[(int64_t)c * nb + b];
+        int c = 0;
+        for (; c + SCAN_BATCH <= nch; c += SCAN_BATCH) {
+            int v[SCAN_BATCH];
+#pragma unroll
+            for (int u = 0; u < SCAN_BATCH; ++u) v[u] = cnt[(int64_t)(c + u) * nb + b];
+#pragma unroll
+            for (int u = 0; u < SCAN_BATCH; ++u) sum += v[u];
+        }
+        for (; c < nch; ++c) sum += cnt[(int64_t)c * nb + b];
         tot[b] = sum;
         start[b] = sum;
     }
     __syncthreads();
-    block_exscan<ENM_SORT_THREADS>(start, nb, part);
-    for (int b = threadIdx.x; b < nb; b += ENM_SORT_THREADS) {
+    block_exscan<SCAN_THREADS>(start, nb, part);
+    if (b < nb) {
         int run = start[b];
-        for (int c = 0; c < nch; ++c) {
+        int c = 0;
+        for (; c + SCAN_BATCH <= nch; c += SCAN_BATCH) {
+            int v[SCAN_BATCH];
+#pragma unroll
+            for (int u = 0; u < SCAN_BATCH; ++u) v[u] = cnt[(int64_t)(c + u) * nb + b];
+#pragma unroll
+            for (int u = 0; u < SCAN_BATCH; ++u) {
+                cnt[(int64_t)(c + u) * nb + b] = run;
+                run += v[u];
+            }
+        }
+        for (; c < nch; ++c) {
             const int v = cnt[(int64_t)c * nb + b];
             cnt[(int64_t)c * nb + b] = run;
             run += v;
@@ -1055,7 +1081,7 @@ hipError_t enm_launch_sample_sort(const SampleSortArgs& a, const double* arena, 
     }
     hipLaunchKernelGGL(ss_count_kernel, dim3(a.ntiles), dim3(ENM_SORT_THREADS), 0, st,
                        a.segs, a.nsegs, a.ss, arena, spk, spi, a.bid, a.counts, hint_bad);
-    hipLaunchKernelGGL(ss_scan_kernel, dim3(a.nsegs), dim3(ENM_SORT_THREADS), 0, st,
+    hipLaunchKernelGGL(ss_scan_kernel, dim3(a.nsegs), dim3(SCAN_THREADS), 0, st,
                        a.segs, a.ss, a.counts, a.tiles, a.qbase, a.qctl, a.long_runs, hint_bad);
     hipLaunchKernelGGL(ss_scatter_kernel, dim3(a.ntiles), dim3(ENM_SORT_THREADS), 0, st,
                        a.segs, a.nsegs, a.ss, arena, a.bid, a.counts, k1, i1, hint_bad);
