@@ -1055,7 +1055,8 @@ int enqueue(enm_ctx* ctx, Plan& P, const double* arena, const int* hint, double*
     nl += 4;
     const bool hints = P.hinted && P.sample_sort && hint != nullptr;
     if (hints) {
-        if ((rc = chk(enm_launch_fill_i32(P.hint_bad.as<int>(), (int64_t)P.segs.size(), 0, st), "fill"))) return rc;
+        if ((rc = chk(enm_launch_fill_i32(P.hint_bad.as<int>(), (int64_t)P.segs.size() + 1, 0, st), "fill")))
+            return rc;
         ++nl;
     }
     if ((rc = mark(1))) return rc;
@@ -1093,6 +1094,7 @@ int enqueue(enm_ctx* ctx, Plan& P, const double* arena, const int* hint, double*
         a.with32 = P.f32 && P.need32 ? 1 : 0;
         a.hint = hints ? hint : nullptr;
         a.hint_bad = hints ? P.hint_bad.as<int>() : nullptr;
+        a.all_hinted = hints && std::all_of(P.segs.begin(), P.segs.end(), [](const SortSeg& g) { return g.hint != 0; });
         if ((rc = chk(enm_launch_sample_sort(a, arena, P.sk0.as<double>(), P.si0.as<int>(), P.sk1.as<double>(),
                                              P.si1.as<int>(), &nl, st), "sort")))
             return rc;

@@ -11,7 +11,8 @@ hipError_t enm_launch_fill_i32(int* p, int64_t n, int v, hipStream_t st);
 
 hipError_t enm_launch_sort(const SortSeg* d_segs, int nsegs, int ntiles, int64_t maxlen, int tile,
                            const double* arena, const int* iarena, double* k0, int* i0, double* k1, int* i1,
-                           int* split, int* final_buf, int* n_launch, hipStream_t st);
+                           int* split, int* final_buf, int* n_launch, hipStream_t st,
+                           const int* skip = nullptr);
 
 // Sample sort of the first-stage segments (all at most ENM_SS_MAXLEN keys),
 // laid out straight into the orders that consume them (the first-stage
@@ -42,7 +43,9 @@ struct SampleSortArgs {
     unsigned long long* maxabs;
     int with32;
     const int* hint;            // caller's sort hints, parallel to the arena (NULL: none)
-    int* hint_bad;              // per segment: 1 when its hint was absent or wrong (zeroed before)
+    int* hint_bad;              // per segment: 1 when its hint was absent or wrong; then one
+                                // more: 1 when any was (all nsegs + 1 zeroed before)
+    int all_hinted;             // every segment has a hint: nothing else is sorted unless one fails
 };
 hipError_t enm_launch_sample_sort(const SampleSortArgs& a, const double* arena, double* k0, int* i0,
                                   double* k1, int* i1, int* n_launch, hipStream_t st);

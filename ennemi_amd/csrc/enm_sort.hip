@@ -234,9 +234,10 @@ template <int TILE>
 __global__ __launch_bounds__(ENM_SORT_THREADS)
 void block_sort_kernel(const SortSeg* __restrict__ segs, int nsegs,
                        const double* __restrict__ arena, const int* __restrict__ iarena,
-                       double* __restrict__ kout, int* __restrict__ iout) {
+                       double* __restrict__ kout, int* __restrict__ iout, const int* __restrict__ skip) {
     __shared__ double sk[ENM_LDS_PAD(TILE)];
     __shared__ int si[ENM_LDS_PAD(TILE)];
+    if (skip != nullptr && *skip == 0) return;   // nothing to sort this run
     const int tile = blockIdx.x;
     const int s = seg_of_tile(segs, nsegs, tile);
     const SortSeg sg = segs[s];
@@ -330,9 +331,10 @@ __device__ __forceinline__ MergeChunk merge_chunk(const SortSeg& sg, int tile, i
 // in parallel (each a binary search of ~17 dependent loads).
 __global__ __launch_bounds__(ENM_SORT_THREADS)
 void co_rank_kernel(const SortSeg* __restrict__ segs, int nsegs, int ntiles, int64_t width,
-                    const double* __restrict__ kin, const int* __restrict__ iin, int* __restrict__ split) {
+                    const double* __restrict__ kin, const int* __restrict__ iin, int* __restrict__ split,
+                    const int* __restrict__ skip) {
     const int tile = blockIdx.x * ENM_SORT_THREADS + threadIdx.x;
-    if (tile >= ntiles) return;
+    if (tile >= ntiles || (skip != nullptr && *skip == 0)) return;
     const SortSeg sg = segs[seg_of_tile(segs, nsegs, tile)];
     const MergeChunk m = merge_chunk(sg, tile, width);
     split[tile] = m.nb == 0 ? m.d0
@@ -344,9 +346,10 @@ __global__ __launch_bounds__(ENM_SORT_THREADS)
 void merge_pass_kernel(const SortSeg* __restrict__ segs, int nsegs, int64_t width,
                        const double* __restrict__ kin, const int* __restrict__ iin,
                        const int* __restrict__ split,
-                       double* __restrict__ kout, int* __restrict__ iout) {
+                       double* __restrict__ kout, int* __restrict__ iout, const int* __restrict__ skip) {
     __shared__ double sk[ENM_LDS_PAD(ENM_SORT_TILE)];
     __shared__ int si[ENM_LDS_PAD(ENM_SORT_TILE)];
+    if (skip != nullptr && *skip == 0) return;
     const int tile = blockIdx.x;
     const SortSeg sg = segs[seg_of_tile(segs, nsegs, tile)];
     const MergeChunk m = merge_chunk(sg, tile, width);
@@ -504,7 +507,8 @@ __device__ __forceinline__ int64_t ss_sample_pos(int i, int ns, int64_t len) {
 __global__ __launch_bounds__(ENM_SORT_THREADS)
 void ss_sample_kernel(const SortSeg* __restrict__ segs, const SampleSeg* __restrict__ ss,
                       const double* __restrict__ arena, double* __restrict__ sk, int* __restrict__ si,
-                      unsigned* __restrict__ qctl) {
+                      unsigned* __restrict__ qctl, const int* __restrict__ skip) {
+    if (skip != nullptr && *skip == 0) return;   // every hint held: nothing to sort this run
     if (blockIdx.x == 0 && threadIdx.x <= 2 * ENM_NQ) qctl[threadIdx.x * ENM_CSTRIDE] = 0u;
     const SortSeg sg = segs[blockIdx.x];
     const SampleSeg q = ss[blockIdx.x];
@@ -523,10 +527,11 @@ __global__ __launch_bounds__(ENM_SORT_THREADS)
 void ss_count_kernel(const SortSeg* __restrict__ segs, int nsegs, const SampleSeg* __restrict__ ss,
                      const double* __restrict__ arena, const double* __restrict__ samp_k,
                      const int* __restrict__ samp_i, unsigned short* __restrict__ bid, int* __restrict__ counts,
-                     const int* __restrict__ hint_bad) {
+                     const int* __restrict__ hint_bad, const int* __restrict__ skip) {
     __shared__ double spk[ENM_SS_MAXB];
     __shared__ int spi[ENM_SS_MAXB];
     __shared__ int hist[ENM_SS_MAXB];
+    if (skip != nullptr && *skip == 0) return;
     const int tile = blockIdx.x;
     const int s = seg_of_tile(segs, nsegs, tile);
     const SortSeg sg = segs[s];
@@ -574,10 +579,12 @@ constexpr int SCAN_BATCH = 16;
 __global__ __launch_bounds__(SCAN_THREADS)
 void ss_scan_kernel(const SortSeg* __restrict__ segs, const SampleSeg* __restrict__ ss,
                     int* __restrict__ counts, SsTile* __restrict__ tiles, const int* __restrict__ qbase,
-                    unsigned* __restrict__ qctl, SsTile* __restrict__ long_runs, const int* __restrict__ hint_bad) {
+                    unsigned* __restrict__ qctl, SsTile* __restrict__ long_runs, const int* __restrict__ hint_bad,
+                    const int* __restrict__ skip) {
     __shared__ int tot[ENM_SS_MAXB];
     __shared__ int start[ENM_SS_MAXB];
     __shared__ int part[SCAN_THREADS / ENM_WAVE];
+    if (skip != nullptr && *skip == 0) return;
     const SortSeg sg = segs[blockIdx.x];
     if (sg.hint && hint_bad != nullptr && hint_bad[blockIdx.x] == 0) return;   // no runs to sort
     const SampleSeg q = ss[blockIdx.x];
@@ -659,7 +666,7 @@ __global__ __launch_bounds__(ENM_SORT_THREADS)
 void ss_scatter_kernel(const SortSeg* __restrict__ segs, int nsegs, const SampleSeg* __restrict__ ss,
                        const double* __restrict__ arena, const unsigned short* __restrict__ bid,
                        const int* __restrict__ offs, double* __restrict__ kout, int* __restrict__ iout,
-                       const int* __restrict__ hint_bad) {
+                       const int* __restrict__ hint_bad, const int* __restrict__ skip) {
     constexpr int PER = ENM_SORT_TILE / ENM_SORT_THREADS;
     __shared__ double sk[ENM_SORT_TILE];
     __shared__ int si[ENM_SORT_TILE];
@@ -667,6 +674,7 @@ void ss_scatter_kernel(const SortSeg* __restrict__ segs, int nsegs, const Sample
     __shared__ int hist[ENM_SS_MAXB];
     __shared__ int goff[ENM_SS_MAXB];
     __shared__ int part[ENM_SORT_THREADS / ENM_WAVE];
+    if (skip != nullptr && *skip == 0) return;
     const int tile = blockIdx.x;
     const int s = seg_of_tile(segs, nsegs, tile);
     const SortSeg sg = segs[s];
@@ -889,7 +897,10 @@ void ss_hint_kernel(const SortSeg* __restrict__ segs, int nsegs, const int* __re
     if (wave_any(bad) && (threadIdx.x & (ENM_WAVE - 1)) == 0) s_bad = 1;
     __syncthreads();
     if (s_bad) {
-        if (threadIdx.x == 0) hint_bad[s] = 1;
+        if (threadIdx.x == 0) {
+            hint_bad[s] = 1;
+            hint_bad[nsegs] = 1;   // some segment is sorted this run
+        }
         return;
     }
     for (int c = 0; c < q.ncons; ++c) {
@@ -933,11 +944,12 @@ void ss_hint_kernel(const SortSeg* __restrict__ segs, int nsegs, const int* __re
 // held to six waves per SIMD to match.
 __global__ __launch_bounds__(ENM_SS_THREADS, 6)
 void ss_tile_kernel(const SsTile* __restrict__ tiles, const int* __restrict__ qbase, unsigned* qctl,
-                    double* k1, int* i1, double* k0, int* i0, SsLayout L) {
+                    double* k1, int* i1, double* k0, int* i0, SsLayout L, const int* __restrict__ skip) {
     __shared__ double sk[ENM_LDS_PAD(ENM_SS_TILE)];
     __shared__ int si[ENM_LDS_PAD(ENM_SS_TILE)];
     __shared__ unsigned s_ticket;
     __shared__ double red[ENM_SS_THREADS / ENM_WAVE];
+    if (skip != nullptr && *skip == 0) return;   // the queues were neither cleared nor filled
     const int home = (int)(blockIdx.x % ENM_NQ);
     for (int r = 0; r < ENM_NQ; ++r) {
         const int qq = (home + r) % ENM_NQ;
@@ -971,11 +983,12 @@ void ss_tile_kernel(const SsTile* __restrict__ tiles, const int* __restrict__ qb
 // ss_tile_kernel, whose registers then hold its LDS runs without spilling.
 __global__ __launch_bounds__(ENM_SS_THREADS)
 void ss_long_kernel(const SsTile* __restrict__ runs, const unsigned* __restrict__ qctl, double* k1, int* i1,
-                    double* k0, int* i0, SsLayout L) {
+                    double* k0, int* i0, SsLayout L, const int* __restrict__ skip) {
     __shared__ double sk[ENM_LDS_PAD(ENM_SS_TILE)];
     __shared__ int si[ENM_LDS_PAD(ENM_SS_TILE)];
     __shared__ int sp[2];
     __shared__ double red[ENM_SS_THREADS / ENM_WAVE];
+    if (skip != nullptr && *skip == 0) return;
     const unsigned n = qctl[2 * ENM_NQ * ENM_CSTRIDE];   // final: ss_scan_kernel has finished
     for (unsigned t = blockIdx.x; t < n; t += gridDim.x) {
         const SsTile tl = runs[t];
@@ -1001,20 +1014,20 @@ hipError_t enm_launch_fill_u64(unsigned long long* p, int64_t n, unsigned long l
 
 hipError_t enm_launch_sort(const SortSeg* d_segs, int nsegs, int ntiles, int64_t maxlen, int tile,
                            const double* arena, const int* iarena, double* k0, int* i0, double* k1, int* i1,
-                           int* split, int* final_buf, int* n_launch, hipStream_t st) {
+                           int* split, int* final_buf, int* n_launch, hipStream_t st, const int* skip) {
     *final_buf = 0;
     if (ntiles <= 0) return hipSuccess;
     // Segments no longer than the tile are sorted by the block sort alone;
     // longer ones (always with 2048-key tiles) continue with merge passes.
     if (tile == 512)
         hipLaunchKernelGGL(block_sort_kernel<512>, dim3(ntiles), dim3(ENM_SORT_THREADS), 0, st,
-                           d_segs, nsegs, arena, iarena, k0, i0);
+                           d_segs, nsegs, arena, iarena, k0, i0, skip);
     else if (tile == 1024)
         hipLaunchKernelGGL(block_sort_kernel<1024>, dim3(ntiles), dim3(ENM_SORT_THREADS), 0, st,
-                           d_segs, nsegs, arena, iarena, k0, i0);
+                           d_segs, nsegs, arena, iarena, k0, i0, skip);
     else if (tile == ENM_SORT_TILE)
         hipLaunchKernelGGL(block_sort_kernel<ENM_SORT_TILE>, dim3(ntiles), dim3(ENM_SORT_THREADS), 0, st,
-                           d_segs, nsegs, arena, iarena, k0, i0);
+                           d_segs, nsegs, arena, iarena, k0, i0, skip);
     else
         return hipErrorInvalidValue;
     if (tile != ENM_SORT_TILE && maxlen > tile) return hipErrorInvalidValue;
@@ -1028,9 +1041,9 @@ hipError_t enm_launch_sort(const SortSeg* d_segs, int nsegs, int ntiles, int64_t
         double* kout = cur ? k0 : k1;
         int* iout = cur ? i0 : i1;
         hipLaunchKernelGGL(co_rank_kernel, dim3((ntiles + ENM_SORT_THREADS - 1) / ENM_SORT_THREADS),
-                           dim3(ENM_SORT_THREADS), 0, st, d_segs, nsegs, ntiles, w, kin, iin, split);
+                           dim3(ENM_SORT_THREADS), 0, st, d_segs, nsegs, ntiles, w, kin, iin, split, skip);
         hipLaunchKernelGGL(merge_pass_kernel, dim3(ntiles), dim3(ENM_SORT_THREADS), 0, st,
-                           d_segs, nsegs, w, kin, iin, split, kout, iout);
+                           d_segs, nsegs, w, kin, iin, split, kout, iout, skip);
         *n_launch += 2;
         e = hipGetLastError();
         if (e != hipSuccess) return e;
@@ -1044,22 +1057,6 @@ hipError_t enm_launch_sample_sort(const SampleSortArgs& a, const double* arena, 
                                   double* k1, int* i1, int* n_launch, hipStream_t st) {
     if (a.nsegs <= 0) return hipSuccess;
     if (a.qctl == nullptr || a.qbase == nullptr) return hipErrorInvalidValue;
-    // Samples go to the second sample buffers, sorted into the first (or
-    // back, as the merge passes end).
-    hipLaunchKernelGGL(ss_sample_kernel, dim3(a.nsegs), dim3(ENM_SORT_THREADS), 0, st,
-                       a.segs, a.ss, arena, a.samp_k1, a.samp_i1, a.qctl);
-    ++*n_launch;
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return e;
-    int fb = 0;
-    if (a.n_samp_segs > 0) {
-        e = enm_launch_sort(a.samp_segs, a.n_samp_segs, a.samp_tiles, a.samp_maxlen, ENM_SORT_TILE,
-                            a.samp_k1, a.samp_i1, a.samp_k0, a.samp_i0, a.samp_k1, a.samp_i1, a.samp_split,
-                            &fb, n_launch, st);
-        if (e != hipSuccess) return e;
-    }
-    const double* spk = fb ? a.samp_k1 : a.samp_k0;
-    const int* spi = fb ? a.samp_i1 : a.samp_i0;
     SsLayout L;
     L.segs = a.segs;
     L.ss = a.ss;
@@ -1071,24 +1068,45 @@ hipError_t enm_launch_sample_sort(const SampleSortArgs& a, const double* arena, 
     L.perm = a.perm;
     L.maxabs = a.maxabs;
     L.with32 = a.with32;
+    // Hinted segments first: when every segment is hinted, the rest of the
+    // sample sort runs only if some hint failed (hint_bad[nsegs], set by
+    // ss_hint_kernel), and otherwise every kernel returns at once.
     const int* hint_bad = nullptr;
+    const int* skip = nullptr;
     if (a.hint != nullptr) {
         if (a.hint_bad == nullptr) return hipErrorInvalidValue;
         hipLaunchKernelGGL(ss_hint_kernel, dim3(a.ntiles), dim3(ENM_SORT_THREADS), 0, st,
                            a.segs, a.nsegs, a.hint, a.hint_bad, L);
         ++*n_launch;
         hint_bad = a.hint_bad;
+        if (a.all_hinted) skip = a.hint_bad + a.nsegs;
     }
+    // Samples go to the second sample buffers, sorted into the first (or
+    // back, as the merge passes end).
+    hipLaunchKernelGGL(ss_sample_kernel, dim3(a.nsegs), dim3(ENM_SORT_THREADS), 0, st,
+                       a.segs, a.ss, arena, a.samp_k1, a.samp_i1, a.qctl, skip);
+    ++*n_launch;
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    int fb = 0;
+    if (a.n_samp_segs > 0) {
+        e = enm_launch_sort(a.samp_segs, a.n_samp_segs, a.samp_tiles, a.samp_maxlen, ENM_SORT_TILE,
+                            a.samp_k1, a.samp_i1, a.samp_k0, a.samp_i0, a.samp_k1, a.samp_i1, a.samp_split,
+                            &fb, n_launch, st, skip);
+        if (e != hipSuccess) return e;
+    }
+    const double* spk = fb ? a.samp_k1 : a.samp_k0;
+    const int* spi = fb ? a.samp_i1 : a.samp_i0;
     hipLaunchKernelGGL(ss_count_kernel, dim3(a.ntiles), dim3(ENM_SORT_THREADS), 0, st,
-                       a.segs, a.nsegs, a.ss, arena, spk, spi, a.bid, a.counts, hint_bad);
+                       a.segs, a.nsegs, a.ss, arena, spk, spi, a.bid, a.counts, hint_bad, skip);
     hipLaunchKernelGGL(ss_scan_kernel, dim3(a.nsegs), dim3(SCAN_THREADS), 0, st,
-                       a.segs, a.ss, a.counts, a.tiles, a.qbase, a.qctl, a.long_runs, hint_bad);
+                       a.segs, a.ss, a.counts, a.tiles, a.qbase, a.qctl, a.long_runs, hint_bad, skip);
     hipLaunchKernelGGL(ss_scatter_kernel, dim3(a.ntiles), dim3(ENM_SORT_THREADS), 0, st,
-                       a.segs, a.nsegs, a.ss, arena, a.bid, a.counts, k1, i1, hint_bad);
+                       a.segs, a.nsegs, a.ss, arena, a.bid, a.counts, k1, i1, hint_bad, skip);
     hipLaunchKernelGGL(ss_tile_kernel, dim3(a.tile_grid), dim3(ENM_SS_THREADS), 0, st,
-                       a.tiles, a.qbase, a.qctl, k1, i1, k0, i0, L);
+                       a.tiles, a.qbase, a.qctl, k1, i1, k0, i0, L, skip);
     hipLaunchKernelGGL(ss_long_kernel, dim3(a.long_grid), dim3(ENM_SS_THREADS), 0, st,
-                       a.long_runs, a.qctl, k1, i1, k0, i0, L);
+                       a.long_runs, a.qctl, k1, i1, k0, i0, L, skip);
     ++*n_launch;
     *n_launch += 4;
     return hipGetLastError();
