@@ -191,87 +191,14 @@ struct Term {
     }
 };
 
-// NumPy's pairwise_sum leaf: fewer than 8 values summed from 0.0; up to 128
-// by 8 interleaved accumulators combined as ((r0+r1)+(r2+r3))+((r4+r5)+(r6+r7)),
-// the remainder added after.
-__device__ __forceinline__ double pw_leaf(const Term& f, int64_t lo, int n) {
-    if (n < 8) {
-        double r = 0.0;
-        for (int i = 0; i < n; ++i) r += f(lo + i);
-        return r;
-    }
-    double r[8];
-#pragma unroll
-    for (int j = 0; j < 8; ++j) r[j] = f(lo + j);
-    int i = 8;
-    for (; i < n - (n % 8); i += 8) {
-#pragma unroll
-        for (int j = 0; j < 8; ++j) r[j] += f(lo + i + j);
-    }
-    double res = ((r[0] + r[1]) + (r[2] + r[3])) + ((r[4] + r[5]) + (r[6] + r[7]));
-    for (; i < n; ++i) res += f(lo + i);
-    return res;
-}
-
-// The leaves of NumPy's pairwise recursion over n <= PW_CHUNK values, in
-// depth-first order: a node of more than 128 values splits at half its
-// length rounded down to a multiple of 8.  Writes (offset, length) of each
-// leaf and returns their number (at most 128 for n <= 8192).
-__device__ int pw_leaves(int n, int* lo_out, int* len_out) {
-    int st_lo[16], st_len[16];
-    int sp = 0, nl = 0;
-    st_lo[0] = 0;
-    st_len[0] = n;
-    while (sp >= 0) {
-        const int lo = st_lo[sp], len = st_len[sp];
-        --sp;
-        if (len <= PW_LEAF) {
-            lo_out[nl] = lo;
-            len_out[nl] = len;
-            ++nl;
-            continue;
-        }
-        int n2 = len / 2;
-        n2 -= n2 % 8;
-        // Right pushed first so the left one is visited first.
-        ++sp; st_lo[sp] = lo + n2; st_len[sp] = len - n2;
-        ++sp; st_lo[sp] = lo; st_len[sp] = n2;
-    }
-    return nl;
-}
-
-// The same recursion, combining leaf sums (in depth-first order, consumed
-// from `leaf`) as NumPy does: left + right at every node.
-__device__ double pw_combine(int n, const double* leaf) {
-    int st_len[16];
-    double st_left[16];
-    bool st_right[16];
-    int sp = 0, next = 0;
-    st_len[0] = n;
-    st_right[0] = false;
-    for (;;) {
-        while (st_len[sp] > PW_LEAF) {
-            int n2 = st_len[sp] / 2;
-            n2 -= n2 % 8;
-            st_len[sp + 1] = n2;
-            st_right[sp + 1] = false;
-            ++sp;
-        }
-        double val = leaf[next++];
-        for (;;) {
-            if (sp == 0) return val;
-            if (!st_right[sp]) {
-                // A left child is done: keep it, continue with its sibling.
-                st_left[sp - 1] = val;
-                st_len[sp] = st_len[sp - 1] - st_len[sp];
-                st_right[sp] = true;
-                break;
-            }
-            val = st_left[sp - 1] + val;
-            --sp;
-        }
-    }
-}
+// NumPy's pairwise recursion over n <= PW_CHUNK values as a heap-ordered
+// binary tree (node 1 the root, nodes 2i and 2i+1 the halves of node i): a
+// node of more than PW_LEAF values splits at half its length rounded down
+// to a multiple of 8; smaller nodes are leaves.  Right halves are at most
+// half their parent plus 8 values, so every leaf of an 8191-value chunk is
+// within depth 7: heap slots 1 .. PW_HEAP - 1.
+constexpr int PW_HEAP = 256;
+constexpr int PW_DEPTH = 7;
 
 // The sum of a full PW_CHUNK-value chunk in NumPy's order, by one wave: the
 // recursion's 64 leaves of 128 values form a balanced tree.  Lane l takes
@@ -307,15 +234,18 @@ __device__ __forceinline__ double pw_full_chunk(const Term& f) {
 
 // Phase 2b: one wave per (array, chunk): the pairwise sum of a chunk of
 // values (pass 0) or of squared deviations from the array's mean (pass 1).
-// A full chunk takes pw_full_chunk; otherwise lanes sum the leaves and
-// lane 0 combines them in NumPy's order.
+// A full chunk takes pw_full_chunk; otherwise the wave builds the
+// recursion's tree in LDS level by level (slot lengths; -1 for no node),
+// sums its leaves, eight lanes to a leaf, and adds every node's two halves
+// bottom-up, left + right, as NumPy does.
 __global__ __launch_bounds__(ENM_WAVE)
 void prep_chunk_kernel(const PrepSumDev* __restrict__ sums, const int* __restrict__ chunk_sum_of,
                        const int* __restrict__ chunk_index, const int* __restrict__ list, int n_list,
                        const double* __restrict__ arena, const double* __restrict__ stats, int pass,
                        double* __restrict__ chunk_val) {
-    __shared__ int s_lo[PW_LEAF], s_len[PW_LEAF];
-    __shared__ double s_leaf[PW_LEAF];
+    __shared__ int s_lo[PW_HEAP], s_len[PW_HEAP];
+    __shared__ double s_val[PW_HEAP];
+    __shared__ short s_leaf[PW_HEAP];
     __shared__ int s_nl;
     if ((int)blockIdx.x >= n_list) return;
     const int c = list[blockIdx.x];
@@ -328,11 +258,80 @@ void prep_chunk_kernel(const PrepSumDev* __restrict__ sums, const int* __restric
         if (threadIdx.x == 0) chunk_val[c] = sum;
         return;
     }
-    if (threadIdx.x == 0) s_nl = pw_leaves(m, s_lo, s_len);
+    const int lane = (int)threadIdx.x;
+    // The tree, top-down: slot 1 is the whole chunk; a slot's halves exist
+    // when it has more than PW_LEAF values.
+    for (int i = lane; i < PW_HEAP; i += ENM_WAVE) s_len[i] = -1;
+    if (lane == 0) {
+        s_lo[1] = 0;
+        s_len[1] = m;
+        s_nl = 0;
+    }
     __syncthreads();
-    for (int l = threadIdx.x; l < s_nl; l += ENM_WAVE) s_leaf[l] = pw_leaf(f, s_lo[l], s_len[l]);
+    for (int d = 0; d < PW_DEPTH; ++d) {
+        for (int i = (1 << d) + lane; i < (2 << d); i += ENM_WAVE) {
+            const int len = s_len[i];
+            if (len > PW_LEAF) {
+                int n2 = len / 2;
+                n2 -= n2 % 8;
+                s_lo[2 * i] = s_lo[i];
+                s_len[2 * i] = n2;
+                s_lo[2 * i + 1] = s_lo[i] + n2;
+                s_len[2 * i + 1] = len - n2;
+            }
+        }
+        __syncthreads();
+    }
+    // The leaves, listed in any order (each is summed on its own).
+    for (int i = 1 + lane; i < PW_HEAP; i += ENM_WAVE) {
+        const int len = s_len[i];
+        if (len >= 0 && len <= PW_LEAF) s_leaf[atomicAdd(&s_nl, 1)] = (short)i;
+    }
     __syncthreads();
-    if (threadIdx.x == 0) chunk_val[c] = pw_combine(m, s_leaf);
+    // A leaf is NumPy's pairwise_sum leaf: fewer than 8 values summed from
+    // 0.0; up to 128 by 8 interleaved accumulators combined as
+    // ((r0+r1)+(r2+r3))+((r4+r5)+(r6+r7)), the remainder added after.
+    // Eight lanes per leaf, eight leaves per round: lane j of a group takes
+    // accumulator j (its up to sixteen values loaded at once, added in
+    // order), butterfly shuffles within the group combine the accumulators,
+    // and the group's first lane adds the remainder.
+    const int j = lane & 7;
+    const int nl = s_nl;
+    for (int l0 = 0; l0 < nl; l0 += ENM_WAVE / 8) {
+        const int l = l0 + (lane >> 3);
+        const int slot = l < nl ? s_leaf[l] : 0;
+        const int lo = l < nl ? s_lo[slot] : 0, len = l < nl ? s_len[slot] : 0;
+        const int full = len >= 8 ? len - len % 8 : 0;
+        double v[PW_LEAF / 8];
+#pragma unroll
+        for (int i = 0; i < PW_LEAF / 8; ++i) v[i] = 8 * i < full ? f(lo + 8 * i + j) : 0.0;
+        double acc = v[0];
+#pragma unroll
+        for (int i = 1; i < PW_LEAF / 8; ++i)
+            if (8 * i < full) acc += v[i];
+        acc += __shfl_xor(acc, 1, ENM_WAVE);
+        acc += __shfl_xor(acc, 2, ENM_WAVE);
+        acc += __shfl_xor(acc, 4, ENM_WAVE);
+        if (j == 0 && l < nl) {
+            double res;
+            if (len < 8) {
+                res = 0.0;
+                for (int t = 0; t < len; ++t) res += f(lo + t);
+            } else {
+                res = acc;
+                for (int t = full; t < len; ++t) res += f(lo + t);
+            }
+            s_val[slot] = res;
+        }
+    }
+    __syncthreads();
+    // Bottom-up: every split node is its left half plus its right half.
+    for (int d = PW_DEPTH - 1; d >= 0; --d) {
+        for (int i = (1 << d) + lane; i < (2 << d); i += ENM_WAVE)
+            if (s_len[i] > PW_LEAF) s_val[i] = s_val[2 * i] + s_val[2 * i + 1];
+        __syncthreads();
+    }
+    if (lane == 0) chunk_val[c] = s_val[1];
 }
 
 // Full chunks only, one per wave, four waves per block (no barriers: each
