@@ -289,20 +289,48 @@ void strip_sort_kernel(const GatherItem* __restrict__ items, const SortSeg* __re
     block_sort_lds<TILE>(sk, si);
     const int64_t n = od.n;
     const int ncol = od.ncol;
-    for (int t = threadIdx.x; t < len; t += ENM_SORT_THREADS) {
-        const int idx = si[lp(t)];
-        const int64_t p = it.pos0 + t;
-        perm[od.perm + p] = perm[od.base_perm + idx];
-        for (int c = 0; c < (od.xy_only ? 0 : ncol); ++c) {
-            col64[od.c64 + c * n + p] = col64[od.base_c64 + c * n + idx];
-            if (with32) col32[od.c32 + c * n + p] = col32[od.base_c32 + c * n + idx];
+    // A thread's PER elements: every load of a column goes out before any
+    // store (a store between two loads would hold the second one back).
+    constexpr int PER = TILE / ENM_SORT_THREADS;
+    int idx[PER];
+    auto in = [&](int k) { return k * ENM_SORT_THREADS + (int)threadIdx.x < len; };
+    auto pos = [&](int k) { return it.pos0 + k * ENM_SORT_THREADS + (int)threadIdx.x; };
+#pragma unroll
+    for (int k = 0; k < PER; ++k) idx[k] = in(k) ? si[lp(k * ENM_SORT_THREADS + (int)threadIdx.x)] : 0;
+    {
+        int pv[PER];
+#pragma unroll
+        for (int k = 0; k < PER; ++k) pv[k] = in(k) ? perm[od.base_perm + idx[k]] : 0;
+#pragma unroll
+        for (int k = 0; k < PER; ++k)
+            if (in(k)) perm[od.perm + pos(k)] = pv[k];
+    }
+    for (int c = 0; c < (od.xy_only ? 0 : ncol); ++c) {
+        double v[PER];
+#pragma unroll
+        for (int k = 0; k < PER; ++k) v[k] = in(k) ? col64[od.base_c64 + c * n + idx[k]] : 0.0;
+#pragma unroll
+        for (int k = 0; k < PER; ++k)
+            if (in(k)) col64[od.c64 + c * n + pos(k)] = v[k];
+        if (with32) {
+            float f[PER];
+#pragma unroll
+            for (int k = 0; k < PER; ++k) f[k] = in(k) ? col32[od.base_c32 + c * n + idx[k]] : 0.0f;
+#pragma unroll
+            for (int k = 0; k < PER; ++k)
+                if (in(k)) col32[od.c32 + c * n + pos(k)] = f[k];
         }
-        if (od.xy >= 0) {
-            double2 v;
-            v.x = col64[od.base_c64 + idx];
-            v.y = col64[od.base_c64 + n + idx];
-            reinterpret_cast<double2*>(col64 + od.xy)[p] = v;
+    }
+    if (od.xy >= 0) {
+        double2 v[PER];
+#pragma unroll
+        for (int k = 0; k < PER; ++k) {
+            v[k].x = in(k) ? col64[od.base_c64 + idx[k]] : 0.0;
+            v[k].y = in(k) ? col64[od.base_c64 + n + idx[k]] : 0.0;
         }
+#pragma unroll
+        for (int k = 0; k < PER; ++k)
+            if (in(k)) reinterpret_cast<double2*>(col64 + od.xy)[pos(k)] = v[k];
     }
 }
 
@@ -399,7 +427,9 @@ void gather_kernel(const GatherItem* __restrict__ items,
                    const double* __restrict__ skey, const int* __restrict__ sidx,
                    double* col64, float* col32,
                    int* perm, unsigned long long* __restrict__ maxabs, int with32) {
-    __shared__ double red[ENM_SORT_THREADS / ENM_WAVE];
+    // A thread's PER elements: every load of a column goes out before any
+    // store (a store between two loads would hold the second one back).
+    constexpr int PER = ENM_SORT_TILE / ENM_SORT_THREADS;
     const GatherItem it = items[blockIdx.x];
     const SortSeg sg = segs[it.sseg];
     // Fields read one by one: a local copy of the whole descriptor, indexed
@@ -407,46 +437,94 @@ void gather_kernel(const GatherItem* __restrict__ items,
     const OrderDev& od = orders[it.order];
     const int len = min(ENM_SORT_TILE, sg.len - it.off);
     const int64_t n = od.n;
-    const bool indirect = od.base_perm >= 0;
-    double m = 0.0;
-    for (int t = threadIdx.x; t < len; t += ENM_SORT_THREADS) {
-        const int64_t sp = sg.sbase + it.off + t;
-        const int64_t p = it.pos0 + it.off + t;
-        const int idx = sidx[sp];
-        if (indirect) {
-            // idx is a base-order position in the same strip: read locally.
-            perm[od.perm + p] = perm[od.base_perm + idx];
-            for (int c = 0; c < (od.xy_only ? 0 : od.ncol); ++c) {
-                col64[od.c64 + c * n + p] = col64[od.base_c64 + c * n + idx];
-                if (with32) col32[od.c32 + c * n + p] = col32[od.base_c32 + c * n + idx];
-            }
-            if (od.xy >= 0) {
-                double2 v;
-                v.x = col64[od.base_c64 + idx];
-                v.y = col64[od.base_c64 + n + idx];
-                reinterpret_cast<double2*>(col64 + od.xy)[p] = v;
-            }
-            continue;
+    const int64_t sp0 = sg.sbase + it.off, p0 = it.pos0 + it.off;
+    int idx[PER];
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+        const int t = k * ENM_SORT_THREADS + (int)threadIdx.x;
+        idx[k] = t < len ? sidx[sp0 + t] : 0;
+    }
+    auto in = [&](int k) { return k * ENM_SORT_THREADS + (int)threadIdx.x < len; };
+    auto pos = [&](int k) { return p0 + k * ENM_SORT_THREADS + (int)threadIdx.x; };
+    if (od.base_perm >= 0) {
+        // idx is a base-order position in the same strip: read locally (the
+        // base order's magnitudes are already folded in).
+        {
+            int pv[PER];
+#pragma unroll
+            for (int k = 0; k < PER; ++k) pv[k] = in(k) ? perm[od.base_perm + idx[k]] : 0;
+#pragma unroll
+            for (int k = 0; k < PER; ++k)
+                if (in(k)) perm[od.perm + pos(k)] = pv[k];
         }
-        perm[od.perm + p] = idx;
-        const double key = skey[sp];
-        col64[od.c64 + p] = key;
-        if (with32) col32[od.c32 + p] = (float)key;
-        m = fmax(m, fabs(key));
-        for (int c = 1; c < od.ncol; ++c) {
-            const double v = arena[od.src[c] + idx];
-            col64[od.c64 + c * n + p] = v;
-            if (with32) col32[od.c32 + c * n + p] = (float)v;
-            m = fmax(m, fabs(v));
+        for (int c = 0; c < (od.xy_only ? 0 : od.ncol); ++c) {
+            double v[PER];
+#pragma unroll
+            for (int k = 0; k < PER; ++k) v[k] = in(k) ? col64[od.base_c64 + c * n + idx[k]] : 0.0;
+#pragma unroll
+            for (int k = 0; k < PER; ++k)
+                if (in(k)) col64[od.c64 + c * n + pos(k)] = v[k];
+            if (with32) {
+                float f[PER];
+#pragma unroll
+                for (int k = 0; k < PER; ++k) f[k] = in(k) ? col32[od.base_c32 + c * n + idx[k]] : 0.0f;
+#pragma unroll
+                for (int k = 0; k < PER; ++k)
+                    if (in(k)) col32[od.c32 + c * n + pos(k)] = f[k];
+            }
+        }
+        if (od.xy >= 0) {
+            double2 v[PER];
+#pragma unroll
+            for (int k = 0; k < PER; ++k) {
+                v[k].x = in(k) ? col64[od.base_c64 + idx[k]] : 0.0;
+                v[k].y = in(k) ? col64[od.base_c64 + n + idx[k]] : 0.0;
+            }
+#pragma unroll
+            for (int k = 0; k < PER; ++k)
+                if (in(k)) reinterpret_cast<double2*>(col64 + od.xy)[pos(k)] = v[k];
+        }
+        return;
+    }
+    double m = 0.0;
+    {
+        double key[PER];
+#pragma unroll
+        for (int k = 0; k < PER; ++k) key[k] = in(k) ? skey[sp0 + k * ENM_SORT_THREADS + threadIdx.x] : 0.0;
+#pragma unroll
+        for (int k = 0; k < PER; ++k) {
+            if (in(k)) {
+                perm[od.perm + pos(k)] = idx[k];
+                col64[od.c64 + pos(k)] = key[k];
+                if (with32) col32[od.c32 + pos(k)] = (float)key[k];
+                m = fmax(m, fabs(key[k]));
+            }
         }
     }
+    for (int c = 1; c < od.ncol; ++c) {
+        const double* src = arena + od.src[c];
+        double v[PER];
+#pragma unroll
+        for (int k = 0; k < PER; ++k) v[k] = in(k) ? src[idx[k]] : 0.0;
+#pragma unroll
+        for (int k = 0; k < PER; ++k) {
+            if (in(k)) {
+                col64[od.c64 + c * n + pos(k)] = v[k];
+                if (with32) col32[od.c32 + c * n + pos(k)] = (float)v[k];
+                m = fmax(m, fabs(v[k]));
+            }
+        }
+    }
+    // One atomic per block (atomics from every wave on one address queue
+    // behind each other); |v| >= 0, so the IEEE bit patterns order like
+    // the values.
+    __shared__ double red[ENM_SORT_THREADS / ENM_WAVE];
     m = wave_max(m);
     if ((threadIdx.x & (ENM_WAVE - 1)) == 0) red[threadIdx.x / ENM_WAVE] = m;
     __syncthreads();
     if (threadIdx.x == 0) {
         double mm = red[0];
         for (int w = 1; w < ENM_SORT_THREADS / ENM_WAVE; ++w) mm = fmax(mm, red[w]);
-        // |v| >= 0, so the IEEE bit patterns order like the values.
         atomicMax(maxabs + od.inst, (unsigned long long)__double_as_longlong(mm));
     }
 }
@@ -812,6 +890,63 @@ struct SsLayout {
     int with32;
 };
 
+// Lays out elements [0, len) of a sorted run into the order `od` from
+// position p0 on: element t has key key_at(t) and point index idx_at(t),
+// and the order's other columns are gathered from the arena.  A thread
+// takes elements threadIdx.x + NT * k, PER at a time, and issues all its
+// gathers of a column before it stores any of them (a store between two
+// gathers would hold the second one back: the gathers may alias the
+// stores as far as the compiler knows).  Returns the largest |value|.
+template <int NT, int PER, class K, class I>
+__device__ __forceinline__ double layout_elems(const SsLayout& L, const OrderDev& od, int64_t p0, int len,
+                                               K key_at, I idx_at) {
+    const int64_t n = od.n;
+    const int ncol = od.ncol;
+    double m = 0.0;
+    for (int base = 0; base < len; base += NT * PER) {
+        int idx[PER];
+        double key[PER];
+#pragma unroll
+        for (int k = 0; k < PER; ++k) {
+            const int t = base + k * NT + (int)threadIdx.x;
+            const bool in = t < len;
+            idx[k] = in ? idx_at(t) : 0;
+            key[k] = in ? key_at(t) : 0.0;
+        }
+#pragma unroll
+        for (int k = 0; k < PER; ++k) {
+            const int t = base + k * NT + (int)threadIdx.x;
+            if (t < len) {
+                const int64_t p = p0 + t;
+                L.perm[od.perm + p] = idx[k];
+                L.col64[od.c64 + p] = key[k];
+                if (L.with32) L.col32[od.c32 + p] = (float)key[k];
+                m = fmax(m, fabs(key[k]));
+            }
+        }
+        for (int cc = 1; cc < ncol; ++cc) {
+            const double* src = L.arena + od.src[cc];
+            double v[PER];
+#pragma unroll
+            for (int k = 0; k < PER; ++k) {
+                const int t = base + k * NT + (int)threadIdx.x;
+                v[k] = t < len ? src[idx[k]] : 0.0;
+            }
+#pragma unroll
+            for (int k = 0; k < PER; ++k) {
+                const int t = base + k * NT + (int)threadIdx.x;
+                if (t < len) {
+                    const int64_t p = p0 + t;
+                    L.col64[od.c64 + cc * n + p] = v[k];
+                    if (L.with32) L.col32[od.c32 + cc * n + p] = (float)v[k];
+                    m = fmax(m, fabs(v[k]));
+                }
+            }
+        }
+    }
+    return m;
+}
+
 template <bool LDS>
 __device__ void ss_layout_run(const SsLayout& L, const SsTile& tl, const double* kk, const int* ii,
                               double* red) {
@@ -877,13 +1012,33 @@ void ss_hint_kernel(const SortSeg* __restrict__ segs, int nsegs, const int* __re
     const int len = min(ENM_SORT_TILE, sg.len - off);
     const int* h = hint + sg.src;
     if (threadIdx.x == 0) s_bad = h[0] < 0 ? 1 : 0;
-    // Slot t holds the element at position off - 1 + t (-1: none or out of range).
-    for (int t = threadIdx.x; t <= len; t += ENM_SORT_THREADS) {
-        const int p = off - 1 + t;
-        const int i = p >= 0 ? h[p] : -1;
-        const bool in = i >= 0 && i < sg.len;
-        s_idx[t] = in ? i : -1;
-        s_key[t] = in ? L.arena[sg.src + i] : 0.0;
+    // Slot t holds the element at position off - 1 + t (-1: none or out of
+    // range).  All of a thread's hint loads go out first, then all its key
+    // gathers.
+    {
+        constexpr int PER = ENM_SORT_TILE / ENM_SORT_THREADS + 1;
+        int hi[PER];
+#pragma unroll
+        for (int k = 0; k < PER; ++k) {
+            const int t = k * ENM_SORT_THREADS + (int)threadIdx.x;
+            const int p = off - 1 + t;
+            hi[k] = (t <= len && p >= 0) ? h[p] : -1;
+        }
+        double hk[PER];
+#pragma unroll
+        for (int k = 0; k < PER; ++k) {
+            const bool in = hi[k] >= 0 && hi[k] < sg.len;
+            hk[k] = in ? L.arena[sg.src + hi[k]] : 0.0;
+        }
+#pragma unroll
+        for (int k = 0; k < PER; ++k) {
+            const int t = k * ENM_SORT_THREADS + (int)threadIdx.x;
+            if (t <= len) {
+                const bool in = hi[k] >= 0 && hi[k] < sg.len;
+                s_idx[t] = in ? hi[k] : -1;
+                s_key[t] = hk[k];
+            }
+        }
     }
     __syncthreads();
     bool bad = false;
@@ -906,24 +1061,12 @@ void ss_hint_kernel(const SortSeg* __restrict__ segs, int nsegs, const int* __re
     for (int c = 0; c < q.ncons; ++c) {
         const SsCons cn = L.cons[q.cons0 + c];
         const OrderDev& od = L.orders[cn.order];
-        const int64_t n = od.n;
-        const int ncol = od.ncol;
-        double m = 0.0;
-        for (int t = threadIdx.x; t < len; t += ENM_SORT_THREADS) {
-            const int64_t p = cn.pos0 + off + t;
-            const int i = s_idx[t + 1];
-            const double key = s_key[t + 1];
-            L.perm[od.perm + p] = sg.p0 + i;
-            L.col64[od.c64 + p] = key;
-            if (L.with32) L.col32[od.c32 + p] = (float)key;
-            m = fmax(m, fabs(key));
-            for (int cc = 1; cc < ncol; ++cc) {
-                const double v = L.arena[od.src[cc] + sg.p0 + i];
-                L.col64[od.c64 + cc * n + p] = v;
-                if (L.with32) L.col32[od.c32 + cc * n + p] = (float)v;
-                m = fmax(m, fabs(v));
-            }
-        }
+        const int p0 = sg.p0;
+        double m = layout_elems<ENM_SORT_THREADS, ENM_SORT_TILE / ENM_SORT_THREADS>(
+            L, od, cn.pos0 + off, len, [&](int t) { return s_key[t + 1]; },
+            [&](int t) { return p0 + s_idx[t + 1]; });
+        // One atomic per block: atomics from every wave on one address
+        // queue behind each other.
         m = wave_max(m);
         if ((threadIdx.x & (ENM_WAVE - 1)) == 0) red[threadIdx.x / ENM_WAVE] = m;
         __syncthreads();
