@@ -468,13 +468,14 @@ void prep_scale_kernel(const PrepTaskDev* __restrict__ tasks, int64_t stride, co
     const int64_t i = (b - ti * stride) * PB_THREADS + threadIdx.x;
     if (i >= t.n) return;
     double* col = arena + t.arena_off;
-    // x and y loaded together before either is stored (a store between the
-    // loads would hold the second one back).
-    const bool sx = !(t.flags & 1u), sy = !(t.flags & 2u);
-    const double vx = sx ? col[i] : 0.0, nx = sx ? noise[t.noise_x + i] : 0.0;
-    const double vy = sy ? col[t.n + i] : 0.0, ny = sy ? noise[t.noise_y + i] : 0.0;
-    if (sx) col[i] = (vx - stats[t.stat]) / stats[t.stat + 1] + nx;
-    if (sy) col[t.n + i] = (vy - stats[t.stat + 2]) / stats[t.stat + 3] + ny;
+    if (!(t.flags & 1u)) {
+        const double v = (col[i] - stats[t.stat]) / stats[t.stat + 1];
+        col[i] = v + noise[t.noise_x + i];
+    }
+    if (!(t.flags & 2u)) {
+        const double v = (col[t.n + i] - stats[t.stat + 2]) / stats[t.stat + 3];
+        col[t.n + i] = v + noise[t.noise_y + i];
+    }
     if (!(t.flags & 4u)) {
         for (int c = 0; c < t.dz; ++c) {
             const double mean = stats[t.stat + 4 + 2 * c], sd = stats[t.stat + 5 + 2 * c];
