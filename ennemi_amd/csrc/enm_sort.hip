@@ -322,11 +322,12 @@ void strip_sort_kernel(const GatherItem* __restrict__ items, const SortSeg* __re
         }
     }
     if (od.xy >= 0) {
+        // Column 1 is the sort key: its sorted values are in LDS already.
         double2 v[PER];
 #pragma unroll
         for (int k = 0; k < PER; ++k) {
             v[k].x = in(k) ? col64[od.base_c64 + idx[k]] : 0.0;
-            v[k].y = in(k) ? col64[od.base_c64 + n + idx[k]] : 0.0;
+            v[k].y = in(k) ? sk[lp(k * ENM_SORT_THREADS + (int)threadIdx.x)] : 0.0;
         }
 #pragma unroll
         for (int k = 0; k < PER; ++k)
