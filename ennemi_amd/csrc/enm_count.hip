@@ -105,9 +105,11 @@ __device__ __forceinline__ int global_left(const double* __restrict__ v, int lo,
 }
 
 // 1-D counts.  An item is ENM_C1_ITEM consecutive positions of a key-sorted
-// order, ENM_C1_Q per thread of a 256-thread workgroup.  The workgroup
+// order, ENM_C1_Q per thread of a 512-thread workgroup.  The workgroup
 // copies the keys of the item and ENM_C1_HALO positions on each side into
-// LDS with coalesced loads (40 KB, shared by its four waves), and each
+// LDS with coalesced loads (40 KB, shared by its eight waves: four
+// workgroups fill a CU's 32 wave slots; 256 threads with eight queries
+// each left half of them empty and took 18 % longer), and each
 // query binary-searches both boundaries there.  A query whose ball reaches
 // past the staged keys on either side (its predicate still holds at the
 // window's edge, about 1 % of queries on Gaussian-like data, but in almost
@@ -651,7 +653,7 @@ hipError_t enm_launch_count(const Item* items, const int* qoff, int n_items, uns
     const int chunk = std::max(1, std::min(ticket > 0 ? ticket : ENM_KCHUNK, n_items / (4 * std::max(1, grid))));
     CountArgs a{items, qoff, counter, chunk, descs, orders, maxabs, perm, psi_tab, brute ? 1 : 0};
     if (dclass == 1) {
-        // `grid` counts waves; a 1-D workgroup has four.
+        // `grid` counts waves; a 1-D workgroup has ENM_C1_THREADS / 64.
         const int blocks = std::max(1, std::min(n_items, grid / (ENM_C1_THREADS / ENM_WAVE)));
         hipLaunchKernelGGL(count1d_kernel, dim3(blocks), dim3(ENM_C1_THREADS), 0, st, a, col64, res64, res32,
                            stats, deferred, n_deferred, seg_cap);

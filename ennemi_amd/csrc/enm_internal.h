@@ -22,8 +22,8 @@
 
 #define ENM_TILE 64            // queries per work item (one per lane of a wave64)
 #define ENM_NQ 8               // work queues per persistent launch (one per XCD)
-#define ENM_C1_THREADS 256     // threads of a 1-D count workgroup
-#define ENM_C1_Q 8             // queries per thread of a 1-D count item
+#define ENM_C1_THREADS 512     // threads of a 1-D count workgroup
+#define ENM_C1_Q 4             // queries per thread of a 1-D count item
 #define ENM_C1_ITEM (ENM_C1_THREADS * ENM_C1_Q)   // positions per 1-D count item (2048)
 #define ENM_C1_HALO 1528       // keys staged on each side of a 1-D count item
 #define ENM_C1_CAP (ENM_C1_ITEM + 2 * ENM_C1_HALO)  // LDS window (doubles, 40 KB: four workgroups per CU)
