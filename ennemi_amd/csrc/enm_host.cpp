@@ -146,7 +146,9 @@ struct Plan {
     bool need32 = false;
     // device
     int64_t heap_bytes = 0;   // global heap lists of kNN groups with k+1 > ENM_LIST_LDS_MAX
-    DevBuf tables, sk0, si0, sk1, si1, split, col64, col32, perm, res64, res32, maxabs, counters, stats,
+    int64_t far_bytes = 0;    // far list of the strip kNN (queries reaching past the staged strips)
+    DevBuf far,
+        tables, sk0, si0, sk1, si1, split, col64, col32, perm, res64, res32, maxabs, counters, stats,
         deferred, heap, samp_k0, samp_k1, samp_i0, samp_i1, samp_split, bid, ss_counts, ss_tiles, ss_long, ss_ctl,
         hint_bad;
     size_t off_segs = 0, off_gitems = 0, off_orders = 0, off_kdescs = 0, off_cdescs = 0, off_insts = 0,
@@ -799,6 +801,14 @@ int plan_batch(enm_ctx* ctx, Plan& P, const enm_instance* inst, int n_inst, cons
                 const int hg = enm_knn_heap_grid(grid_for(ctx, g.waves, g.n_items), g.k1max);
                 P.heap_bytes = std::max<int64_t>(P.heap_bytes, (int64_t)hg * ENM_TILE * g.k1max * 8);
             }
+            if (!is_count && g.walk == 2) {
+                // Room for one query in 16 of the group (about 1 % reach past
+                // their neighbours on the lag scans), at most 64 MiB; the
+                // groups run one after the other and share it.
+                const int64_t rec = 16 + 8 * (int64_t)g.pclass;
+                const int64_t want = (int64_t)g.n_items * g.smax / 16 * rec;
+                P.far_bytes = std::max<int64_t>(P.far_bytes, std::min<int64_t>(want, (int64_t)64 << 20));
+            }
             const size_t blk = std::max<size_t>(64, (j - i + 4 * ENM_NQ - 1) / (4 * ENM_NQ));
             struct Block { int64_t cost; size_t a, b; };
             std::vector<Block> blocks;
@@ -974,6 +984,7 @@ std::vector<BufSpec> workspace_layout(Plan& P) {
     v.push_back({&P.deferred, (size_t)P.deferred_seg * ENM_NDEF * 4, "deferred queries"});
     v.push_back({&P.stats, ST_N * 8, "stats"});
     if (P.heap_bytes > 0) v.push_back({&P.heap, (size_t)P.heap_bytes, "heap lists"});
+    if (P.far_bytes > 0) v.push_back({&P.far, (size_t)P.far_bytes, "strip kNN far list"});
     if (P.sample_sort) {
         v.push_back({&P.samp_k0, (size_t)P.n_samp * 8 + 8, "samples"});
         v.push_back({&P.samp_k1, (size_t)P.n_samp * 8 + 8, "samples"});
@@ -1166,8 +1177,10 @@ int enqueue(enm_ctx* ctx, Plan& P, const double* arena, const int* hint, double*
                                tab<KnnDev>(P, P.off_kdescs), orders, maxabs, P.perm.as<int>(),
                                P.col64.as<double>(), P.col32.as<float>(), P.res64.as<double>(), stats,
                                g.dclass, g.pclass, P.f32 && P.brute, P.brute, g.walk, grid,
-                               g.k1max, P.heap.as<double>(), g.ticket, g.smax, st);
-            ++nl;
+                               g.k1max, P.heap.as<double>(), g.ticket, g.smax,
+                               g.walk == 2 && P.far_bytes > 0 ? P.far.as<char>() : nullptr,
+                               counters + g.counter + ENM_NQ * ENM_CSTRIDE, P.far_bytes, st);
+            nl += g.walk == 2 && P.far_bytes > 0 ? 2 : 1;
             return chk(e, "knn");
         }
         e = enm_launch_count(tab<Item>(P, P.off_citems) + g.first, qoffs + g.qoff, g.n_items,

@@ -63,6 +63,21 @@ struct KnnArgs {
     double* heap;       // global heap lists (list class ENM_LIST_HEAP_GLOBAL): per block
     int64_t heap_stride;  // doubles per block in `heap` (64 lanes x the group's largest k+1)
     int smax;           // strip kernel: the group's largest strip (LDS slot size, points)
+    // Strip kernel: queries whose radius still reaches past the staged
+    // neighbours are listed here (KnnFar records, as many as far_bytes
+    // hold; the others are finished in place) and finished by
+    // knn_far_kernel; NULL: all finished in place.
+    char* far;
+    unsigned* far_count;  // zeroed before the launch
+    int64_t far_bytes;
+};
+
+// A strip-kNN query whose radius reaches past its staged neighbours: its
+// item and grid-order position (knn_far_kernel walks it again from the
+// start, in global memory).
+struct KnnFar {
+    int32_t item;
+    int32_t i;
 };
 
 // Launch of the kNN kernels of one joint dimension D (one translation unit
@@ -895,8 +910,10 @@ __device__ __forceinline__ void far_strips(L& list, double& T, const double2* __
     }
 }
 
+// Four-slot lists are held to 80 VGPRs, six waves per SIMD (a few bytes
+// spill: the far-list bookkeeping would otherwise cost a wave per SIMD).
 template <int P>
-__global__ __launch_bounds__(SK_THREADS_MAX)
+__global__ __launch_bounds__(SK_THREADS_MAX, P == 4 ? 6 : 1)
 void knn_strip_kernel(KnnArgs A, const double* __restrict__ col64, double* __restrict__ res64,
                       unsigned long long* __restrict__ stats) {
     extern __shared__ double2 sxy[];   // three slots of A.smax pairs: strips s-1, s, s+1
@@ -921,9 +938,12 @@ void knn_strip_kernel(KnnArgs A, const double* __restrict__ col64, double* __res
             slen[k] = (ss < 0 || ss >= nstrip) ? 0 : min(hi, slo[k] + S) - slo[k];
             for (int t = threadIdx.x; t < slen[k]; t += NT) sxy[k * A.smax + t] = xy[slo[k] + t];
         }
-        // The x bounds of the neighbours: largest x left, smallest x right.
-        const double xl = slen[0] > 0 ? xk[slo[1] - 1] : 0.0;
-        const double xr = slen[2] > 0 ? xk[slo[2]] : 0.0;
+        // The x bounds of the neighbours: largest x left, smallest x right;
+        // and of the strips beyond them.
+        const double xl = uni_f64(slen[0] > 0 ? xk[slo[1] - 1] : 0.0);
+        const double xr = uni_f64(slen[2] > 0 ? xk[slo[2]] : 0.0);
+        const double xl2 = uni_f64(s >= 2 ? xk[slo[0] - 1] : 0.0);
+        const double xr2 = uni_f64(s + 2 < nstrip ? xk[lo + (s + 2) * S] : 0.0);
         __syncthreads();
         const double2* own = sxy + A.smax;
         // Two queries per thread, NT apart, walked in lockstep.
@@ -964,17 +984,48 @@ void knn_strip_kernel(KnnArgs A, const double* __restrict__ col64, double* __res
                 run_walk2(l0, a, l1, b, w, slen[2], visited);
             }
             // Further strips in global memory (a side is done once its
-            // nearer strip was out of reach).
-            far_strips(l0, a.T, xy, xk, lo, hi, S, s, nstrip, a.q, !la, !ra, visited);
-            if (act1) far_strips(l1, b.T, xy, xk, lo, hi, S, s, nstrip, b.q, !lb, !rb, visited);
+            // nearer strip was out of reach).  A query that needs them goes
+            // to the far list, so that its long walk through global memory
+            // holds up neither its wave nor, at the next barrier, its
+            // workgroup; knn_far_kernel finishes it with every lane busy.
+            const bool fa = (la && s >= 2 && a.q.x - xl2 < a.T) || (ra && s + 2 < nstrip && xr2 - a.q.x < a.T);
+            const bool fb = act1 && ((lb && s >= 2 && b.q.x - xl2 < b.T) ||
+                                     (rb && s + 2 < nstrip && xr2 - b.q.x < b.T));
+            bool da = false, db = false;
+            if (A.far != nullptr) {
+                const unsigned long long ma = __builtin_amdgcn_ballot_w64(fa);
+                const unsigned long long mb = __builtin_amdgcn_ballot_w64(fb);
+                const unsigned tot = (unsigned)(__popcll(ma) + __popcll(mb));
+                if (tot != 0) {
+                    const int lane = (int)(threadIdx.x & (ENM_WAVE - 1));
+                    const int leader = __ffsll((long long)__builtin_amdgcn_read_exec()) - 1;
+                    unsigned base = 0;
+                    if (lane == leader) base = atomicAdd(A.far_count, tot);
+                    base = (unsigned)__shfl((int)base, leader, ENM_WAVE);
+                    const unsigned ra_ = __builtin_amdgcn_mbcnt_hi((unsigned)(ma >> 32),
+                                                                   __builtin_amdgcn_mbcnt_lo((unsigned)ma, 0u));
+                    const unsigned rb_ = __builtin_amdgcn_mbcnt_hi((unsigned)(mb >> 32),
+                                                                   __builtin_amdgcn_mbcnt_lo((unsigned)mb, 0u));
+                    const unsigned sa = base + ra_, sb = base + (unsigned)__popcll(ma) + rb_;
+                    KnnFar* far = reinterpret_cast<KnnFar*>(A.far);
+                    const int64_t cap = A.far_bytes / (int64_t)sizeof(KnnFar);
+                    da = fa && (int64_t)sa < cap;
+                    db = fb && (int64_t)sb < cap;
+                    if (da) far[sa] = KnnFar{idx, i0};
+                    if (db) far[sb] = KnnFar{idx, i1};
+                }
+            }
+            if (fa && !da) far_strips(l0, a.T, xy, xk, lo, hi, S, s, nstrip, a.q, !la, !ra, visited);
+            if (fb && !db) far_strips(l1, b.T, xy, xk, lo, hi, S, s, nstrip, b.q, !lb, !rb, visited);
             const double e0 = l0.top(), e1 = l1.top();
+            const bool w0 = !da, w1 = act1 && !db;
             if (kd.eps >= 0) {
-                res64[kd.eps + i0] = e0;
-                if (act1) res64[kd.eps + i1] = e1;
+                if (w0) res64[kd.eps + i0] = e0;
+                if (w1) res64[kd.eps + i1] = e1;
             }
             if (kd.eps_orig >= 0) {
-                res64[kd.eps_orig + A.perm[od.perm + i0]] = e0;
-                if (act1) res64[kd.eps_orig + A.perm[od.perm + i1]] = e1;
+                if (w0) res64[kd.eps_orig + A.perm[od.perm + i0]] = e0;
+                if (w1) res64[kd.eps_orig + A.perm[od.perm + i1]] = e1;
             }
             pairs += visited;
         }
@@ -982,6 +1033,63 @@ void knn_strip_kernel(KnnArgs A, const double* __restrict__ col64, double* __res
     });
     pairs = wave_sum_u64(pairs);
     if ((threadIdx.x & (ENM_WAVE - 1)) == 0) atomicAdd(stats + ST_KNN_PAIRS, pairs);
+}
+
+// The far list of a knn_strip_kernel launch, one query per thread: its
+// walk again from the start, by itself in global memory (its own strip
+// from its own position, the neighbours its radius reaches, then the
+// strips beyond), and its radius written as the strip kernel would have.
+// The candidates are those of the strip kernel's walk, so the radius is the
+// same.
+template <int P>
+__global__ __launch_bounds__(256)
+void knn_far_kernel(KnnArgs A, const double* __restrict__ col64, double* __restrict__ res64,
+                    unsigned long long* __restrict__ stats) {
+    const int64_t cap = A.far_bytes / (int64_t)sizeof(KnnFar);
+    const unsigned n = (unsigned)min((int64_t)*A.far_count, cap);
+    const KnnFar* far = reinterpret_cast<const KnnFar*>(A.far);
+    unsigned long long pairs = 0;
+    for (unsigned r = blockIdx.x * blockDim.x + threadIdx.x; r < n; r += gridDim.x * blockDim.x) {
+        const KnnFar f = far[r];
+        const Item it = A.items[f.item];
+        const KnnDev& kd = A.descs[it.desc];
+        const OrderDev& od = A.orders[kd.order];
+        const int S = kd.strip;
+        const double2* __restrict__ xy = reinterpret_cast<const double2*>(col64 + od.xy);
+        const double* __restrict__ xk = col64 + kd.base_key;
+        const int lo = it.lo, hi = it.hi;
+        const int s = (it.t0 - lo) / S;
+        const int nstrip = (hi - lo + S - 1) / S;
+        const int i = f.i;
+        const double2 q = xy[i];
+        RegList<P> L;
+        L.init(kd.k + 1, nullptr);
+        L.insert(0.0);   // the query itself
+        double T = L.top();
+        unsigned visited = 1;
+        const int ss = lo + s * S, se = min(hi, ss + S);
+        run_walk(L, T, xy + ss, se - ss, i - ss + 1, i - ss - 1, q.x, q.y, visited);
+        const bool la = s >= 1 && q.x - xk[ss - 1] < T;
+        if (la) {
+            const double2* w = xy + ss - S;
+            const int p = slot_lower_bound(w, S, q.y);
+            run_walk(L, T, w, S, p, p - 1, q.x, q.y, visited);
+        }
+        const bool ra = s + 1 < nstrip && xk[se] - q.x < T;
+        if (ra) {
+            const int len = min(hi, se + S) - se;
+            const double2* w = xy + se;
+            const int p = slot_lower_bound(w, len, q.y);
+            run_walk(L, T, w, len, p, p - 1, q.x, q.y, visited);
+        }
+        far_strips(L, T, xy, xk, lo, hi, S, s, nstrip, q, !la, !ra, visited);
+        const double e = L.top();
+        if (kd.eps >= 0) res64[kd.eps + i] = e;
+        if (kd.eps_orig >= 0) res64[kd.eps_orig + A.perm[od.perm + i]] = e;
+        pairs += visited;
+    }
+    pairs = wave_sum_u64(pairs);
+    if ((threadIdx.x & (ENM_WAVE - 1)) == 0 && pairs) atomicAdd(stats + ST_KNN_PAIRS, pairs);
 }
 
 template <int D, int P, bool F32>
@@ -1027,6 +1135,9 @@ hipError_t launch_dp(const KnnArgs& a, bool f32, int walk, int grid, const doubl
                     const int nt = a.smax >= SK_THREADS_MAX ? SK_THREADS_MAX : ((a.smax + 63) / 64) * 64;
                     const unsigned lds = (unsigned)(3 * a.smax * sizeof(double2));
                     hipLaunchKernelGGL((knn_strip_kernel<P>), dim3(grid), dim3(nt), lds, st, a, col64, res64, stats);
+                    if (a.far != nullptr)
+                        hipLaunchKernelGGL((knn_far_kernel<P>), dim3(std::max(1, grid)), dim3(256), 0, st, a, col64,
+                                           res64, stats);
                     return hipGetLastError();
                 }
             }

@@ -25,13 +25,14 @@ hipError_t enm_launch_knn(const Item* items, const int* qoff, int n_items, unsig
                           const unsigned long long* maxabs, const int* perm,
                           const double* col64, const float* col32, double* res64,
                           unsigned long long* stats, int dclass, int pclass, bool f32, bool brute,
-                          int walk, int grid, int k1max, double* heap, int ticket, int smax, hipStream_t st) {
+                          int walk, int grid, int k1max, double* heap, int ticket, int smax, char* far,
+                          unsigned* far_count, int64_t far_bytes, hipStream_t st) {
     if (n_items <= 0) return hipSuccess;
     // Tickets of up to ENM_KCHUNK items (or the tuned items per ticket), but
     // at least four per wave of the grid, so small launches still fill it.
     const int chunk = std::max(1, std::min(ticket > 0 ? ticket : ENM_KCHUNK, n_items / (4 * std::max(1, grid))));
     KnnArgs a{items, qoff, counter, chunk, descs, orders, maxabs, perm, brute ? 1 : 0,
-              heap, (int64_t)ENM_WAVE * k1max, smax};
+              heap, (int64_t)ENM_WAVE * k1max, smax, far, far_count, far_bytes};
     if (pclass == ENM_LIST_HEAP_GLOBAL && heap == nullptr) return hipErrorInvalidValue;
     switch (dclass) {
         case 1: return enm_knn_launch_d1(a, pclass, f32, walk, grid, col64, col32, res64, stats, st);

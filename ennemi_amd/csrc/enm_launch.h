@@ -65,7 +65,8 @@ hipError_t enm_launch_knn(const Item* items, const int* qoff, int n_items, unsig
                           const unsigned long long* maxabs, const int* perm,
                           const double* col64, const float* col32, double* res64,
                           unsigned long long* stats, int dclass, int pclass, bool f32, bool brute,
-                          int walk, int grid, int k1max, double* heap, int ticket, int smax, hipStream_t st);
+                          int walk, int grid, int k1max, double* heap, int ticket, int smax, char* far,
+                          unsigned* far_count, int64_t far_bytes, hipStream_t st);
 // Grid of a heap-list kNN launch (ENM_LIST_HEAP_GLOBAL) whose heaps fit
 // ENM_HEAP_GLOBAL_BYTES: at most `grid` blocks of 64 x k1max doubles.
 int enm_knn_heap_grid(int grid, int k1max);
