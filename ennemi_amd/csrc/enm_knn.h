@@ -919,7 +919,7 @@ void knn_strip_kernel(KnnArgs A, const double* __restrict__ col64, double* __res
     extern __shared__ double2 sxy[];   // three slots of A.smax pairs: strips s-1, s, s+1
     __shared__ unsigned s_ticket;
     const int NT = (int)blockDim.x;
-    unsigned long long pairs = 0;
+    unsigned pairs = 0;   // this thread's candidates (a few thousand): one register
     for_each_block_item(A.qoff, A.counter, &s_ticket, [&](int idx) {
         const Item it = A.items[idx];
         const KnnDev kd = A.descs[it.desc];
@@ -1031,8 +1031,8 @@ void knn_strip_kernel(KnnArgs A, const double* __restrict__ col64, double* __res
         }
         __syncthreads();   // the slots are restaged by the next item
     });
-    pairs = wave_sum_u64(pairs);
-    if ((threadIdx.x & (ENM_WAVE - 1)) == 0) atomicAdd(stats + ST_KNN_PAIRS, pairs);
+    const unsigned long long total = wave_sum_u64((unsigned long long)pairs);
+    if ((threadIdx.x & (ENM_WAVE - 1)) == 0) atomicAdd(stats + ST_KNN_PAIRS, total);
 }
 
 // The far list of a knn_strip_kernel launch, one query per thread: its
