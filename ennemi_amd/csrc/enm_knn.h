@@ -1135,9 +1135,11 @@ hipError_t launch_dp(const KnnArgs& a, bool f32, int walk, int grid, const doubl
                     const int nt = a.smax >= SK_THREADS_MAX ? SK_THREADS_MAX : ((a.smax + 63) / 64) * 64;
                     const unsigned lds = (unsigned)(3 * a.smax * sizeof(double2));
                     hipLaunchKernelGGL((knn_strip_kernel<P>), dim3(grid), dim3(nt), lds, st, a, col64, res64, stats);
+                    // The far list: eight 4-wave blocks per CU of the strip grid's
+                    // three (latency-bound walks need every wave slot).
                     if (a.far != nullptr)
-                        hipLaunchKernelGGL((knn_far_kernel<P>), dim3(std::max(1, grid)), dim3(256), 0, st, a, col64,
-                                           res64, stats);
+                        hipLaunchKernelGGL((knn_far_kernel<P>), dim3(std::max(1, grid) * 8 / 3 + 1), dim3(256), 0, st,
+                                           a, col64, res64, stats);
                     return hipGetLastError();
                 }
             }
