@@ -73,11 +73,15 @@ struct KnnArgs {
 };
 
 // A strip-kNN query whose radius reaches past its staged neighbours: its
-// item and grid-order position (knn_far_kernel walks it again from the
-// start, in global memory).
+// item, grid-order position, the sides still open (bit 0 left, bit 1
+// right) and its list so far; knn_far_kernel continues its walk.
+template <int P>
 struct KnnFar {
     int32_t item;
     int32_t i;
+    int32_t open;
+    int32_t pad;
+    double L[P];
 };
 
 // Launch of the kNN kernels of one joint dimension D (one translation unit
@@ -1007,12 +1011,26 @@ void knn_strip_kernel(KnnArgs A, const double* __restrict__ col64, double* __res
                     const unsigned rb_ = __builtin_amdgcn_mbcnt_hi((unsigned)(mb >> 32),
                                                                    __builtin_amdgcn_mbcnt_lo((unsigned)mb, 0u));
                     const unsigned sa = base + ra_, sb = base + (unsigned)__popcll(ma) + rb_;
-                    KnnFar* far = reinterpret_cast<KnnFar*>(A.far);
-                    const int64_t cap = A.far_bytes / (int64_t)sizeof(KnnFar);
+                    KnnFar<P>* far = reinterpret_cast<KnnFar<P>*>(A.far);
+                    const int64_t cap = A.far_bytes / (int64_t)sizeof(KnnFar<P>);
                     da = fa && (int64_t)sa < cap;
                     db = fb && (int64_t)sb < cap;
-                    if (da) far[sa] = KnnFar{idx, i0};
-                    if (db) far[sb] = KnnFar{idx, i1};
+                    if (da) {
+                        KnnFar<P>& f = far[sa];
+                        f.item = idx;
+                        f.i = i0;
+                        f.open = (la ? 1 : 0) | (ra ? 2 : 0);
+#pragma unroll
+                        for (int t = 0; t < P; ++t) f.L[t] = l0.L[t];
+                    }
+                    if (db) {
+                        KnnFar<P>& f = far[sb];
+                        f.item = idx;
+                        f.i = i1;
+                        f.open = (lb ? 1 : 0) | (rb ? 2 : 0);
+#pragma unroll
+                        for (int t = 0; t < P; ++t) f.L[t] = l1.L[t];
+                    }
                 }
             }
             if (fa && !da) far_strips(l0, a.T, xy, xk, lo, hi, S, s, nstrip, a.q, !la, !ra, visited);
@@ -1036,21 +1054,19 @@ void knn_strip_kernel(KnnArgs A, const double* __restrict__ col64, double* __res
 }
 
 // The far list of a knn_strip_kernel launch, one query per thread: its
-// walk again from the start, by itself in global memory (its own strip
-// from its own position, the neighbours its radius reaches, then the
-// strips beyond), and its radius written as the strip kernel would have.
-// The candidates are those of the strip kernel's walk, so the radius is the
-// same.
+// walk continues through the strips beyond its staged neighbours
+// (far_strips, from its list so far), and its radius is written as the
+// strip kernel would have.
 template <int P>
 __global__ __launch_bounds__(256)
 void knn_far_kernel(KnnArgs A, const double* __restrict__ col64, double* __restrict__ res64,
                     unsigned long long* __restrict__ stats) {
-    const int64_t cap = A.far_bytes / (int64_t)sizeof(KnnFar);
+    const int64_t cap = A.far_bytes / (int64_t)sizeof(KnnFar<P>);
     const unsigned n = (unsigned)min((int64_t)*A.far_count, cap);
-    const KnnFar* far = reinterpret_cast<const KnnFar*>(A.far);
+    const KnnFar<P>* far = reinterpret_cast<const KnnFar<P>*>(A.far);
     unsigned long long pairs = 0;
     for (unsigned r = blockIdx.x * blockDim.x + threadIdx.x; r < n; r += gridDim.x * blockDim.x) {
-        const KnnFar f = far[r];
+        const KnnFar<P>& f = far[r];
         const Item it = A.items[f.item];
         const KnnDev& kd = A.descs[it.desc];
         const OrderDev& od = A.orders[kd.order];
@@ -1060,29 +1076,13 @@ void knn_far_kernel(KnnArgs A, const double* __restrict__ col64, double* __restr
         const int lo = it.lo, hi = it.hi;
         const int s = (it.t0 - lo) / S;
         const int nstrip = (hi - lo + S - 1) / S;
-        const int i = f.i;
-        const double2 q = xy[i];
         RegList<P> L;
-        L.init(kd.k + 1, nullptr);
-        L.insert(0.0);   // the query itself
+#pragma unroll
+        for (int t = 0; t < P; ++t) L.L[t] = f.L[t];
         double T = L.top();
-        unsigned visited = 1;
-        const int ss = lo + s * S, se = min(hi, ss + S);
-        run_walk(L, T, xy + ss, se - ss, i - ss + 1, i - ss - 1, q.x, q.y, visited);
-        const bool la = s >= 1 && q.x - xk[ss - 1] < T;
-        if (la) {
-            const double2* w = xy + ss - S;
-            const int p = slot_lower_bound(w, S, q.y);
-            run_walk(L, T, w, S, p, p - 1, q.x, q.y, visited);
-        }
-        const bool ra = s + 1 < nstrip && xk[se] - q.x < T;
-        if (ra) {
-            const int len = min(hi, se + S) - se;
-            const double2* w = xy + se;
-            const int p = slot_lower_bound(w, len, q.y);
-            run_walk(L, T, w, len, p, p - 1, q.x, q.y, visited);
-        }
-        far_strips(L, T, xy, xk, lo, hi, S, s, nstrip, q, !la, !ra, visited);
+        const int i = f.i;
+        unsigned visited = 0;
+        far_strips(L, T, xy, xk, lo, hi, S, s, nstrip, xy[i], !(f.open & 1), !(f.open & 2), visited);
         const double e = L.top();
         if (kd.eps >= 0) res64[kd.eps + i] = e;
         if (kd.eps_orig >= 0) res64[kd.eps_orig + A.perm[od.perm + i]] = e;
