@@ -125,7 +125,9 @@ __global__ __launch_bounds__(ENM_C1_THREADS)
 void count1d_kernel(CountArgs A, const double* __restrict__ col64, const double* __restrict__ res64,
                     int* __restrict__ res32, unsigned long long* __restrict__ stats,
                     unsigned* __restrict__ deferred, unsigned* __restrict__ n_deferred, unsigned seg_cap) {
-    __shared__ double win[ENM_C1_CAP];
+    // Slot t + 1 holds key w0 + t; slots 0 and nw + 1 hold NaN, which fails
+    // both boundary predicates, so the searches need no range checks.
+    __shared__ double win[ENM_C1_CAP + 2];
     __shared__ unsigned s_ticket;
     const int tid = threadIdx.x;
     const int seg = (int)(blockIdx.x % ENM_NDEF);
@@ -142,9 +144,13 @@ void count1d_kernel(CountArgs A, const double* __restrict__ col64, const double*
         const int w0 = max(it.lo, it.t0 - ENM_C1_HALO);
         const int w1 = min(it.hi, it.t1 + ENM_C1_HALO);
         const int nw = w1 - w0;
-        for (int t = tid; t < nw; t += ENM_C1_THREADS) win[t] = v[w0 + t];
+        for (int t = tid; t < nw; t += ENM_C1_THREADS) win[t + 1] = v[w0 + t];
+        if (tid == 0) {
+            win[0] = __builtin_nan("");
+            win[nw + 1] = __builtin_nan("");
+        }
         __syncthreads();
-        const double wlo = win[0], whi = win[nw - 1];
+        const double wlo = win[1], whi = win[nw];
         // All radii first: their dependent gathers (permutation, then
         // radius) overlap instead of stalling every query in turn.
         double rq[ENM_C1_Q];
@@ -165,22 +171,26 @@ void count1d_kernel(CountArgs A, const double* __restrict__ col64, const double*
             const int i = it.t0 + tid + ENM_C1_THREADS * q;
             const double r = rq[q];
             const int p = i < it.t1 ? i - w0 : 0;
-            const double x = win[p];
+            const double x = win[p + 1];
             xq[q] = x;
             pr[q] = p;
             pl[q] = p;
             live[q] = i < it.t1 && !(r < 0.0);
             dq[q] = live[q] && ((w1 < it.hi && whi - x <= r) || (w0 > it.lo && x - wlo <= r));
         }
-        int top = 1;   // largest power of two below the window: the steps reach any slot
-        while (2 * top < ENM_C1_CAP) top *= 2;
+        // A query sits at most ENM_C1_ITEM + ENM_C1_HALO - 1 slots from
+        // either end of the window, and steps top, top / 2, ..., 1 reach
+        // any offset up to 2 * top - 1.
+        int top = 1;
+        while (2 * top - 1 < ENM_C1_ITEM + ENM_C1_HALO - 1) top *= 2;
         for (int step = top; step >= 1; step >>= 1) {
 #pragma unroll
             for (int q = 0; q < ENM_C1_Q; ++q) {
                 const int jr = pr[q] + step, jl = pl[q] - step;
-                const double vr = win[min(jr, nw - 1)], vl = win[max(jl, 0)];
-                if (jr < nw && vr - xq[q] <= rq[q]) pr[q] = jr;
-                if (jl >= 0 && xq[q] - vl <= rq[q]) pl[q] = jl;
+                // Past either end of the window the read is a NaN sentinel.
+                const double vr = win[min(jr, nw) + 1], vl = win[max(jl, -1) + 1];
+                if (vr - xq[q] <= rq[q]) pr[q] = jr;
+                if (xq[q] - vl <= rq[q]) pl[q] = jl;
             }
         }
 #pragma unroll
