@@ -104,6 +104,19 @@ __device__ __forceinline__ double vmax(double a, double b) {
     return r;
 }
 
+// max(a, |b|) of values that are never NaN: one instruction, the absolute
+// value taken by a source modifier.
+__device__ __forceinline__ double vmax_abs(double a, double b) {
+    double r;
+    asm("v_max_f64 %0, %1, |%2|" : "=v"(r) : "v"(a), "v"(b));
+    return r;
+}
+
+// Holds v in registers here: the load that produces it stays where it is
+// written, with its neighbours, instead of being moved into the branch that
+// first uses it, where it would wait out its own round trip.
+__device__ __forceinline__ void pin(double& v) { asm volatile("" : "+v"(v)); }
+
 // Insert v into the ascending list L[0..P) keeping the P smallest.  Slots
 // below P-K1 hold -inf, so L[P-1] is the K1-th smallest value seen.
 template <int P>

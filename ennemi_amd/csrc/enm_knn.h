@@ -788,11 +788,14 @@ __device__ __forceinline__ void run_walk(L& list, double& T, const double2* __re
                                          double qx, double qy, unsigned& visited) {
     bool up = u < len, down = dn >= 0;
     while (up || down) {
+        // Both next candidates are read together (a finished direction
+        // rereads the other's), so their round trips overlap.
+        double2 cu = w[up ? u : dn], cd = w[down ? dn : u];
+        pin(cu.x); pin(cu.y); pin(cd.x); pin(cd.y);
         if (up) {
-            const double2 c = w[u];
-            const double dy = c.y - qy;
+            const double dy = cu.y - qy;
             if (dy < T) {
-                list.insert(fmax(dy, fabs(c.x - qx)));
+                list.insert(vmax_abs(dy, cu.x - qx));
                 T = list.top();
                 ++visited;
                 up = ++u < len;
@@ -801,10 +804,9 @@ __device__ __forceinline__ void run_walk(L& list, double& T, const double2* __re
             }
         }
         if (down) {
-            const double2 c = w[dn];
-            const double dy = qy - c.y;
+            const double dy = qy - cd.y;
             if (dy < T) {
-                list.insert(fmax(dy, fabs(c.x - qx)));
+                list.insert(vmax_abs(dy, cd.x - qx));
                 T = list.top();
                 ++visited;
                 down = --dn >= 0;
@@ -834,12 +836,14 @@ __device__ __forceinline__ void run_walk2(L& l0, WalkQ& a, L& l1, WalkQ& b, cons
     b.up = b.up && b.u < len;
     b.down = b.down && b.dn >= 0;
     while (a.up || a.down || b.up || b.down) {
-        const double2 ca = w[a.up ? a.u : 0], da = w[a.down ? a.dn : 0];
-        const double2 cb = w[b.up ? b.u : 0], db = w[b.down ? b.dn : 0];
+        double2 ca = w[a.up ? a.u : 0], da = w[a.down ? a.dn : 0];
+        double2 cb = w[b.up ? b.u : 0], db = w[b.down ? b.dn : 0];
+        pin(ca.x); pin(ca.y); pin(da.x); pin(da.y);
+        pin(cb.x); pin(cb.y); pin(db.x); pin(db.y);
         if (a.up) {
             const double dy = ca.y - a.q.y;
             if (dy < a.T) {
-                l0.insert(fmax(dy, fabs(ca.x - a.q.x)));
+                l0.insert(vmax_abs(dy, ca.x - a.q.x));
                 a.T = l0.top();
                 ++visited;
                 a.up = ++a.u < len;
@@ -850,7 +854,7 @@ __device__ __forceinline__ void run_walk2(L& l0, WalkQ& a, L& l1, WalkQ& b, cons
         if (a.down) {
             const double dy = a.q.y - da.y;
             if (dy < a.T) {
-                l0.insert(fmax(dy, fabs(da.x - a.q.x)));
+                l0.insert(vmax_abs(dy, da.x - a.q.x));
                 a.T = l0.top();
                 ++visited;
                 a.down = --a.dn >= 0;
@@ -861,7 +865,7 @@ __device__ __forceinline__ void run_walk2(L& l0, WalkQ& a, L& l1, WalkQ& b, cons
         if (b.up) {
             const double dy = cb.y - b.q.y;
             if (dy < b.T) {
-                l1.insert(fmax(dy, fabs(cb.x - b.q.x)));
+                l1.insert(vmax_abs(dy, cb.x - b.q.x));
                 b.T = l1.top();
                 ++visited;
                 b.up = ++b.u < len;
@@ -872,7 +876,7 @@ __device__ __forceinline__ void run_walk2(L& l0, WalkQ& a, L& l1, WalkQ& b, cons
         if (b.down) {
             const double dy = b.q.y - db.y;
             if (dy < b.T) {
-                l1.insert(fmax(dy, fabs(db.x - b.q.x)));
+                l1.insert(vmax_abs(dy, db.x - b.q.x));
                 b.T = l1.top();
                 ++visited;
                 b.down = --b.dn >= 0;
