@@ -292,14 +292,27 @@ void psi_tile_kernel(const Item* __restrict__ items, int n_items, const CountDev
     const Item it = items[item];
     const CountDev& cd = descs[it.desc];
     const int lane = threadIdx.x;
+    constexpr int Q = ENM_PSI_TILE / ENM_WAVE;
+    // Every count is read (positions past the tile reread its last one),
+    // then every table entry, each batch issued together; the sum then takes
+    // the tile's positions in order.
+    int c[Q];
+#pragma unroll
+    for (int q = 0; q < Q; ++q) c[q] = counts_in[cd.counts + min(it.t0 + lane + ENM_WAVE * q, it.t1 - 1)];
+#pragma unroll
+    for (int q = 0; q < Q; ++q) pin(c[q]);
+    double pv[Q];
+#pragma unroll
+    for (int q = 0; q < Q; ++q) pv[q] = psi_tab[c[q]];
+#pragma unroll
+    for (int q = 0; q < Q; ++q) pin(pv[q]);
     double ps = 0.0;
     bool zero = false;
 #pragma unroll
-    for (int q = 0; q < ENM_PSI_TILE / ENM_WAVE; ++q) {
+    for (int q = 0; q < Q; ++q) {
         const int i = it.t0 + lane + ENM_WAVE * q;
         if (i < it.t1) {
-            const int c = counts_in[cd.counts + i];
-            if (c > 0) ps += psi_tab[c];
+            if (c[q] > 0) ps += pv[q];
             else zero = true;
         }
     }
@@ -396,8 +409,11 @@ __device__ __forceinline__ int count_sorted(const double* __restrict__ y, int a,
     int l1 = a, n1 = b - a;   // first j with fl(y_j - q) > r
     while (n0 > 0 || n1 > 0) {
         const int h0 = n0 >> 1, h1 = n1 >> 1;
-        const double v0 = n0 > 0 ? y[l0 + h0] : 0.0;
-        const double v1 = n1 > 0 ? y[l1 + h1] : 0.0;
+        // Both reads every round (a finished search rereads a position of
+        // the range), issued together ahead of the branches that use them.
+        double v0 = y[min(l0 + h0, b - 1)], v1 = y[min(l1 + h1, b - 1)];
+        pin(v0);
+        pin(v1);
         if (n0 > 0) {
             if (v0 - q >= -r) n0 = h0;
             else { l0 += h0 + 1; n0 -= h0 + 1; }

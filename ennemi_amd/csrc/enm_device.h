@@ -116,6 +116,7 @@ __device__ __forceinline__ double vmax_abs(double a, double b) {
 // written, with its neighbours, instead of being moved into the branch that
 // first uses it, where it would wait out its own round trip.
 __device__ __forceinline__ void pin(double& v) { asm volatile("" : "+v"(v)); }
+__device__ __forceinline__ void pin(int& v) { asm volatile("" : "+v"(v)); }
 
 // Insert v into the ascending list L[0..P) keeping the P smallest.  Slots
 // below P-K1 hold -inf, so L[P-1] is the K1-th smallest value seen.
