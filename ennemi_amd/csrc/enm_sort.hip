@@ -277,21 +277,26 @@ void strip_sort_kernel(const GatherItem* __restrict__ items, const SortSeg* __re
     const SortSeg sg = segs[it.sseg];
     const OrderDev& od = orders[it.order];
     const int len = sg.len;
-    for (int t = threadIdx.x; t < TILE; t += ENM_SORT_THREADS) {
-        if (t < len) {
-            sk[lp(t)] = col64[sg.src + t];
-            si[lp(t)] = sg.p0 + t;
-        } else {
-            sk[lp(t)] = INFINITY;
-            si[lp(t)] = 0x7fffffff;
+    // A thread's PER elements: every load of a column goes out before any
+    // store (a store between two loads would hold the second one back).
+    constexpr int PER = TILE / ENM_SORT_THREADS;
+    {
+        double kv[PER];
+#pragma unroll
+        for (int k = 0; k < PER; ++k) {
+            const int t = k * ENM_SORT_THREADS + (int)threadIdx.x;
+            kv[k] = t < len ? col64[sg.src + t] : INFINITY;
+        }
+#pragma unroll
+        for (int k = 0; k < PER; ++k) {
+            const int t = k * ENM_SORT_THREADS + (int)threadIdx.x;
+            sk[lp(t)] = kv[k];
+            si[lp(t)] = t < len ? sg.p0 + t : 0x7fffffff;
         }
     }
     block_sort_lds<TILE>(sk, si);
     const int64_t n = od.n;
     const int ncol = od.ncol;
-    // A thread's PER elements: every load of a column goes out before any
-    // store (a store between two loads would hold the second one back).
-    constexpr int PER = TILE / ENM_SORT_THREADS;
     int idx[PER];
     auto in = [&](int k) { return k * ENM_SORT_THREADS + (int)threadIdx.x < len; };
     auto pos = [&](int k) { return it.pos0 + k * ENM_SORT_THREADS + (int)threadIdx.x; };
