@@ -121,7 +121,7 @@ __device__ __forceinline__ int global_left(const double* __restrict__ v, int lo,
 // queue on one address; a full segment makes the query search in place.
 // Counts only: the psi sums of 1-D spaces are taken afterwards per
 // ENM_PSI_TILE positions (psi_tile_kernel), in a fixed order.
-__global__ __launch_bounds__(ENM_C1_THREADS)
+__global__ __launch_bounds__(ENM_C1_THREADS, 8)
 void count1d_kernel(CountArgs A, const double* __restrict__ col64, const double* __restrict__ res64,
                     int* __restrict__ res32, unsigned long long* __restrict__ stats,
                     unsigned* __restrict__ deferred, unsigned* __restrict__ n_deferred, unsigned seg_cap) {
@@ -144,20 +144,48 @@ void count1d_kernel(CountArgs A, const double* __restrict__ col64, const double*
         const int w0 = max(it.lo, it.t0 - ENM_C1_HALO);
         const int w1 = min(it.hi, it.t1 + ENM_C1_HALO);
         const int nw = w1 - w0;
-        for (int t = tid; t < nw; t += ENM_C1_THREADS) win[t + 1] = v[w0 + t];
+        {
+            // All of a thread's window loads go out before its LDS writes.
+            constexpr int WPER = (ENM_C1_CAP + ENM_C1_THREADS - 1) / ENM_C1_THREADS;
+            double wv[WPER];
+#pragma unroll
+            for (int k = 0; k < WPER; ++k) {
+                const int t = tid + k * ENM_C1_THREADS;
+                wv[k] = t < nw ? v[w0 + t] : 0.0;
+            }
+#pragma unroll
+            for (int k = 0; k < WPER; ++k) {
+                const int t = tid + k * ENM_C1_THREADS;
+                if (t < nw) win[t + 1] = wv[k];
+            }
+        }
         if (tid == 0) {
             win[0] = __builtin_nan("");
             win[nw + 1] = __builtin_nan("");
         }
         __syncthreads();
         const double wlo = win[1], whi = win[nw];
-        // All radii first: their dependent gathers (permutation, then
-        // radius) overlap instead of stalling every query in turn.
-        double rq[ENM_C1_Q];
+        // All radii first, each step of their dependent gathers for every
+        // query at once (the permutation entries, then the radii), so the
+        // round trips overlap instead of stalling every query in turn.
+        const bool same = uni((int)cd.same_order) != 0;
+        const int64_t eps_at = cd.eps, eps_perm = cd.eps_perm;
+        int src[ENM_C1_Q];
 #pragma unroll
         for (int q = 0; q < ENM_C1_Q; ++q) {
+            const int i = min(it.t0 + tid + ENM_C1_THREADS * q, it.t1 - 1);
+            src[q] = same ? i : A.perm[eps_perm + i];
+        }
+#pragma unroll
+        for (int q = 0; q < ENM_C1_Q; ++q) pin(src[q]);
+        double rq[ENM_C1_Q];
+#pragma unroll
+        for (int q = 0; q < ENM_C1_Q; ++q) rq[q] = res64[eps_at + src[q]];
+#pragma unroll
+        for (int q = 0; q < ENM_C1_Q; ++q) {
+            pin(rq[q]);
             const int i = it.t0 + tid + ENM_C1_THREADS * q;
-            rq[q] = i < it.t1 ? query_radius(cd, A, res64, i) : -1.0;
+            rq[q] = i < it.t1 ? rq[q] - 1e-12 : -1.0;
         }
         // Both boundaries of all ENM_C1_Q queries are searched in lockstep
         // (halving power-of-two steps from the query's own position, which
