@@ -752,16 +752,32 @@ void knn_walk_kernel(KnnArgs A, const double* __restrict__ col64, double* __rest
 
 constexpr int SK_THREADS_MAX = 512;
 
-// First position in [0, len) of a y-sorted run whose y is >= qy.
+// First position in [0, len) (len >= 1) of a y-sorted run whose y is >= qy,
+// i.e. the number of entries below qy, by a four-way search (for runs in
+// global memory, where each round trip is long): the last entry is tested
+// on its own first, then each round reads three entries a quarter of the
+// remaining range apart, together (y < qy holds on a prefix, so the number
+// of hits is how many quarters to skip).  L rounds cover 4^L positions: a
+// strip of 1024 takes five rounds where a binary search takes ten.
 __device__ __forceinline__ int slot_lower_bound(const double2* __restrict__ w, int len, double qy) {
-    int first = 0, n = len;
-    while (n > 0) {
-        const int half = n >> 1;
-        const bool go = w[first + half].y < qy;
-        first = go ? first + half + 1 : first;
-        n = go ? n - half - 1 : half;
+    const int n = len - 1;
+    double yl = w[n].y;
+    int st = 1;   // 4 * st - 1 >= n: the rounds reach any position
+    while (4 * st < len) st *= 4;
+    int p = 0;
+    for (; st > 0; st >>= 2) {
+        double v[3];
+#pragma unroll
+        for (int m = 0; m < 3; ++m) v[m] = w[max(min(p + (m + 1) * st, n), 1) - 1].y;
+#pragma unroll
+        for (int m = 0; m < 3; ++m) pin(v[m]);
+        pin(yl);   // read with the first round
+        int c = 0;
+#pragma unroll
+        for (int m = 0; m < 3; ++m) c += (p + (m + 1) * st <= n && v[m] < qy) ? 1 : 0;
+        p += c * st;
     }
-    return first;
+    return yl < qy ? len : p;
 }
 
 // The same for two queries at once (fixed trip count, the two chains of
@@ -944,7 +960,26 @@ void knn_strip_kernel(KnnArgs A, const double* __restrict__ col64, double* __res
             const int ss = s - 1 + k;
             slo[k] = lo + ss * S;
             slen[k] = (ss < 0 || ss >= nstrip) ? 0 : min(hi, slo[k] + S) - slo[k];
-            for (int t = threadIdx.x; t < slen[k]; t += NT) sxy[k * A.smax + t] = xy[slo[k] + t];
+        }
+        // Staged two pairs per strip per round, all six loads of a round
+        // issued before its LDS writes.
+        const int most = max(slen[0], max(slen[1], slen[2]));
+        for (int t0 = (int)threadIdx.x; t0 < most; t0 += 2 * NT) {
+            double2 v[3][2];
+#pragma unroll
+            for (int k = 0; k < 3; ++k)
+#pragma unroll
+                for (int h = 0; h < 2; ++h) {
+                    const int t = t0 + h * NT;
+                    v[k][h] = t < slen[k] ? xy[slo[k] + t] : make_double2(0.0, 0.0);
+                }
+#pragma unroll
+            for (int k = 0; k < 3; ++k)
+#pragma unroll
+                for (int h = 0; h < 2; ++h) {
+                    const int t = t0 + h * NT;
+                    if (t < slen[k]) sxy[k * A.smax + t] = v[k][h];
+                }
         }
         // The x bounds of the neighbours: largest x left, smallest x right;
         // and of the strips beyond them.
