@@ -31,19 +31,39 @@ constexpr int PB_ROUNDS = PB_ROWS / PB_THREADS;  // rows per thread
 constexpr int PW_CHUNK = 8192;                   // NumPy's reduction buffer
 constexpr int PW_LEAF = 128;                     // NumPy's pairwise block
 
-// Whether window row r of task t is kept (the mask first, then, with
-// drop_nan, no NaN in x, y or the condition), and whether a row that passes
-// the mask holds a NaN.  x and y are returned for the caller to store; rows
-// at or past the window's end are neither kept nor NaN (row 0 is read
-// instead, so every load stays inside the series).
-__device__ __forceinline__ bool row_state(const PrepTaskDev& t, const double* __restrict__ raw,
-                                          const unsigned char* __restrict__ mask, int drop_nan, int64_t r,
-                                          bool& has_nan, double& xv, double& yv) {
+// Window row r of task t in two steps, so that a thread issues the loads of
+// all its rows before it uses any: row_load reads x, y and the mask byte;
+// row_keep says whether the row is kept (the mask first, then, with
+// drop_nan, no NaN in x, y or the condition) and whether a row that passes
+// the mask holds a NaN.  Rows at or past the window's end are neither kept
+// nor NaN (row 0 is read instead, so every load stays inside the series).
+__device__ __forceinline__ void row_load(const PrepTaskDev& t, const double* __restrict__ raw,
+                                         const unsigned char* __restrict__ mask, int64_t r0,
+                                         double (&xv)[PB_ROUNDS], double (&yv)[PB_ROUNDS], int (&mk)[PB_ROUNDS]) {
+#pragma unroll
+    for (int j = 0; j < PB_ROUNDS; ++j) {
+        const int64_t r = r0 + j * PB_THREADS + threadIdx.x;
+        const int64_t rr = r < t.w ? r : 0;
+        xv[j] = raw[t.x_col + t.x0 + rr];
+        yv[j] = raw[t.y_col + t.y0 + rr];
+    }
+    if (mask != nullptr) {   // uniform: the bytes' loads go out together too
+#pragma unroll
+        for (int j = 0; j < PB_ROUNDS; ++j) {
+            const int64_t r = r0 + j * PB_THREADS + threadIdx.x;
+            mk[j] = mask[t.y0 + (r < t.w ? r : 0)];
+        }
+    } else {
+#pragma unroll
+        for (int j = 0; j < PB_ROUNDS; ++j) mk[j] = 1;
+    }
+}
+
+__device__ __forceinline__ bool row_keep(const PrepTaskDev& t, const double* __restrict__ raw, int drop_nan,
+                                         int64_t r, double xv, double yv, int mk, bool& has_nan) {
     const bool in = r < t.w;
     const int64_t rr = in ? r : 0;
-    xv = raw[t.x_col + t.x0 + rr];
-    yv = raw[t.y_col + t.y0 + rr];
-    const bool pass = in && (mask == nullptr || mask[t.y0 + rr] != 0);
+    const bool pass = in && mk != 0;
     bool nan = isnan(xv) || isnan(yv);
     for (int c = 0; c < t.dz; ++c) nan = nan || isnan(raw[t.z_col[c] + t.z0[c] + rr]);
     has_nan = pass && nan;
@@ -70,11 +90,13 @@ void prep_count_kernel(const PrepTaskDev* __restrict__ tasks, int64_t stride, co
     __syncthreads();
     int cnt = 0;
     bool any_nan = false;
+    double xv[PB_ROUNDS], yv[PB_ROUNDS];
+    int mk[PB_ROUNDS];
+    row_load(t, raw, mask, r0, xv, yv, mk);
 #pragma unroll
     for (int j = 0; j < PB_ROUNDS; ++j) {
         bool nan;
-        double xv, yv;
-        cnt += row_state(t, raw, mask, drop_nan, r0 + j * PB_THREADS + threadIdx.x, nan, xv, yv) ? 1 : 0;
+        cnt += row_keep(t, raw, drop_nan, r0 + j * PB_THREADS + threadIdx.x, xv[j], yv[j], mk[j], nan) ? 1 : 0;
         any_nan = any_nan || nan;
     }
     const int lane = threadIdx.x & (ENM_WAVE - 1), wv = threadIdx.x / ENM_WAVE;
@@ -134,10 +156,12 @@ void prep_compact_kernel(const PrepTaskDev* __restrict__ tasks, int64_t stride,
     bool keep[PB_ROUNDS];
     double xv[PB_ROUNDS], yv[PB_ROUNDS];
     int below[PB_ROUNDS];
+    int mk[PB_ROUNDS];
+    row_load(t, raw, mask, r0, xv, yv, mk);
 #pragma unroll
     for (int j = 0; j < PB_ROUNDS; ++j) {
         bool nan;
-        keep[j] = row_state(t, raw, mask, drop_nan, r0 + j * PB_THREADS + threadIdx.x, nan, xv[j], yv[j]);
+        keep[j] = row_keep(t, raw, drop_nan, r0 + j * PB_THREADS + threadIdx.x, xv[j], yv[j], mk[j], nan);
     }
     unsigned long long wm[PB_ROUNDS];
 #pragma unroll
