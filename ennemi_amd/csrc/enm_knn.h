@@ -1081,8 +1081,10 @@ void knn_strip_kernel(KnnArgs A, const double* __restrict__ col64, double* __res
                 if (w1) res64[kd.eps + i1] = e1;
             }
             if (kd.eps_orig >= 0) {
-                if (w0) res64[kd.eps_orig + A.perm[od.perm + i0]] = e0;
-                if (w1) res64[kd.eps_orig + A.perm[od.perm + i1]] = e1;
+                // Both permutation entries are read before either store.
+                const int o0 = A.perm[od.perm + i0], o1 = A.perm[od.perm + (act1 ? i1 : i0)];
+                if (w0) res64[kd.eps_orig + o0] = e0;
+                if (w1) res64[kd.eps_orig + o1] = e1;
             }
             pairs += visited;
         }
