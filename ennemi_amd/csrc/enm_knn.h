@@ -961,25 +961,16 @@ void knn_strip_kernel(KnnArgs A, const double* __restrict__ col64, double* __res
             slo[k] = lo + ss * S;
             slen[k] = (ss < 0 || ss >= nstrip) ? 0 : min(hi, slo[k] + S) - slo[k];
         }
-        // Staged two pairs per strip per round, all six loads of a round
-        // issued before its LDS writes.
+        // Staged a pair per strip per round, the three loads of a round
+        // issued before its LDS writes (two per strip would spill).
         const int most = max(slen[0], max(slen[1], slen[2]));
-        for (int t0 = (int)threadIdx.x; t0 < most; t0 += 2 * NT) {
-            double2 v[3][2];
+        for (int t = (int)threadIdx.x; t < most; t += NT) {
+            double2 v[3];
+#pragma unroll
+            for (int k = 0; k < 3; ++k) v[k] = t < slen[k] ? xy[slo[k] + t] : make_double2(0.0, 0.0);
 #pragma unroll
             for (int k = 0; k < 3; ++k)
-#pragma unroll
-                for (int h = 0; h < 2; ++h) {
-                    const int t = t0 + h * NT;
-                    v[k][h] = t < slen[k] ? xy[slo[k] + t] : make_double2(0.0, 0.0);
-                }
-#pragma unroll
-            for (int k = 0; k < 3; ++k)
-#pragma unroll
-                for (int h = 0; h < 2; ++h) {
-                    const int t = t0 + h * NT;
-                    if (t < slen[k]) sxy[k * A.smax + t] = v[k][h];
-                }
+                if (t < slen[k]) sxy[k * A.smax + t] = v[k];
         }
         // The x bounds of the neighbours: largest x left, smallest x right;
         // and of the strips beyond them.
